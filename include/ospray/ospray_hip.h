@@ -1,0 +1,47 @@
+/* Extensions of the "hip" device (libospray_module_hip.so).
+ *
+ * These have no counterpart in the reference C API; they expose what the
+ * MI355X device measures so that applications, benchmarks and tests can read
+ * it next to the standard API (the reference offers only ospGetTaskDuration,
+ * render/RenderingFuture.h:99-103).  All functions act on the current device
+ * and return a negative value if it is not a hip device.
+ */
+#pragma once
+
+#include "ospray.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct
+{
+  char name[32]; /* kernel / stage name */
+  int64_t launches;
+  double totalMs; /* sum of hipEvent-timed durations on the device stream */
+} OSPHipKernelStat;
+
+/* Per-stage GPU times accumulated while the device parameter
+ * "profileKernels" (bool) is set; returns the number of entries written. */
+OSPRAY_INTERFACE int ospHipGetKernelStats(OSPHipKernelStat *out, int maxEntries);
+OSPRAY_INTERFACE int ospHipResetKernelStats(void);
+
+/* Node visits, primitive tests and rays of the extension-ray traversals,
+ * counted while the device parameter "traversalStats" (bool) is set. */
+OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out3[3]);
+
+/* Closest hits of n rays (org.xyz, dir.xyz, t0, t1) against a committed
+ * world, traced by the device's BVH kernels.  outTUV: t,u,v; outIDs:
+ * instance index, model index within the group, primitive index
+ * (0xffffffff on miss); stats2 (optional): node visits, primitive tests. */
+OSPRAY_INTERFACE int ospHipTraceRays(
+    OSPWorld world, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2);
+
+/* GPU name, compute-unit count and the BVH statistics of a committed world
+ * (total nodes, total primitives, deepest BLAS). */
+OSPRAY_INTERFACE int ospHipGetDeviceInfo(char *name, int len, int *numCUs);
+OSPRAY_INTERFACE int ospHipGetWorldInfo(OSPWorld world, int64_t out4[4]);
+
+#ifdef __cplusplus
+}
+#endif

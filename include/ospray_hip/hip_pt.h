@@ -1,0 +1,67 @@
+/* Thin C-ABI launch layer of the hip path tracer (libospray_module_hip.so).
+ *
+ * Replaces the reference's device-runtime seam: devicert::Device::
+ * launchRendererKernel / launchFrameOpKernel (ospray/common/DeviceRT.h:170-195)
+ * and the ISPC/SYCL launch templates DEFINE_RENDERER_KERNEL_LAUNCHER /
+ * DEFINE_FRAMEOP_KERNEL_LAUNCHER (modules/cpu/common/DeviceRT.ih:12-115).
+ * Every function enqueues on the given stream, never throws, never
+ * allocates, and returns the hipError_t of the launch.  Arguments are plain
+ * structs of device pointers and sizes whose layout is defined in
+ * ospray_amd/csrc/hip/pt_types.h (HptScene = hpt::DScene,
+ * HptFrame = hpt::DFrame, HptPaths = hpt::DPaths).
+ */
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct HptScene HptScene; /* committed world: BVH, primitives, materials, lights */
+typedef struct HptFrame HptFrame; /* camera, renderer parameters, pass range, fb buffers */
+typedef struct HptPaths HptPaths; /* wavefront path state, queues and counters */
+
+#define HPT_API __attribute__((visibility("default")))
+
+/* Queue counters: [0],[1] extension queues, [2] shadow records, [3] NEE
+ * vertices; all are set to 0 except counter `keepIndex` (-1: none). */
+HPT_API hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s);
+HPT_API hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s);
+
+/* Camera rays of the pass's samples (PathTracer.ispc:75-170). */
+HPT_API hipError_t hpt_generate(const HptFrame *F, const HptPaths *P, hipStream_t s);
+/* Closest hit of every ray in extension queue `queue` (World.ih:247-286). */
+HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int residentBlocks, int stats, hipStream_t s);
+/* One samplePath bounce (PathSampler.ispc:143-407) for queue `queue`. */
+HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
+/* transparentShadow of every NEE record (TransparentShadow.ispc:18-85). */
+HPT_API hipError_t hpt_shadow(const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, hipStream_t s);
+/* Ordered sum of the NEE samples per vertex (NextEventEstimation.ispc:150-159). */
+HPT_API hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s);
+/* ScreenSample_accumulate over the pass (PathTracer.ispc:170-197). */
+HPT_API hipError_t hpt_finish_pass(const HptFrame *F, const HptPaths *P, hipStream_t s);
+/* LocalFB_accumulateSample (fb/LocalFB.ispc:14-78). */
+HPT_API hipError_t hpt_fb_accumulate(const HptFrame *F, hipStream_t s);
+/* Variance_kernel (fb/frame_ops/Variance.ispc:11-59). */
+HPT_API hipError_t hpt_variance(const HptFrame *F, hipStream_t s);
+/* getAvgError (fb/frame_ops/Variance.cpp:32-46), FP64 sum, one float out. */
+HPT_API hipError_t hpt_variance_reduce(const float *taskVariance, int n, float threshold, float *out, hipStream_t s);
+/* ColorConversion_kernel (fb/frame_ops/ColorConversion.ispc:11-36);
+ * format is an OSPFrameBufferFormat. */
+HPT_API hipError_t hpt_color_convert(const float *color, int npix, int format, void *out, hipStream_t s);
+/* Closest hits of an explicit ray list: rays8 = org.xyz, dir.xyz, t0, t1;
+ * outTUV = t,u,v; outIDs = instance, model, primitive (0xffffffff on miss);
+ * stats (optional) += node visits, primitive tests. */
+HPT_API hipError_t hpt_trace_rays(const HptScene *S, const float *rays8, int n, float *outTUV, uint32_t *outIDs,
+    uint32_t *overflow, int residentBlocks, unsigned long long *stats, hipStream_t s);
+
+HPT_API int hpt_trace_block_size(void);
+HPT_API int hpt_overflow_depth(void);
+HPT_API int hpt_stack_capacity(void);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,304 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of oracle/liborc.so, the CPU restatement of the reference
+path tracer (see oracle/refcpu/*.h for the reference files each part
+follows).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg use it, as the checker; the product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liborc.so")
+
+ORC_GEOM = {"triangles": 0, "quads": 1, "spheres": 2}
+ORC_MAT = {"obj": 0, "metal": 1, "glass": 2}
+ORC_LIGHT = {"quad": 0, "distant": 1, "ambient": 2}
+
+
+class OrcGeometry(C.Structure):
+    _fields_ = [("type", C.c_int32), ("vertex", C.c_void_p), ("numVertices", C.c_int64), ("index", C.c_void_p),
+                ("numPrimitives", C.c_int64), ("normal", C.c_void_p), ("spheres", C.c_void_p)]
+
+
+class OrcModel(C.Structure):
+    _fields_ = [("geometry", C.c_int32), ("material", C.c_int32), ("userID", C.c_uint32), ("invertNormals", C.c_int32)]
+
+
+class OrcGroup(C.Structure):
+    _fields_ = [("firstModel", C.c_int32), ("numModels", C.c_int32)]
+
+
+class OrcInstance(C.Structure):
+    _fields_ = [("group", C.c_int32), ("userID", C.c_uint32), ("xfm", C.c_float * 12)]
+
+
+class OrcMaterial(C.Structure):
+    _fields_ = [("type", C.c_int32), ("kd", C.c_float * 3), ("d", C.c_float), ("tf", C.c_float * 3),
+                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("roughness", C.c_float), ("etaInside", C.c_float),
+                ("etaOutside", C.c_float), ("attenuationColorInside", C.c_float * 3),
+                ("attenuationColorOutside", C.c_float * 3), ("attenuationDistance", C.c_float)]
+
+
+class OrcLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("visible", C.c_int32), ("color", C.c_float * 3), ("intensity", C.c_float),
+                ("intensityQuantity", C.c_int32), ("position", C.c_float * 3), ("edge1", C.c_float * 3),
+                ("edge2", C.c_float * 3), ("direction", C.c_float * 3), ("angularDiameter", C.c_float)]
+
+
+class OrcScene(C.Structure):
+    _fields_ = [("geometries", C.POINTER(OrcGeometry)), ("numGeometries", C.c_int32),
+                ("models", C.POINTER(OrcModel)), ("numModels", C.c_int32),
+                ("groupModels", C.POINTER(C.c_int32)), ("groups", C.POINTER(OrcGroup)), ("numGroups", C.c_int32),
+                ("instances", C.POINTER(OrcInstance)), ("numInstances", C.c_int32),
+                ("materials", C.POINTER(OrcMaterial)), ("numMaterials", C.c_int32),
+                ("lights", C.POINTER(OrcLight)), ("numLights", C.c_int32)]
+
+
+class OrcCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy", C.c_float), ("aspect", C.c_float), ("nearClip", C.c_float)]
+
+
+class OrcRenderer(C.Structure):
+    _fields_ = [("pixelSamples", C.c_int32), ("maxPathLength", C.c_int32), ("roulettePathLength", C.c_int32),
+                ("maxScatteringEvents", C.c_int32), ("lightSamples", C.c_int32),
+                ("limitIndirectLightSamples", C.c_int32), ("minContribution", C.c_float),
+                ("maxContribution", C.c_float), ("backgroundColor", C.c_float * 4), ("pixelFilter", C.c_int32),
+                ("varianceThreshold", C.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = C.CDLL(LIBPATH)
+        L.orc_scene_create.restype = C.c_void_p
+        L.orc_scene_create.argtypes = [C.POINTER(OrcScene)]
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_fb_create.restype = C.c_void_p
+        L.orc_fb_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32]
+        L.orc_fb_destroy.argtypes = [C.c_void_p]
+        L.orc_fb_reset.argtypes = [C.c_void_p]
+        L.orc_fb_set_projected_depth.argtypes = [C.c_void_p, C.c_int]
+        L.orc_render_frame.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrcCamera), C.POINTER(OrcRenderer),
+                                       C.c_int, C.c_int, C.c_int]
+        L.orc_fb_variance.restype = C.c_float
+        L.orc_fb_variance.argtypes = [C.c_void_p, C.POINTER(OrcRenderer)]
+        L.orc_fb_map.restype = C.c_void_p
+        L.orc_fb_map.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_fb_accum_color.restype = C.c_void_p
+        L.orc_fb_accum_color.argtypes = [C.c_void_p]
+        L.orc_fb_task_variance.restype = C.c_void_p
+        L.orc_fb_task_variance.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.orc_sobol_rev_table.argtypes = [C.c_void_p]
+        L.orc_hash.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_pcg_raw.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p, C.c_int]
+        L.orc_pcg_init_sequence.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_int]
+        L.orc_ld_samples5.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_fmath.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_pixel_filter.argtypes = [C.POINTER(OrcRenderer), C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_camera_rays.argtypes = [C.POINTER(OrcCamera), C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.POINTER(C.c_uint64)]
+        _lib = L
+    return _lib
+
+
+def _f3(v, default=(0.0, 0.0, 0.0)):
+    v = default if v is None else v
+    return (C.c_float * 3)(*[float(np.float32(x)) for x in v])
+
+
+def make_camera(cam: dict, near_clip: float = 1e-6) -> OrcCamera:
+    c = OrcCamera()
+    c.position = _f3(cam.get("position"))
+    c.direction = _f3(cam.get("direction"), (0, 0, 1))
+    c.up = _f3(cam.get("up"), (0, 1, 0))
+    c.fovy = cam.get("fovy", 60.0)
+    c.aspect = cam.get("aspect", 1.0)
+    c.nearClip = cam.get("nearClip", near_clip)
+    return c
+
+
+def make_renderer(p: dict) -> OrcRenderer:
+    """Renderer parameters with the reference defaults (Renderer.cpp:38-92,
+    PathTracer.cpp:29-51)."""
+    r = OrcRenderer()
+    r.pixelSamples = int(p.get("pixelSamples", 1))
+    r.maxPathLength = int(p.get("maxPathLength", 20))
+    r.roulettePathLength = int(p.get("roulettePathLength", 5))
+    r.maxScatteringEvents = int(p.get("maxScatteringEvents", 20))
+    r.lightSamples = int(p.get("lightSamples", -1))
+    r.limitIndirectLightSamples = int(bool(p.get("limitIndirectLightSamples", True)))
+    r.minContribution = float(p.get("minContribution", 0.001))
+    r.maxContribution = float(p.get("maxContribution", float("inf")))
+    bg = p.get("backgroundColor", (0.0, 0.0, 0.0, 0.0))
+    if isinstance(bg, (int, float)):
+        bg = (bg, bg, bg, 0.0)
+    if len(bg) == 3:
+        bg = tuple(bg) + (0.0,)
+    r.backgroundColor = (C.c_float * 4)(*bg)
+    pf = p.get("pixelFilter", 2)
+    r.pixelFilter = int(getattr(pf, "v", pf))
+    r.varianceThreshold = float(p.get("varianceThreshold", 0.0))
+    return r
+
+
+class OracleScene:
+    """The Scene packed for liborc.so, with its BVHs built."""
+
+    def __init__(self, scene):
+        L = lib()
+        self._keep = []
+        geos = (OrcGeometry * len(scene.geometries))()
+        for i, g in enumerate(scene.geometries):
+            geos[i].type = ORC_GEOM[g.type]
+            if g.type == "spheres":
+                sp = np.ascontiguousarray(g.spheres, np.float32)
+                self._keep.append(sp)
+                geos[i].spheres = sp.ctypes.data
+                geos[i].numPrimitives = sp.shape[0]
+            else:
+                v = np.ascontiguousarray(g.vertex, np.float32)
+                ix = np.ascontiguousarray(g.index, np.uint32)
+                self._keep += [v, ix]
+                geos[i].vertex = v.ctypes.data
+                geos[i].numVertices = v.shape[0]
+                geos[i].index = ix.ctypes.data
+                geos[i].numPrimitives = ix.shape[0]
+                if g.normal is not None:
+                    nrm = np.ascontiguousarray(g.normal, np.float32)
+                    self._keep.append(nrm)
+                    geos[i].normal = nrm.ctypes.data
+        models = (OrcModel * len(scene.models))()
+        for i, m in enumerate(scene.models):
+            models[i].geometry = m.geometry
+            models[i].material = m.material
+            models[i].userID = 0xFFFFFFFF if m.user_id is None else m.user_id
+        gm, groups = [], (OrcGroup * len(scene.groups))()
+        for i, grp in enumerate(scene.groups):
+            groups[i].firstModel = len(gm)
+            groups[i].numModels = len(grp)
+            gm += list(grp)
+        gmodels = (C.c_int32 * max(1, len(gm)))(*gm)
+        insts = (OrcInstance * len(scene.instances))()
+        for i, inst in enumerate(scene.instances):
+            insts[i].group = inst.group
+            insts[i].userID = 0xFFFFFFFF if inst.user_id is None else inst.user_id
+            insts[i].xfm = (C.c_float * 12)(*[float(np.float32(x)) for x in inst.xfm])
+        mats = (OrcMaterial * max(1, len(scene.materials)))()
+        for i, m in enumerate(scene.materials):
+            t = m["type"]
+            mats[i].type = ORC_MAT[t]
+            mats[i].kd = _f3(m.get("kd"), (0.8, 0.8, 0.8))
+            mats[i].d = m.get("d", 1.0)
+            mats[i].tf = _f3(m.get("tf"))
+            mats[i].eta = _f3(m.get("eta") if t == "metal" else None, (1.47, 0.984, 0.602))
+            mats[i].k = _f3(m.get("k"), (7.64, 6.55, 5.36))
+            mats[i].roughness = m.get("roughness", 0.1)
+            eta = m.get("eta", 1.5) if t == "glass" else 1.5
+            mats[i].etaInside = m.get("etaInside", eta)
+            mats[i].etaOutside = m.get("etaOutside", 1.0)
+            ac = m.get("attenuationColor", (1.0, 1.0, 1.0))
+            mats[i].attenuationColorInside = _f3(m.get("attenuationColorInside", ac))
+            mats[i].attenuationColorOutside = _f3(m.get("attenuationColorOutside", (1.0, 1.0, 1.0)))
+            mats[i].attenuationDistance = m.get("attenuationDistance", 1.0)
+        lights = (OrcLight * max(1, len(scene.lights)))()
+        for i, l in enumerate(scene.lights):
+            lights[i].type = ORC_LIGHT[l["type"]]
+            lights[i].visible = int(l.get("visible", True))
+            lights[i].color = _f3(l.get("color"), (1, 1, 1))
+            lights[i].intensity = l.get("intensity", 1.0)
+            iq = l.get("intensityQuantity", -1)
+            lights[i].intensityQuantity = int(getattr(iq, "v", iq))
+            lights[i].position = _f3(l.get("position"))
+            lights[i].edge1 = _f3(l.get("edge1"), (1, 0, 0))
+            lights[i].edge2 = _f3(l.get("edge2"), (0, 1, 0))
+            lights[i].direction = _f3(l.get("direction"), (0, 0, 1))
+            lights[i].angularDiameter = l.get("angularDiameter", 0.0)
+        self._keep += [geos, models, groups, gmodels, insts, mats, lights]
+        desc = OrcScene(geos, len(scene.geometries), models, len(scene.models), gmodels, groups, len(scene.groups),
+                        insts, len(scene.instances), mats, len(scene.materials), lights, len(scene.lights))
+        self._desc = desc
+        self.h = L.orc_scene_create(C.byref(desc))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_scene_destroy(self.h)
+            self.h = None
+
+    def intersect(self, rays8: np.ndarray, fp64: bool = False):
+        L = lib()
+        r = np.ascontiguousarray(rays8, np.float32)
+        n = r.shape[0]
+        tuv = np.empty((n, 3), np.float32)
+        ids = np.empty((n, 3), np.uint32)
+        st = (C.c_uint64 * 2)()
+        L.orc_intersect(self.h, r.ctypes.data, n, int(fp64), tuv.ctypes.data, ids.ctypes.data, st)
+        return tuv, ids, (int(st[0]), int(st[1]))
+
+
+class OracleFrameBuffer:
+    def __init__(self, w: int, h: int, fmt: int, channels: int):
+        self.w, self.h, self.fmt = w, h, fmt
+        self.ptr = lib().orc_fb_create(w, h, fmt, channels)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.orc_fb_destroy(self.ptr)
+            self.ptr = None
+
+    def render(self, scene: OracleScene, camera: OrcCamera, renderer: OrcRenderer, nthreads: int = 8,
+               rows: tuple = (0, -1)):
+        lib().orc_render_frame(scene.h, self.ptr, C.byref(camera), C.byref(renderer), nthreads, rows[0], rows[1])
+
+    def variance(self, renderer: OrcRenderer) -> float:
+        return float(lib().orc_fb_variance(self.ptr, C.byref(renderer)))
+
+    def reset(self):
+        lib().orc_fb_reset(self.ptr)
+
+    def channel(self, ch: int) -> np.ndarray:
+        p = lib().orc_fb_map(self.ptr, ch)
+        if not p:
+            raise ValueError(f"channel {ch} not present")
+        n = self.w * self.h
+        if ch == 1:
+            if self.fmt == 3:
+                return np.ctypeslib.as_array((C.c_float * (4 * n)).from_address(p)).copy().reshape(self.h, self.w, 4)
+            return np.ctypeslib.as_array((C.c_uint8 * (4 * n)).from_address(p)).copy().reshape(self.h, self.w, 4)
+        if ch == 2:
+            return np.ctypeslib.as_array((C.c_float * n).from_address(p)).copy().reshape(self.h, self.w)
+        if ch in (64, 128, 256):
+            return np.ctypeslib.as_array((C.c_uint32 * n).from_address(p)).copy().reshape(self.h, self.w)
+        return np.ctypeslib.as_array((C.c_float * (3 * n)).from_address(p)).copy().reshape(self.h, self.w, 3)
+
+    def accum_color(self) -> np.ndarray:
+        p = lib().orc_fb_accum_color(self.ptr)
+        n = self.w * self.h
+        return np.ctypeslib.as_array((C.c_float * (4 * n)).from_address(p)).copy().reshape(self.h, self.w, 4)
+
+
+def render_scene(scene, frames: int = 1, fmt: int = 3, channels: int = 1 | 4, nthreads: int = 8,
+                 renderer_params: dict | None = None, oscene: OracleScene | None = None, rows=(0, -1)):
+    """Render `frames` accumulated frames of a Scene with the oracle."""
+    os_ = oscene or OracleScene(scene)
+    fb = OracleFrameBuffer(scene.width, scene.height, fmt, channels)
+    rp = dict(scene.renderer)
+    if renderer_params:
+        rp.update(renderer_params)
+    cam = make_camera(scene.camera)
+    ren = make_renderer(rp)
+    for _ in range(frames):
+        fb.render(os_, cam, ren, nthreads, rows)
+    return fb, os_, ren

@@ -1,0 +1,231 @@
+// ORACLE — test infrastructure only.
+//
+// C ABI of liborc.so, loaded with ctypes by tests/ and by bench.py's
+// cpu_baseline leg.  Nothing in the product library links or loads it.
+#include <cstring>
+#include <memory>
+
+#include "orc_render.h"
+
+using namespace orc;
+
+namespace {
+struct OrcSceneHandle
+{
+  Scene scene;
+};
+struct OrcFbHandle
+{
+  FrameBuffer fb;
+};
+} // namespace
+
+extern "C" {
+
+void *orc_scene_create(const OrcScene *desc)
+{
+  auto *h = new OrcSceneHandle;
+  setupScene(h->scene, *desc);
+  return h;
+}
+
+void orc_scene_destroy(void *h) { delete (OrcSceneHandle *)h; }
+
+void *orc_fb_create(int w, int h, int format, uint32_t channels)
+{
+  auto *f = new OrcFbHandle;
+  initFrameBuffer(f->fb, w, h, format, channels);
+  return f;
+}
+
+void orc_fb_destroy(void *h) { delete (OrcFbHandle *)h; }
+
+void orc_fb_reset(void *h) { ((OrcFbHandle *)h)->fb.clear(); }
+
+void orc_fb_set_projected_depth(void *h, int on) { ((OrcFbHandle *)h)->fb.projectedDepth = on != 0; }
+
+int orc_fb_frame_id(void *h) { return ((OrcFbHandle *)h)->fb.frameID; }
+
+void orc_render_frame(void *scene, void *fb, const OrcCamera *cam, const OrcRenderer *ren, int nthreads, int y0, int y1)
+{
+  CameraData c;
+  setupCamera(c, *cam);
+  RendererData r;
+  setupRenderer(r, *ren);
+  FrameBuffer &f = ((OrcFbHandle *)fb)->fb;
+  if (y1 < 0 || y1 > f.h)
+    y1 = f.h;
+  renderFrame(((OrcSceneHandle *)scene)->scene, f, c, r, nthreads, y0, y1);
+}
+
+float orc_fb_variance(void *fb, const OrcRenderer *ren)
+{
+  RendererData r;
+  setupRenderer(r, *ren);
+  return getVariance(((OrcFbHandle *)fb)->fb, r);
+}
+
+// pointer to a channel buffer (OSPFrameBufferChannel value); colour returns
+// the converted buffer in the framebuffer format
+const void *orc_fb_map(void *fb, uint32_t channel)
+{
+  FrameBuffer &f = ((OrcFbHandle *)fb)->fb;
+  switch (channel) {
+  case 1:
+    return f.mapped.empty() ? nullptr : (const void *)f.mapped.data();
+  case 2:
+    return f.depth.empty() ? nullptr : (const void *)f.depth.data();
+  case 16:
+    return f.normal.empty() ? nullptr : (const void *)f.normal.data();
+  case 32:
+    return f.albedo.empty() ? nullptr : (const void *)f.albedo.data();
+  case 64:
+    return f.primID.empty() ? nullptr : (const void *)f.primID.data();
+  case 128:
+    return f.objID.empty() ? nullptr : (const void *)f.objID.data();
+  case 256:
+    return f.instID.empty() ? nullptr : (const void *)f.instID.data();
+  case 512:
+    return f.firstNormal.empty() ? nullptr : (const void *)f.firstNormal.data();
+  case 1024:
+    return f.position.empty() ? nullptr : (const void *)f.position.data();
+  default:
+    return nullptr;
+  }
+}
+
+// raw accumulated RGBA32F colour (before format conversion)
+const float *orc_fb_accum_color(void *fb)
+{
+  FrameBuffer &f = ((OrcFbHandle *)fb)->fb;
+  return f.color.empty() ? nullptr : &f.color[0].x;
+}
+
+const float *orc_fb_task_variance(void *fb, int *n)
+{
+  FrameBuffer &f = ((OrcFbHandle *)fb)->fb;
+  *n = (int)f.taskVariance.size();
+  return f.taskVariance.data();
+}
+
+// ---- unit-level hooks used by the golden / property tests ------------------
+
+void orc_sobol_rev_table(uint32_t *out128) { std::memcpy(out128, sobolTable().rev, sizeof(uint32_t) * 128); }
+
+// kind: 0 MurmurHash3_mix(x, seed), 1 MurmurHash3_finalize(x), 2 tripleHash,
+// 3 OwenHash2(x, seed), 4 OwenScramble4(x, seed), 5 reverseBits,
+// 6 interleaveZOrder16(x, seed)
+void orc_hash(int kind, const uint32_t *x, const uint32_t *seed, uint32_t *out, int64_t n)
+{
+  for (int64_t i = 0; i < n; i++) {
+    const uint32_t a = x[i], s = seed ? seed[i] : 0;
+    switch (kind) {
+    case 0: out[i] = MurmurHash3_mix(a, s); break;
+    case 1: out[i] = MurmurHash3_finalize(a); break;
+    case 2: out[i] = tripleHash(a); break;
+    case 3: out[i] = OwenHash2(a, s); break;
+    case 4: out[i] = OwenScramble4(a, s); break;
+    case 5: out[i] = reverseBits(a); break;
+    default: out[i] = interleaveZOrder16(a, s); break;
+    }
+  }
+}
+
+// raw PCG32 steps from (state, increment-stream) without the hashing init
+void orc_pcg_raw(uint64_t state, uint32_t stream, uint32_t *out, int n)
+{
+  RandomSampler r{state, stream};
+  for (int i = 0; i < n; i++)
+    out[i] = pcg32(r);
+}
+
+void orc_pcg_init_sequence(uint32_t seed, uint32_t stream, uint32_t *out, int n)
+{
+  RandomSampler r;
+  RandomSampler_init(r, seed, stream);
+  for (int i = 0; i < n; i++)
+    out[i] = pcg32(r);
+}
+
+// first five LD dimensions of a pixel sample: pixel (2), lens (2), time (1)
+void orc_ld_samples5(uint32_t seed, uint32_t sampleIndex, uint32_t bits, float *out5)
+{
+  LDSampler ld;
+  LDSampler_init(ld, seed, sampleIndex, 0, bits, bits);
+  vec3ui ss;
+  const vec2f p = LDSampler_getNext5Samples(ld, ss);
+  const vec2f lens = LDSampler_finalizeDim23(ld, ss);
+  out5[0] = p.x;
+  out5[1] = p.y;
+  out5[2] = lens.x;
+  out5[3] = lens.y;
+  out5[4] = LDSampler_finalizeDim4(ld, ss);
+}
+
+// kind: 0 sin, 1 cos, 2 acos, 3 exp, 4 log
+void orc_fmath(int kind, const float *in, float *out, int64_t n)
+{
+  for (int64_t i = 0; i < n; i++) {
+    float s, c;
+    switch (kind) {
+    case 0: fsincos(in[i], s, c); out[i] = s; break;
+    case 1: fsincos(in[i], s, c); out[i] = c; break;
+    case 2: out[i] = facos(in[i]); break;
+    case 3: out[i] = fexp(in[i]); break;
+    default: out[i] = flog(in[i]); break;
+    }
+  }
+}
+
+void orc_pixel_filter(const OrcRenderer *ren, const float *s2, float *out2, int64_t n)
+{
+  RendererData r;
+  setupRenderer(r, *ren);
+  for (int64_t i = 0; i < n; i++) {
+    const vec2f o = pixelFilterSample(r, v2(s2[2 * i], s2[2 * i + 1]));
+    out2[2 * i] = o.x;
+    out2[2 * i + 1] = o.y;
+  }
+}
+
+void orc_camera_rays(const OrcCamera *cam, const float *screen2, float *out6, int64_t n)
+{
+  CameraData c;
+  setupCamera(c, *cam);
+  for (int64_t i = 0; i < n; i++) {
+    Ray r;
+    cameraRay(c, v2(screen2[2 * i], screen2[2 * i + 1]), r);
+    out6[6 * i + 0] = r.org.x;
+    out6[6 * i + 1] = r.org.y;
+    out6[6 * i + 2] = r.org.z;
+    out6[6 * i + 3] = r.dir.x;
+    out6[6 * i + 4] = r.dir.y;
+    out6[6 * i + 5] = r.dir.z;
+  }
+}
+
+// closest hits for rays given as (org3, dir3, t0, t1); out: t,u,v (floats)
+// and inst, model, prim (uint32, 0xffffffff on miss).  fp64 != 0 uses the
+// double-precision brute force instead of the BVH.
+void orc_intersect(void *scene, const float *rays8, int64_t n, int fp64, float *outTUV, uint32_t *outIDs, uint64_t *stats2)
+{
+  const World &w = ((OrcSceneHandle *)scene)->scene.world;
+  uint64_t nodes = 0, prims = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const float *r = rays8 + 8 * i;
+    Ray ray{v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7]};
+    const Hit h = fp64 ? intersectBruteForceFP64(w, ray) : w.intersect(ray, &nodes, &prims);
+    outTUV[3 * i] = h.hit ? h.t : kInf;
+    outTUV[3 * i + 1] = h.u;
+    outTUV[3 * i + 2] = h.v;
+    outIDs[3 * i] = h.hit ? h.inst : kInvalidID;
+    outIDs[3 * i + 1] = h.hit ? h.model : kInvalidID;
+    outIDs[3 * i + 2] = h.hit ? h.prim : kInvalidID;
+  }
+  if (stats2) {
+    stats2[0] = nodes;
+    stats2[1] = prims;
+  }
+}
+
+} // extern "C"

@@ -1,0 +1,132 @@
+// ORACLE — test infrastructure only.
+//
+// Restatement of the reference `pathtracer` renderer on the CPU:
+//   per pixel      modules/cpu/render/pathtracer/PathTracer.ispc:75-216
+//   bounce loop    modules/cpu/render/pathtracer/PathSampler.ispc:109-415
+//   NEE            modules/cpu/render/pathtracer/NextEventEstimation.ispc:25-160
+//   shadows        modules/cpu/render/pathtracer/TransparentShadow.ispc:18-85
+//   virtual lights modules/cpu/render/pathtracer/VirtualLight.ispc:17-107
+//   accumulation   modules/cpu/fb/LocalFB.ispc:14-78, fb/FrameBuffer.cpp:115-137
+//   variance       modules/cpu/fb/frame_ops/Variance.ispc:11-59, Variance.cpp:32-46
+//   colour formats modules/cpu/fb/frame_ops/ColorConversion.ispc:11-36
+#pragma once
+
+#include <cstdint>
+#include <random>
+#include <vector>
+
+#include "orc_geom.h"
+#include "orc_sampler.h"
+
+namespace orc {
+
+static constexpr float kUlpEpsilon = 0.000000504636764526367187500f; // 0x1.0fp-21f
+
+struct DG
+{
+  vec3f P, Ng, Ns, dPds, dPdt;
+  vec4f color;
+  float epsilon;
+  int material; // -1 = none
+  uint32_t primID, objID, instID;
+};
+
+struct Medium
+{
+  vec3f attenuation;
+  float ior;
+};
+inline bool eq(const Medium &a, const Medium &b) { return a.ior == b.ior && eq(a.attenuation, b.attenuation); }
+
+struct MaterialData
+{
+  int type;
+  // obj
+  vec3f Kd, Tf;
+  float d;
+  // metal
+  vec3f eta, k;
+  float roughness;
+  // glass
+  Medium inside, outside;
+};
+
+struct LightData
+{
+  int type;
+  bool visible;
+  // quad
+  vec3f radiance, position, edge1, edge2, nnormal;
+  float ppdf;
+  // distant
+  linear3f frame;
+  vec3f irradiance;
+  float cosAngle, pdf;
+};
+
+struct CameraData
+{
+  vec3f org, dir_00, du_size, dv_up, direction;
+  float nearClip;
+};
+
+struct RendererData
+{
+  int spp, maxDepth, rouletteDepth, maxScatteringEvents, numLightSamples;
+  bool limitIndirectLightSamples;
+  float minContribution, maxRadiance;
+  vec4f bgColor;
+  int filterType; // 0 point, 1 box, 2 LUT
+  float filterWidth;
+  int lutSize;
+  std::vector<float> lutCdfX, lutCdfY;
+  float errorThreshold;
+};
+
+struct Scene
+{
+  World world;
+  std::vector<MaterialData> materials;
+  std::vector<LightData> lights;
+  std::vector<float> lightsCDF;
+};
+
+struct FrameBuffer
+{
+  int w, h;
+  uint32_t channels;
+  int format;
+  int frameID = -1;
+  bool accumulateVariance = false;
+  std::mt19937 mtGen{43};
+  bool pickOdd = false;
+  int targetFrames = 0;
+  bool projectedDepth = false;
+  std::vector<vec4f> color, variance;
+  std::vector<float> depth;
+  std::vector<vec3f> normal, firstNormal, albedo, position;
+  std::vector<uint32_t> primID, objID, instID;
+  std::vector<float> taskVariance;
+  bool varianceValid = false;
+  std::vector<uint8_t> mapped; // converted colour for OSP_FB_RGBA8 / SRGBA / RGBA32F
+  void clear();
+};
+
+void setupScene(Scene &s, const OrcScene &desc);
+void setupCamera(CameraData &c, const OrcCamera &desc);
+void setupRenderer(RendererData &r, const OrcRenderer &desc);
+void initFrameBuffer(FrameBuffer &fb, int w, int h, int format, uint32_t channels);
+
+// one ospRenderFrame: beginFrame, render rows [y0, y1) (all rows: 0, h),
+// then postProcess (variance, colour conversion)
+void renderFrame(const Scene &s, FrameBuffer &fb, const CameraData &cam, const RendererData &r, int nthreads, int y0, int y1);
+float getVariance(const FrameBuffer &fb, const RendererData &r);
+
+// pixel filter sample and camera ray of one sample (test hooks)
+vec2f pixelFilterSample(const RendererData &r, vec2f s);
+void cameraRay(const CameraData &c, vec2f screen, Ray &ray);
+
+// portable transcendental float functions
+float hostExp(float x);
+
+} // namespace orc

@@ -1,0 +1,1696 @@
+// The "hip" OSPRay device: object types, scene assembly, framebuffer state
+// and frame orchestration on one HIP stream.
+//
+// Mirrors the reference CPU device (modules/cpu/ISPCDevice.cpp:154-658) and
+// its object commits (Mesh.cpp:37-204, Spheres.cpp:46-106,
+// GeometricModel.cpp:21-87, Group.cpp:69-209, Instance.cpp:20-68,
+// World.cpp:70-188, OBJ/Metal/Glass.cpp, QuadLight/DirectionalLight/
+// AmbientLight.cpp, PerspectiveCamera.cpp:30-110, Renderer.cpp:38-178,
+// PathTracer.cpp:29-107, FrameBuffer.cpp:31-193, LocalFB.cpp:49-528).
+// Rendering is the wavefront pipeline of pt_kernels.hip, launched through
+// the C-ABI layer of include/ospray_hip/hip_pt.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "../core/core.h"
+#include "bvh_build.h"
+#include "ospray/ospray_hip.h"
+#include "ospray_hip/hip_pt.h"
+#include "pt_sampler.h"
+#include "pt_types.h"
+
+using namespace ospcore;
+
+namespace hipdev {
+
+using hpt::vec3f;
+using hpt::v3;
+
+#define HIP_CHECK(expr)                                                                                            \
+  do {                                                                                                             \
+    const hipError_t e_ = (expr);                                                                                  \
+    if (e_ != hipSuccess)                                                                                          \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr);                 \
+  } while (0)
+
+struct DevBuf
+{
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (ptr)
+      (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n)
+  {
+    if (n <= bytes && ptr)
+      return;
+    release();
+    if (n == 0)
+      return;
+    HIP_CHECK(hipMalloc(&ptr, n));
+    bytes = n;
+  }
+  template <typename T>
+  void upload(const std::vector<T> &v)
+  {
+    ensure(std::max<size_t>(sizeof(T) * v.size(), 16));
+    if (!v.empty())
+      HIP_CHECK(hipMemcpy(ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  }
+  template <typename T>
+  T *as() const
+  {
+    return (T *)ptr;
+  }
+};
+
+class HipDevice;
+
+// ---- data helpers -------------------------------------------------------------------
+
+static Data *getData(ManagedObject *o, const char *name)
+{
+  return (Data *)o->getObject(name, OSP_DATA);
+}
+
+static std::vector<float> dataFloats(Data *d, int comps, const char *what)
+{
+  const size_t es = sizeOfDataType(d->elemType);
+  if (es != sizeof(float) * comps || (d->elemType != OSP_VEC3F && d->elemType != OSP_FLOAT && d->elemType != OSP_VEC4F))
+    throw std::runtime_error(std::string("unsupported element type for ") + what);
+  std::vector<float> out(d->size() * comps);
+  for (uint64_t i = 0; i < d->size(); i++)
+    std::memcpy(&out[i * comps], d->item(i), es);
+  return out;
+}
+
+template <typename T>
+static std::vector<T *> dataObjects(Data *d, OSPDataType type)
+{
+  std::vector<T *> out;
+  if (!d)
+    return out;
+  if (!isObjectType(d->elemType))
+    throw std::runtime_error("expected an array of objects");
+  for (uint64_t i = 0; i < d->size(); i++) {
+    ManagedObject *o = d->get<ManagedObject *>(i);
+    if (o && o->managedType != type)
+      throw std::runtime_error("array holds objects of the wrong type");
+    out.push_back((T *)o);
+  }
+  return out;
+}
+
+// holds a reference for as long as it lives
+template <typename T>
+struct Ref
+{
+  T *p = nullptr;
+  Ref() = default;
+  Ref(const Ref &) = delete;
+  Ref &operator=(const Ref &) = delete;
+  ~Ref() { reset(nullptr); }
+  void reset(T *n)
+  {
+    if (n)
+      n->refInc();
+    if (p)
+      p->refDec();
+    p = n;
+  }
+  T *operator->() const { return p; }
+  explicit operator bool() const { return p != nullptr; }
+};
+
+// ---- scene objects ---------------------------------------------------------------------
+
+struct Geometry : public ManagedObject
+{
+  int geomType = -1; // GeomType
+  std::vector<float> vertex, normal, spheres;
+  std::vector<uint32_t> index;
+  size_t numPrims = 0;
+  DevBuf dVertex, dIndex, dNormal, dSpheres;
+  uint64_t version = 0;
+
+  Geometry(const std::string &t)
+  {
+    managedType = OSP_GEOMETRY;
+    subtype = t;
+  }
+  std::string toString() const override { return "ospray::Geometry(" + subtype + ")"; }
+
+  void commit() override
+  {
+    vertex.clear();
+    normal.clear();
+    spheres.clear();
+    index.clear();
+    if (subtype == "mesh") { // Mesh.cpp:37-204
+      Data *pos = getData(this, "vertex.position");
+      Data *idx = getData(this, "index");
+      if (!pos)
+        throw std::runtime_error("mesh geometry requires 'vertex.position'");
+      vertex = dataFloats(pos, 3, "vertex.position");
+      if (Data *nrm = getData(this, "vertex.normal")) {
+        normal = dataFloats(nrm, 3, "vertex.normal");
+        if (normal.size() != vertex.size())
+          throw std::runtime_error("'vertex.normal' must have one normal per vertex");
+      }
+      if (!idx)
+        throw std::runtime_error("mesh geometry requires 'index'");
+      int comps;
+      if (idx->elemType == OSP_VEC3UI || idx->elemType == OSP_VEC3I)
+        comps = 3;
+      else if (idx->elemType == OSP_VEC4UI || idx->elemType == OSP_VEC4I)
+        comps = 4;
+      else
+        throw std::runtime_error("mesh 'index' must be vec3ui (triangles) or vec4ui (quads)");
+      geomType = comps == 3 ? hpt::GEOM_TRIANGLES : hpt::GEOM_QUADS;
+      numPrims = idx->size();
+      index.resize(numPrims * comps);
+      const size_t nv = vertex.size() / 3;
+      for (uint64_t i = 0; i < numPrims; i++) {
+        std::memcpy(&index[i * comps], idx->item(i), 4 * comps);
+        for (int k = 0; k < comps; k++)
+          if (index[i * comps + k] >= nv)
+            throw std::runtime_error("mesh index out of range of 'vertex.position'");
+      }
+      dVertex.upload(vertex);
+      dIndex.upload(index);
+      if (!normal.empty())
+        dNormal.upload(normal);
+      else
+        dNormal.release();
+    } else if (subtype == "sphere") { // Spheres.cpp:46-106
+      Data *pos = getData(this, "sphere.position");
+      if (!pos)
+        throw std::runtime_error("sphere geometry requires 'sphere.position'");
+      const std::vector<float> p = dataFloats(pos, 3, "sphere.position");
+      const float radius = getFloat("radius", 0.01f);
+      std::vector<float> r;
+      if (Data *rd = getData(this, "sphere.radius"))
+        r = dataFloats(rd, 1, "sphere.radius");
+      geomType = hpt::GEOM_SPHERES;
+      numPrims = p.size() / 3;
+      if (!r.empty() && r.size() != numPrims)
+        throw std::runtime_error("'sphere.radius' must have one radius per sphere");
+      spheres.resize(numPrims * 4);
+      for (size_t i = 0; i < numPrims; i++) {
+        spheres[4 * i] = p[3 * i];
+        spheres[4 * i + 1] = p[3 * i + 1];
+        spheres[4 * i + 2] = p[3 * i + 2];
+        spheres[4 * i + 3] = r.empty() ? radius : r[i];
+      }
+      dSpheres.upload(spheres);
+    } else {
+      throw std::runtime_error("unsupported geometry type '" + subtype + "'");
+    }
+    version++;
+  }
+};
+
+struct Material : public ManagedObject
+{
+  hpt::DMaterial m{};
+  hpt::DMedium inside{}, outside{};
+  Material(const std::string &t)
+  {
+    managedType = OSP_MATERIAL;
+    subtype = t;
+  }
+  std::string toString() const override { return "ospray::Material(" + subtype + ")"; }
+  void commit() override
+  {
+    std::memset(&m, 0, sizeof(m));
+    if (subtype == "obj") { // OBJ.cpp:32-57
+      m.type = hpt::MAT_OBJ;
+      const float kdDef[3] = {0.8f, 0.8f, 0.8f}, zero[3] = {0.f, 0.f, 0.f};
+      float kd[3], tf[3], ks[3];
+      getVec3f("kd", kd, kdDef);
+      getVec3f("tf", tf, zero);
+      getVec3f("ks", ks, zero);
+      m.d = getFloat("d", 1.f);
+      (void)getFloat("ns", 10.f);
+      if (std::max(ks[0], std::max(ks[1], ks[2])) > 0.f)
+        throw std::runtime_error("obj material: specular 'ks' is not supported by the hip device");
+      vec3f Kd = v3(kd[0], kd[1], kd[2]), Tf = v3(tf[0], tf[1], tf[2]);
+      const float total = hpt::reduce_max(Kd + v3(0.f) + Tf);
+      if (total > 1.f) {
+        Kd = Kd / total;
+        Tf = Tf / total;
+      }
+      m.Kd[0] = Kd.x, m.Kd[1] = Kd.y, m.Kd[2] = Kd.z;
+      m.Tf[0] = Tf.x, m.Tf[1] = Tf.y, m.Tf[2] = Tf.z;
+    } else if (subtype == "metal") { // Metal.cpp:30-66
+      m.type = hpt::MAT_METAL;
+      const float etaDef[3] = {1.47f, 0.984f, 0.602f}, kDef[3] = {7.64f, 6.55f, 5.36f};
+      getVec3f("eta", m.eta, etaDef);
+      getVec3f("k", m.k, kDef);
+      m.roughness = getFloat("roughness", 0.1f);
+      if (getData(this, "ior"))
+        throw std::runtime_error("metal material: spectral 'ior' is not supported by the hip device");
+    } else if (subtype == "glass") { // Glass.cpp:36-62
+      m.type = hpt::MAT_GLASS;
+      const float etaInside = getFloat("etaInside", getFloat("eta", 1.5f));
+      const float etaOutside = getFloat("etaOutside", 1.f);
+      const float one[3] = {1.f, 1.f, 1.f};
+      float acol[3], acolIn[3], acolOut[3];
+      getVec3f("attenuationColor", acol, one);
+      getVec3f("attenuationColorInside", acolIn, acol);
+      getVec3f("attenuationColorOutside", acolOut, one);
+      const float dist = std::max(getFloat("attenuationDistance", 1.0f), 1e-5f);
+      inside.ior = etaInside;
+      outside.ior = etaOutside;
+      for (int k = 0; k < 3; k++) {
+        inside.attenuation[k] = std::log(acolIn[k]) / dist;
+        outside.attenuation[k] = std::log(acolOut[k]) / dist;
+      }
+    } else {
+      throw std::runtime_error("unsupported material type '" + subtype + "'");
+    }
+  }
+  bool opaque() const
+  {
+    if (m.type == hpt::MAT_METAL)
+      return true;
+    if (m.type == hpt::MAT_GLASS)
+      return false;
+    const vec3f T = hpt::ld3(m.Tf) * m.d + v3(1.f - m.d);
+    return T.x == 0.f && T.y == 0.f && T.z == 0.f;
+  }
+};
+
+struct GeometricModel : public ManagedObject
+{
+  Ref<Geometry> geom;
+  Ref<Material> material;
+  uint32_t userID = hpt::kInvalidID;
+  bool invertNormals = false;
+  GeometricModel() { managedType = OSP_GEOMETRIC_MODEL; }
+  std::string toString() const override { return "ospray::GeometricModel"; }
+  void commit() override // GeometricModel.cpp:21-87
+  {
+    geom.reset((Geometry *)getObject("geometry", OSP_GEOMETRY));
+    if (!geom)
+      throw std::runtime_error("geometric model requires a 'geometry'");
+    Material *mat = (Material *)getObject("material", OSP_MATERIAL);
+    if (!mat) {
+      if (Data *md = getData(this, "material")) {
+        if (md->elemType == OSP_MATERIAL && md->size() > 0) {
+          if (md->size() > 1)
+            throw std::runtime_error("per-primitive material arrays are not supported by the hip device");
+          mat = (Material *)md->get<ManagedObject *>(0);
+        } else if (md->elemType == OSP_UINT || md->elemType == OSP_UCHAR) {
+          throw std::runtime_error("renderer material lists are not supported by the hip device");
+        }
+      }
+      if (findParamAny("material") && !mat && getUInt("material", 0xffffffffu) != 0xffffffffu)
+        throw std::runtime_error("renderer material indices are not supported by the hip device");
+    }
+    material.reset(mat);
+    userID = getUInt("id", hpt::kInvalidID);
+    invertNormals = getBool("invertNormals", false);
+  }
+};
+
+struct Group : public ManagedObject
+{
+  std::vector<GeometricModel *> models;
+  std::vector<std::unique_ptr<Ref<GeometricModel>>> refs;
+  Group() { managedType = OSP_GROUP; }
+  std::string toString() const override { return "ospray::Group"; }
+  void commit() override // Group.cpp:69-209
+  {
+    models = dataObjects<GeometricModel>(getData(this, "geometry"), OSP_GEOMETRIC_MODEL);
+    refs.clear();
+    for (auto *m : models) {
+      if (!m)
+        throw std::runtime_error("group 'geometry' holds a null model");
+      refs.emplace_back(new Ref<GeometricModel>());
+      refs.back()->reset(m);
+    }
+    if (getData(this, "volume"))
+      throw std::runtime_error("volumes are not supported by the hip device");
+    if (getData(this, "light"))
+      throw std::runtime_error("group (instance) lights are not supported by the hip device");
+    (void)getData(this, "clippingGeometry");
+  }
+};
+
+struct Instance : public ManagedObject
+{
+  Ref<Group> group;
+  float xfm[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+  uint32_t userID = hpt::kInvalidID;
+  Instance() { managedType = OSP_INSTANCE; }
+  std::string toString() const override { return "ospray::Instance"; }
+  void commit() override // Instance.cpp:20-34
+  {
+    group.reset((Group *)getObject("group", OSP_GROUP));
+    if (!group)
+      throw std::runtime_error("instance requires a 'group'");
+    const float id[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    std::memcpy(xfm, id, sizeof(xfm));
+    if (!getAffine3f("transform", xfm))
+      getAffine3f("xfm", xfm);
+    userID = getUInt("id", hpt::kInvalidID);
+  }
+};
+
+struct Light : public ManagedObject
+{
+  hpt::DLight l{};
+  Light(const std::string &t)
+  {
+    managedType = OSP_LIGHT;
+    subtype = t;
+  }
+  std::string toString() const override { return "ospray::Light(" + subtype + ")"; }
+  void commit() override
+  {
+    std::memset(&l, 0, sizeof(l));
+    l.visible = getBool("visible", true) ? 1 : 0; // Light.cpp:20-24
+    const float one[3] = {1.f, 1.f, 1.f};
+    float col[3];
+    getVec3f("color", col, one);
+    const float intensity = getFloat("intensity", 1.f);
+    const vec3f colored = v3(col[0], col[1], col[2]) * intensity;
+    auto put = [](float *dst, vec3f v) {
+      dst[0] = v.x;
+      dst[1] = v.y;
+      dst[2] = v.z;
+    };
+    if (subtype == "quad") { // QuadLight.cpp:68-105, QuadLight.ispc:15-33
+      l.type = hpt::LIGHT_QUAD;
+      const float p0[3] = {0.f, 0.f, 0.f}, e1d[3] = {1.f, 0.f, 0.f}, e2d[3] = {0.f, 1.f, 0.f};
+      getVec3f("position", l.position, p0);
+      getVec3f("edge1", l.edge1, e1d);
+      getVec3f("edge2", l.edge2, e2d);
+      const uint32_t q = getUInt("intensityQuantity", OSP_INTENSITY_QUANTITY_RADIANCE);
+      const vec3f e1 = hpt::ld3(l.edge1), e2 = hpt::ld3(l.edge2);
+      const float area = hpt::length(hpt::cross(e1, e2));
+      vec3f rad = v3(0.f);
+      if (q == OSP_INTENSITY_QUANTITY_INTENSITY)
+        rad = colored / area;
+      else if (q == OSP_INTENSITY_QUANTITY_POWER)
+        rad = colored / (hpt::kPi * area);
+      else if (q == OSP_INTENSITY_QUANTITY_RADIANCE)
+        rad = colored;
+      put(l.radiance, rad);
+      const vec3f nd = hpt::cross(e2, e1);
+      l.ppdf = hpt::rcp(hpt::length(nd));
+      put(l.nnormal, nd * l.ppdf);
+    } else if (subtype == "distant") { // DirectionalLight.cpp:20-106
+      l.type = hpt::LIGHT_DISTANT;
+      const float dDef[3] = {0.f, 0.f, 1.f};
+      float d[3];
+      getVec3f("direction", d, dDef);
+      const vec3f dir = hpt::neg(hpt::normalize(v3(d[0], d[1], d[2])));
+      const float ad = hpt::clampf(getFloat("angularDiameter", 0.f), 0.f, 180.f);
+      l.cosAngle = std::cos(ad * 0.5f * 0.01745329251994329577f);
+      const uint32_t q = getUInt("intensityQuantity", OSP_INTENSITY_QUANTITY_IRRADIANCE);
+      vec3f irr = v3(0.f);
+      if (q == OSP_INTENSITY_QUANTITY_RADIANCE)
+        irr = (2.0f * hpt::kPi * (1.0f - l.cosAngle)) * colored;
+      else if (q == OSP_INTENSITY_QUANTITY_IRRADIANCE)
+        irr = colored;
+      put(l.irradiance, irr);
+      const hpt::linear3f fr = hpt::frame(dir);
+      put(l.frame, fr.vx);
+      put(l.frame + 3, fr.vy);
+      put(l.frame + 6, fr.vz);
+      l.pdf = l.cosAngle < 0.99999988f ? hpt::rcp(hpt::kTwoPi * (1.0f - l.cosAngle)) : hpt::kInf;
+    } else if (subtype == "ambient") { // AmbientLight.cpp:41-60
+      l.type = hpt::LIGHT_AMBIENT;
+      const uint32_t q = getUInt("intensityQuantity", OSP_INTENSITY_QUANTITY_RADIANCE);
+      vec3f rad = v3(0.f);
+      if (q == OSP_INTENSITY_QUANTITY_IRRADIANCE)
+        rad = colored / hpt::kPi;
+      else if (q == OSP_INTENSITY_QUANTITY_RADIANCE)
+        rad = colored;
+      put(l.radiance, rad);
+    } else {
+      throw std::runtime_error("unsupported light type '" + subtype + "'");
+    }
+  }
+};
+
+// BLAS of one list of geometries (shared by groups with the same list)
+struct Blas
+{
+  std::vector<hpt::DNode> nodes;
+  std::vector<hpt::DPrim> prims;
+  hpt::BuildBox bounds;
+  int maxDepth = 0;
+};
+
+struct World : public ManagedObject
+{
+  HipDevice *dev;
+  std::vector<std::unique_ptr<Ref<Instance>>> instRefs;
+  std::vector<std::unique_ptr<Ref<Light>>> lightRefs;
+  std::vector<Instance *> instances;
+  std::vector<Material *> materials; // index order of the material table
+  DevBuf dNodes, dPrims, dInstances, dModels, dMaterials, dMedia, dLights, dLightsCDF;
+  hpt::DScene scene{};
+  bool built = false;
+  int64_t totalNodes = 0, totalPrims = 0;
+  int maxDepth = 0;
+  float bounds[6] = {0, 0, 0, 0, 0, 0};
+  World(HipDevice *d) : dev(d) { managedType = OSP_WORLD; }
+  std::string toString() const override { return "ospray::World"; }
+  void commit() override;
+};
+
+struct Camera : public ManagedObject
+{
+  float pos[3], dir[3], up[3], fovy, aspect, nearClip;
+  Camera(const std::string &t)
+  {
+    managedType = OSP_CAMERA;
+    subtype = t;
+  }
+  std::string toString() const override { return "ospray::Camera(" + subtype + ")"; }
+  void commit() override // Camera.cpp:38-52, PerspectiveCamera.cpp:30-44
+  {
+    const float z[3] = {0.f, 0.f, 0.f}, zd[3] = {0.f, 0.f, 1.f}, yu[3] = {0.f, 1.f, 0.f};
+    getVec3f("position", pos, z);
+    getVec3f("direction", dir, zd);
+    getVec3f("up", up, yu);
+    nearClip = std::max(getFloat("nearClip", 1e-6f), 0.f);
+    fovy = getFloat("fovy", 60.f);
+    aspect = getFloat("aspect", 1.f);
+    if (getFloat("apertureRadius", 0.f) != 0.f)
+      throw std::runtime_error("perspective camera: depth of field is not supported by the hip device");
+    if (getUInt("stereoMode", OSP_STEREO_NONE) != OSP_STEREO_NONE)
+      throw std::runtime_error("perspective camera: stereo is not supported by the hip device");
+  }
+  // PerspectiveCamera::commit set-up (PerspectiveCamera.cpp:44-110)
+  void setup(hpt::DFrame &F) const
+  {
+    const float imgY = 2.f * std::tan(0.5f * fovy * 0.01745329251994329577f);
+    const float imgX = imgY * aspect;
+    vec3f d00 = hpt::normalize(v3(dir[0], dir[1], dir[2]));
+    vec3f du = hpt::normalize(hpt::cross(d00, v3(up[0], up[1], up[2])));
+    vec3f dv = hpt::cross(du, d00);
+    du = du * imgX;
+    dv = dv * imgY;
+    d00 = d00 - 0.5f * du - 0.5f * dv;
+    auto put = [](float *dst, vec3f v) {
+      dst[0] = v.x;
+      dst[1] = v.y;
+      dst[2] = v.z;
+    };
+    put(F.camOrg, v3(pos[0], pos[1], pos[2]));
+    put(F.camDir00, d00);
+    put(F.camDu, du);
+    put(F.camDv, dv);
+    put(F.camDirection, v3(dir[0], dir[1], dir[2]));
+    F.nearClip = nearClip;
+  }
+};
+
+struct Renderer : public ManagedObject
+{
+  int spp = 1, maxDepth = 20, rouletteDepth = 5, maxScatteringEvents = 20, numLightSamples = -1;
+  bool limitIndirect = true;
+  float minContribution = 0.001f, maxRadiance = hpt::kInf, errorThreshold = 0.f;
+  float bg[4] = {0, 0, 0, 0};
+  int filterType = 2;
+  float filterWidth = 3.f;
+  int lutSize = 0;
+  DevBuf dLutX, dLutY;
+  Renderer(const std::string &t)
+  {
+    managedType = OSP_RENDERER;
+    subtype = t;
+  }
+  std::string toString() const override { return "ospray::PathTracer"; }
+  void commit() override // Renderer.cpp:38-92, PathTracer.cpp:29-51
+  {
+    spp = std::max(1, getInt("pixelSamples", 1));
+    maxDepth = std::max(0, getInt("maxPathLength", 20));
+    minContribution = getFloat("minContribution", 0.001f);
+    errorThreshold = getFloat("varianceThreshold", 0.f);
+    rouletteDepth = getInt("roulettePathLength", 5);
+    maxScatteringEvents = getInt("maxScatteringEvents", 20);
+    maxRadiance = getFloat("maxContribution", hpt::kInf);
+    numLightSamples = getInt("lightSamples", -1);
+    limitIndirect = getBool("limitIndirectLightSamples", true);
+    const float bgF = getFloat("backgroundColor", 0.f);
+    const float bg3d[3] = {bgF, bgF, bgF};
+    float bg3[3];
+    getVec3f("backgroundColor", bg3, bg3d);
+    const float bg4d[4] = {bg3[0], bg3[1], bg3[2], 0.f};
+    getVec4f("backgroundColor", bg, bg4d);
+    if (getBool("backgroundRefraction", false))
+      throw std::runtime_error("pathtracer: 'backgroundRefraction' is not supported by the hip device");
+    float scp[4];
+    const float scpd[4] = {0, 0, 0, 0};
+    getVec4f("shadowCatcherPlane", scp, scpd);
+    if (scp[0] != 0.f || scp[1] != 0.f || scp[2] != 0.f)
+      throw std::runtime_error("pathtracer: shadow catcher is not supported by the hip device");
+    (void)getBool("geometryLights", true);
+    (void)getFloat("mipMapBias", 0.f);
+    const uint32_t pf = getUInt("pixelFilter", OSP_PIXELFILTER_GAUSS);
+    if (pf == OSP_PIXELFILTER_POINT) {
+      filterType = 0;
+      filterWidth = 0.f;
+    } else if (pf == OSP_PIXELFILTER_BOX) {
+      filterType = 1;
+      filterWidth = 1.f;
+    } else if (pf == OSP_PIXELFILTER_GAUSS) {
+      // Gaussian LUT of width 3, 5 bins per pixel (PixelFilter.cpp:19-56,
+      // LUTPixelFilter.ispc:17-120)
+      filterType = 2;
+      filterWidth = 3.f;
+      const int n = (int)std::ceil(filterWidth) * 5;
+      lutSize = n;
+      std::vector<float> cx(n * n), cy(n);
+      const float sigma = (0.5f * filterWidth) / 3.0f;
+      const float sigmaSqr = sigma * sigma;
+      for (int y = 0; y < n; y++) {
+        const float fy = (y + 0.5f) * (1.0f / (float)n);
+        for (int x = 0; x < n; x++) {
+          const float fx = (x + 0.5f) * (1.0f / (float)n);
+          const float ux = (fx - 0.5f) * filterWidth, uy = (fy - 0.5f) * filterWidth;
+          const float distSqr = ux * ux + uy * uy;
+          cx[y * n + x] = hpt::maxf(0.0f, hpt::fexp(-0.5f * (distSqr / sigmaSqr)));
+        }
+        cy[y] = hpt::distribution1DCreate(n, cx.data() + y * n);
+      }
+      hpt::distribution1DCreate(n, cy.data());
+      dLutX.upload(cx);
+      dLutY.upload(cy);
+    } else {
+      throw std::runtime_error("pixel filter type not supported by the hip device");
+    }
+  }
+};
+
+struct FrameBuffer : public ManagedObject
+{
+  HipDevice *dev;
+  int w, h;
+  OSPFrameBufferFormat format;
+  uint32_t channels;
+  int frameID = -1;
+  std::mt19937 mtGen{43};
+  bool pickOdd = false;
+  bool accumulateVariance = false;
+  int targetFrames = 0;
+  bool projectedDepth = false;
+  bool varianceValid = false;
+  float errorThreshold = 0.f;
+  DevBuf color, variance, depth, normal, albedo, firstNormal, position, primID, objID, instID, taskVar;
+  DevBuf accRGBA, accAux, accZ, accIDs, converted, varianceOut;
+  std::map<int, std::vector<unsigned char>> staging;
+  float *hostVariance = nullptr; // pinned
+  FrameBuffer(HipDevice *d, int w_, int h_, OSPFrameBufferFormat f, uint32_t ch);
+  ~FrameBuffer() override;
+  std::string toString() const override { return "ospray::LocalFrameBuffer"; }
+  void commit() override // FrameBuffer.cpp:85-93
+  {
+    targetFrames = std::max(0, getInt("targetFrames", 0));
+    projectedDepth = getBool("projectedDepth", false);
+    if (!(channels & OSP_FB_ACCUM))
+      targetFrames = 1;
+  }
+  void clear() // FrameBuffer.cpp:95-103
+  {
+    frameID = -1;
+    mtGen.seed(43);
+    varianceValid = false;
+  }
+};
+
+struct Future : public ManagedObject
+{
+  HipDevice *dev;
+  hipEvent_t start = nullptr, rendered = nullptr, finished = nullptr;
+  Ref<ManagedObject> fb, renderer, camera, world;
+  bool empty = false;
+  Future(HipDevice *d) : dev(d) { managedType = OSP_FUTURE; }
+  ~Future() override;
+  std::string toString() const override { return "ospray::Future"; }
+};
+
+// ---- the device --------------------------------------------------------------------------
+
+struct KernelTimer
+{
+  std::string name;
+  int64_t launches = 0;
+  double totalMs = 0;
+};
+
+class HipDevice : public Device
+{
+ public:
+  hipStream_t stream = nullptr;
+  int numCUs = 0;
+  std::string gpuName;
+  // path state, shared by all framebuffers of this device
+  DevBuf pathBuf, recBuf, queueBuf, counters, overflow, stats;
+  hpt::DPaths paths{};
+  size_t pathSlots = 0, recSlots = 0;
+  bool auxAllocated = false;
+  float *auxBase = nullptr;
+  int residentTraceBlocks = 0, shadeBlocks = 0;
+  size_t pathBudgetBytes = (size_t)16 << 30;
+  bool profile = false, travStats = false;
+  std::vector<KernelTimer> timers;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pendingEvents;
+  std::map<std::vector<std::pair<const Geometry *, uint64_t>>, std::shared_ptr<Blas>> blasCache;
+
+  HipDevice() {}
+  ~HipDevice() override
+  {
+    collectTimers();
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+
+  void commitDevice() override
+  {
+    if (!stream) {
+      int dev = getInt("deviceID", 0);
+      HIP_CHECK(hipSetDevice(dev));
+      HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      hipDeviceProp_t prop;
+      HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+      numCUs = prop.multiProcessorCount;
+      gpuName = prop.gcnArchName;
+      residentTraceBlocks = std::max(1, numCUs) * 8;
+      shadeBlocks = std::max(1, numCUs) * 8;
+      HIP_CHECK(hipMalloc(&counters.ptr, 64));
+      counters.bytes = 64;
+      stats.ensure(64);
+      HIP_CHECK(hipMemset(stats.ptr, 0, 64));
+      overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
+    }
+    profile = getBool("profileKernels", false);
+    travStats = getBool("traversalStats", false);
+    const int budgetMB = getInt("pathBudgetMB", 16384);
+    pathBudgetBytes = (size_t)std::max(64, budgetMB) << 20;
+  }
+
+  // ---- timing ----
+  int timerIndex(const char *name)
+  {
+    for (size_t i = 0; i < timers.size(); i++)
+      if (timers[i].name == name)
+        return (int)i;
+    timers.push_back({name, 0, 0.0});
+    return (int)timers.size() - 1;
+  }
+  template <typename F>
+  void launch(const char *name, F &&fn)
+  {
+    if (!profile) {
+      HIP_CHECK(fn());
+      return;
+    }
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(a, stream));
+    HIP_CHECK(fn());
+    HIP_CHECK(hipEventRecord(b, stream));
+    pendingEvents.push_back({timerIndex(name), {a, b}});
+  }
+  void collectTimers()
+  {
+    for (auto &pe : pendingEvents) {
+      float ms = 0.f;
+      if (hipEventSynchronize(pe.second.second) == hipSuccess
+          && hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+        timers[pe.first].launches++;
+        timers[pe.first].totalMs += ms;
+      }
+      (void)hipEventDestroy(pe.second.first);
+      (void)hipEventDestroy(pe.second.second);
+    }
+    pendingEvents.clear();
+  }
+
+  // ---- objects ----
+  ManagedObject *newObject(OSPDataType kind, const char *type) override
+  {
+    const std::string t(type ? type : "");
+    switch (kind) {
+    case OSP_CAMERA:
+      if (t == "perspective")
+        return new Camera(t);
+      break;
+    case OSP_GEOMETRY:
+      if (t == "mesh" || t == "sphere")
+        return new Geometry(t);
+      break;
+    case OSP_MATERIAL:
+      if (t == "obj" || t == "metal" || t == "glass")
+        return new Material(t);
+      break;
+    case OSP_LIGHT:
+      if (t == "quad" || t == "distant" || t == "ambient")
+        return new Light(t);
+      break;
+    case OSP_RENDERER:
+      if (t == "pathtracer")
+        return new Renderer(t);
+      break;
+    case OSP_GEOMETRIC_MODEL:
+      return new GeometricModel();
+    case OSP_GROUP:
+      return new Group();
+    case OSP_INSTANCE:
+      return new Instance();
+    case OSP_WORLD:
+      return new World(this);
+    default:
+      break;
+    }
+    static const std::map<OSPDataType, std::string> names = {{OSP_CAMERA, "camera"}, {OSP_GEOMETRY, "geometry"},
+        {OSP_MATERIAL, "material"}, {OSP_LIGHT, "light"}, {OSP_RENDERER, "renderer"}, {OSP_VOLUME, "volume"},
+        {OSP_TEXTURE, "texture"}, {OSP_TRANSFER_FUNCTION, "transfer function"},
+        {OSP_IMAGE_OPERATION, "image operation"}, {OSP_VOLUMETRIC_MODEL, "volumetric model"}};
+    auto it = names.find(kind);
+    throw std::runtime_error("Could not find " + (it != names.end() ? it->second : std::string("object")) + " of type: " + t
+        + " (not provided by the hip device)");
+  }
+
+  OSPBounds getBounds(ManagedObject *obj) override
+  {
+    OSPBounds b;
+    for (int i = 0; i < 3; i++) {
+      b.lower[i] = hpt::kInf;
+      b.upper[i] = -hpt::kInf;
+    }
+    if (obj->managedType == OSP_WORLD) {
+      World *w = (World *)obj;
+      if (w->built)
+        for (int i = 0; i < 3; i++) {
+          b.lower[i] = w->bounds[i];
+          b.upper[i] = w->bounds[3 + i];
+        }
+    } else if (obj->managedType == OSP_GEOMETRY) {
+      Geometry *g = (Geometry *)obj;
+      const std::vector<float> &v = g->geomType == hpt::GEOM_SPHERES ? g->spheres : g->vertex;
+      const int stride = g->geomType == hpt::GEOM_SPHERES ? 4 : 3;
+      for (size_t i = 0; i + 2 < v.size(); i += stride)
+        for (int k = 0; k < 3; k++) {
+          const float r = stride == 4 ? v[i + 3] : 0.f;
+          b.lower[k] = std::min(b.lower[k], v[i + k] - r);
+          b.upper[k] = std::max(b.upper[k], v[i + k] + r);
+        }
+    }
+    return b;
+  }
+
+  // ---- framebuffer ----
+  ManagedObject *frameBufferCreate(int w, int h, OSPFrameBufferFormat f, uint32_t ch) override
+  {
+    if (w <= 0 || h <= 0)
+      throw std::runtime_error("framebuffer has invalid size. Dimensions must be greater than 0");
+    return new FrameBuffer(this, w, h, f, ch);
+  }
+
+  const void *frameBufferMap(ManagedObject *o, OSPFrameBufferChannel ch) override // LocalFB.cpp:489-522
+  {
+    FrameBuffer *fb = (FrameBuffer *)o;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    const size_t npix = (size_t)fb->w * fb->h;
+    const DevBuf *src = nullptr;
+    size_t bytes = 0;
+    switch (ch) {
+    case OSP_FB_COLOR:
+      if (fb->format == OSP_FB_NONE || !(fb->channels & OSP_FB_COLOR))
+        return nullptr;
+      src = &fb->converted;
+      bytes = npix * (fb->format == OSP_FB_RGBA32F ? 16 : 4);
+      break;
+    case OSP_FB_DEPTH: src = &fb->depth; bytes = npix * 4; break;
+    case OSP_FB_NORMAL: src = &fb->normal; bytes = npix * 12; break;
+    case OSP_FB_ALBEDO: src = &fb->albedo; bytes = npix * 12; break;
+    case OSP_FB_FIRST_NORMAL: src = &fb->firstNormal; bytes = npix * 12; break;
+    case OSP_FB_POSITION: src = &fb->position; bytes = npix * 12; break;
+    case OSP_FB_ID_PRIMITIVE: src = &fb->primID; bytes = npix * 4; break;
+    case OSP_FB_ID_OBJECT: src = &fb->objID; bytes = npix * 4; break;
+    case OSP_FB_ID_INSTANCE: src = &fb->instID; bytes = npix * 4; break;
+    default: return nullptr;
+    }
+    if (!src->ptr)
+      return nullptr;
+    auto &st = fb->staging[(int)ch];
+    st.resize(bytes);
+    HIP_CHECK(hipMemcpy(st.data(), src->ptr, bytes, hipMemcpyDeviceToHost));
+    fb->refInc();
+    return st.data();
+  }
+
+  void frameBufferUnmap(const void *mapped, ManagedObject *o) override
+  {
+    if (mapped)
+      o->refDec();
+  }
+
+  float getVariance(ManagedObject *o) override // LocalFB.cpp:263-272
+  {
+    FrameBuffer *fb = (FrameBuffer *)o;
+    if (!fb->variance.ptr || !fb->varianceValid)
+      return hpt::kInf;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    return *fb->hostVariance;
+  }
+
+  void resetAccumulation(ManagedObject *o) override { ((FrameBuffer *)o)->clear(); }
+
+  // ---- rendering ----
+  ManagedObject *renderFrame(ManagedObject *fbo, ManagedObject *ro, ManagedObject *co, ManagedObject *wo) override;
+  int isReady(ManagedObject *fo, OSPSyncEvent e) override
+  {
+    Future *f = (Future *)fo;
+    if (f->empty)
+      return 1;
+    hipEvent_t ev = e <= OSP_WORLD_RENDERED ? f->rendered : f->finished;
+    return hipEventQuery(ev) == hipSuccess ? 1 : 0;
+  }
+  void wait(ManagedObject *fo, OSPSyncEvent e) override
+  {
+    Future *f = (Future *)fo;
+    if (f->empty)
+      return;
+    HIP_CHECK(hipEventSynchronize(e <= OSP_WORLD_RENDERED ? f->rendered : f->finished));
+    if (e >= OSP_FRAME_FINISHED)
+      collectTimers();
+  }
+  void cancel(ManagedObject *) override
+  {
+    // queued GPU work is not interrupted; the frame completes normally
+  }
+  float getProgress(ManagedObject *fo) override { return isReady(fo, OSP_TASK_FINISHED) ? 1.f : 0.f; }
+  float getTaskDuration(ManagedObject *fo) override
+  {
+    Future *f = (Future *)fo;
+    if (f->empty)
+      return 0.f;
+    HIP_CHECK(hipEventSynchronize(f->finished));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, f->start, f->finished));
+    return ms * 1e-3f;
+  }
+  OSPPickResult pick(ManagedObject *fbo, ManagedObject *ro, ManagedObject *co, ManagedObject *wo, float x, float y) override;
+
+  void ensurePaths(size_t slots, int maxRec, bool aux);
+  int traceRays(World *w, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2);
+};
+
+FrameBuffer::FrameBuffer(HipDevice *d, int w_, int h_, OSPFrameBufferFormat f, uint32_t ch)
+    : dev(d), w(w_), h(h_), format(f), channels(ch)
+{
+  managedType = OSP_FRAMEBUFFER;
+  targetFrames = (ch & OSP_FB_ACCUM) ? 0 : 1;
+  const size_t n = (size_t)w * h;
+  auto zero = [&](DevBuf &b, size_t bytes) {
+    b.ensure(bytes);
+    HIP_CHECK(hipMemset(b.ptr, 0, bytes));
+  };
+  if (ch & OSP_FB_COLOR)
+    zero(color, n * 16);
+  if (ch & OSP_FB_VARIANCE)
+    zero(variance, n * 16);
+  if (ch & OSP_FB_DEPTH)
+    zero(depth, n * 4);
+  if (ch & OSP_FB_NORMAL)
+    zero(normal, n * 12);
+  if (ch & OSP_FB_ALBEDO)
+    zero(albedo, n * 12);
+  if (ch & OSP_FB_FIRST_NORMAL)
+    zero(firstNormal, n * 12);
+  if (ch & OSP_FB_POSITION)
+    zero(position, n * 12);
+  if (ch & OSP_FB_ID_PRIMITIVE)
+    zero(primID, n * 4);
+  if (ch & OSP_FB_ID_OBJECT)
+    zero(objID, n * 4);
+  if (ch & OSP_FB_ID_INSTANCE)
+    zero(instID, n * 4);
+  const size_t ntask = (size_t)((w + 7) / 8) * ((h + 7) / 8);
+  zero(taskVar, ntask * 4);
+  zero(accRGBA, n * 16);
+  if (ch & (OSP_FB_NORMAL | OSP_FB_ALBEDO | OSP_FB_FIRST_NORMAL | OSP_FB_POSITION))
+    zero(accAux, n * 48);
+  zero(accZ, n * 4);
+  zero(accIDs, n * 12);
+  if ((ch & OSP_FB_COLOR) && f != OSP_FB_NONE)
+    zero(converted, n * (f == OSP_FB_RGBA32F ? 16 : 4));
+  zero(varianceOut, 16);
+  HIP_CHECK(hipHostMalloc((void **)&hostVariance, sizeof(float), hipHostMallocDefault));
+  *hostVariance = hpt::kInf;
+}
+
+FrameBuffer::~FrameBuffer()
+{
+  if (dev && dev->stream)
+    (void)hipStreamSynchronize(dev->stream);
+  if (hostVariance)
+    (void)hipHostFree(hostVariance);
+}
+
+Future::~Future()
+{
+  if (!empty && finished)
+    (void)hipEventSynchronize(finished);
+  if (start)
+    (void)hipEventDestroy(start);
+  if (rendered)
+    (void)hipEventDestroy(rendered);
+  if (finished)
+    (void)hipEventDestroy(finished);
+}
+
+// ---- world build -------------------------------------------------------------------------
+
+static hpt::BuildBox primBox(const Geometry &g, size_t p, int half)
+{
+  hpt::BuildBox b;
+  auto vtx = [&](uint32_t i) { return &g.vertex[3 * (size_t)i]; };
+  if (g.geomType == hpt::GEOM_SPHERES) {
+    const float *s = &g.spheres[4 * p];
+    for (int k = 0; k < 3; k++) {
+      b.lo[k] = s[k] - s[3];
+      b.hi[k] = s[k] + s[3];
+    }
+    return b;
+  }
+  const float *v[3];
+  if (g.geomType == hpt::GEOM_TRIANGLES) {
+    const uint32_t *ix = &g.index[3 * p];
+    v[0] = vtx(ix[0]), v[1] = vtx(ix[1]), v[2] = vtx(ix[2]);
+  } else {
+    const uint32_t *ix = &g.index[4 * p];
+    if (half == 0)
+      v[0] = vtx(ix[0]), v[1] = vtx(ix[1]), v[2] = vtx(ix[3]);
+    else
+      v[0] = vtx(ix[2]), v[1] = vtx(ix[3]), v[2] = vtx(ix[1]);
+  }
+  for (int k = 0; k < 3; k++) {
+    b.lo[k] = std::min(v[0][k], std::min(v[1][k], v[2][k]));
+    b.hi[k] = std::max(v[0][k], std::max(v[1][k], v[2][k]));
+  }
+  return b;
+}
+
+static std::shared_ptr<Blas> buildBlas(const std::vector<Geometry *> &geoms)
+{
+  struct Item
+  {
+    uint32_t model, prim, half;
+  };
+  std::vector<Item> items;
+  std::vector<hpt::BuildBox> boxes;
+  for (uint32_t m = 0; m < geoms.size(); m++) {
+    const Geometry &g = *geoms[m];
+    const int halves = g.geomType == hpt::GEOM_QUADS ? 2 : 1;
+    for (size_t p = 0; p < g.numPrims; p++)
+      for (int h = 0; h < halves; h++) {
+        items.push_back({m, (uint32_t)p, (uint32_t)h});
+        boxes.push_back(primBox(g, p, h));
+      }
+  }
+  const int nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+  hpt::BuildResult R = hpt::buildBVH(boxes, 4, std::min(nthreads, 16));
+  auto B = std::make_shared<Blas>();
+  B->nodes = std::move(R.nodes);
+  B->bounds = R.bounds;
+  B->maxDepth = R.maxDepth;
+  B->prims.resize(R.order.size());
+  for (size_t k = 0; k < R.order.size(); k++) {
+    const Item &it = items[R.order[k]];
+    const Geometry &g = *geoms[it.model];
+    hpt::DPrim &P = B->prims[k];
+    std::memset(&P, 0, sizeof(P));
+    P.primID = it.prim;
+    if (g.geomType == hpt::GEOM_SPHERES) {
+      const float *s = &g.spheres[4 * (size_t)it.prim];
+      P.a[0] = s[0], P.a[1] = s[1], P.a[2] = s[2];
+      P.b[0] = s[3];
+      P.modelKind = it.model | ((uint32_t)hpt::PRIM_SPHERE << 28);
+      continue;
+    }
+    uint32_t vi[3];
+    uint32_t kind;
+    if (g.geomType == hpt::GEOM_TRIANGLES) {
+      const uint32_t *ix = &g.index[3 * (size_t)it.prim];
+      vi[0] = ix[0], vi[1] = ix[1], vi[2] = ix[2];
+      kind = hpt::PRIM_TRIANGLE;
+    } else {
+      const uint32_t *ix = &g.index[4 * (size_t)it.prim];
+      if (it.half == 0)
+        vi[0] = ix[0], vi[1] = ix[1], vi[2] = ix[3], kind = hpt::PRIM_QUAD0;
+      else
+        vi[0] = ix[2], vi[1] = ix[3], vi[2] = ix[1], kind = hpt::PRIM_QUAD1;
+    }
+    for (int k = 0; k < 3; k++) {
+      P.a[k] = g.vertex[3 * (size_t)vi[0] + k];
+      P.b[k] = g.vertex[3 * (size_t)vi[1] + k];
+      P.c[k] = g.vertex[3 * (size_t)vi[2] + k];
+    }
+    P.modelKind = it.model | (kind << 28);
+  }
+  if (geoms.size() >= (1u << 28))
+    throw std::runtime_error("too many geometric models in one group");
+  return B;
+}
+
+void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:93-143)
+{
+  HipDevice &D = *dev;
+  instances = dataObjects<Instance>(getData(this, "instance"), OSP_INSTANCE);
+  std::vector<Light *> lights = dataObjects<Light>(getData(this, "light"), OSP_LIGHT);
+  instRefs.clear();
+  lightRefs.clear();
+  for (auto *i : instances) {
+    if (!i || !i->group)
+      throw std::runtime_error("world 'instance' holds an uncommitted or null instance");
+    instRefs.emplace_back(new Ref<Instance>());
+    instRefs.back()->reset(i);
+  }
+  for (auto *l : lights) {
+    if (!l)
+      throw std::runtime_error("world 'light' holds a null light");
+    lightRefs.emplace_back(new Ref<Light>());
+    lightRefs.back()->reset(l);
+  }
+  if (getData(this, "volume"))
+    throw std::runtime_error("volumes are not supported by the hip device");
+
+  // BLAS per distinct geometry list, cached across commits
+  std::vector<Group *> groups;
+  std::map<Group *, int> groupIndex;
+  for (auto *i : instances)
+    if (!groupIndex.count(i->group.p)) {
+      groupIndex[i->group.p] = (int)groups.size();
+      groups.push_back(i->group.p);
+    }
+  std::vector<std::shared_ptr<Blas>> groupBlas(groups.size());
+  std::map<const Blas *, int> blasSlot;
+  std::vector<std::shared_ptr<Blas>> uniqueBlas;
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    std::vector<Geometry *> geoms;
+    std::vector<std::pair<const Geometry *, uint64_t>> key;
+    for (auto *m : groups[gi]->models) {
+      if (!m->geom)
+        throw std::runtime_error("group holds an uncommitted geometric model");
+      geoms.push_back(m->geom.p);
+      key.push_back({m->geom.p, m->geom->version});
+    }
+    auto it = D.blasCache.find(key);
+    if (it == D.blasCache.end())
+      it = D.blasCache.emplace(key, buildBlas(geoms)).first;
+    groupBlas[gi] = it->second;
+    if (!blasSlot.count(it->second.get())) {
+      blasSlot[it->second.get()] = (int)uniqueBlas.size();
+      uniqueBlas.push_back(it->second);
+    }
+  }
+  // drop cache entries no longer referenced by this world
+  for (auto it = D.blasCache.begin(); it != D.blasCache.end();) {
+    if (!blasSlot.count(it->second.get()))
+      it = D.blasCache.erase(it);
+    else
+      ++it;
+  }
+
+  // materials, media and models
+  materials.clear();
+  std::map<Material *, int> matIndex;
+  std::vector<hpt::DMaterial> dmats;
+  std::vector<hpt::DMedium> media(1);
+  media[0] = {{0.f, 0.f, 0.f}, 1.0f}; // vacuum (Medium.ih make_Medium_vacuum)
+  auto materialOf = [&](Material *m) -> int {
+    if (!m)
+      return -1;
+    auto it = matIndex.find(m);
+    if (it != matIndex.end())
+      return it->second;
+    hpt::DMaterial dm = m->m;
+    if (dm.type == hpt::MAT_GLASS) {
+      dm.inside = (int)media.size();
+      media.push_back(m->inside);
+      dm.outside = (int)media.size();
+      media.push_back(m->outside);
+    }
+    const int idx = (int)dmats.size();
+    dmats.push_back(dm);
+    materials.push_back(m);
+    matIndex[m] = idx;
+    return idx;
+  };
+  std::vector<hpt::DModel> models;
+  std::vector<int> groupModelOffset(groups.size());
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    groupModelOffset[gi] = (int)models.size();
+    for (auto *m : groups[gi]->models) {
+      const Geometry &g = *m->geom.p;
+      hpt::DModel dm;
+      std::memset(&dm, 0, sizeof(dm));
+      dm.vertex = g.dVertex.as<const float>();
+      dm.index = g.dIndex.as<const uint32_t>();
+      dm.normal = g.normal.empty() ? nullptr : g.dNormal.as<const float>();
+      dm.spheres = g.dSpheres.as<const float>();
+      dm.geomType = g.geomType;
+      dm.material = materialOf(m->material.p);
+      dm.userID = m->userID;
+      dm.invertNormals = m->invertNormals ? 1 : 0;
+      dm.opaque = m->material ? (m->material->opaque() ? 1 : 0) : 1;
+      models.push_back(dm);
+    }
+  }
+  if (media.size() >= (1u << 12))
+    throw std::runtime_error("too many glass materials in one world");
+
+  // concatenate BLAS nodes and primitives
+  std::vector<int> blasNodeOff(uniqueBlas.size()), blasPrimOff(uniqueBlas.size());
+  size_t nNodes = 0, nPrims = 0;
+  maxDepth = 0;
+  for (size_t k = 0; k < uniqueBlas.size(); k++) {
+    blasNodeOff[k] = (int)nNodes;
+    blasPrimOff[k] = (int)nPrims;
+    nNodes += uniqueBlas[k]->nodes.size();
+    nPrims += uniqueBlas[k]->prims.size();
+    maxDepth = std::max(maxDepth, uniqueBlas[k]->maxDepth);
+  }
+  // instances and TLAS
+  std::vector<hpt::DInstance> dinst(instances.size());
+  std::vector<hpt::BuildBox> instBoxes(instances.size());
+  for (size_t i = 0; i < instances.size(); i++) {
+    const Instance &I = *instances[i];
+    const int gi = groupIndex[I.group.p];
+    const Blas &B = *groupBlas[gi];
+    hpt::DInstance &di = dinst[i];
+    std::memset(&di, 0, sizeof(di));
+    const float *x = I.xfm;
+    const hpt::affine3f a = {{{x[0], x[1], x[2]}, {x[3], x[4], x[5]}, {x[6], x[7], x[8]}}, {x[9], x[10], x[11]}};
+    const hpt::affine3f r = hpt::inverse(a);
+    const float rv[12] = {r.l.vx.x, r.l.vx.y, r.l.vx.z, r.l.vy.x, r.l.vy.y, r.l.vy.z, r.l.vz.x, r.l.vz.y, r.l.vz.z, r.p.x, r.p.y, r.p.z};
+    std::memcpy(di.rcp, rv, sizeof(rv));
+    std::memcpy(di.xfm, x, 48);
+    di.blasRoot = blasNodeOff[blasSlot[&B]];
+    di.modelOffset = groupModelOffset[gi];
+    di.userID = I.userID;
+    di.epsScale = hpt::maxf(std::fabs(x[0]), hpt::maxf(std::fabs(x[4]), std::fabs(x[8])));
+    hpt::BuildBox wb;
+    for (int k = 0; k < 3; k++) {
+      wb.lo[k] = hpt::kInf;
+      wb.hi[k] = -hpt::kInf;
+    }
+    if (B.bounds.lo[0] <= B.bounds.hi[0]) {
+      for (int c = 0; c < 8; c++) {
+        const vec3f p = v3(c & 1 ? B.bounds.hi[0] : B.bounds.lo[0], c & 2 ? B.bounds.hi[1] : B.bounds.lo[1],
+            c & 4 ? B.bounds.hi[2] : B.bounds.lo[2]);
+        const vec3f q = hpt::mul(a.l, p) + a.p;
+        const float qq[3] = {q.x, q.y, q.z};
+        for (int k = 0; k < 3; k++) {
+          wb.lo[k] = std::min(wb.lo[k], qq[k]);
+          wb.hi[k] = std::max(wb.hi[k], qq[k]);
+        }
+      }
+      for (int k = 0; k < 3; k++) {
+        const float pad = (std::fabs(wb.lo[k]) + std::fabs(wb.hi[k])) * 1e-6f;
+        wb.lo[k] -= pad;
+        wb.hi[k] += pad;
+      }
+    }
+    instBoxes[i] = wb;
+  }
+  std::vector<uint32_t> tlasInst; // instances with geometry
+  std::vector<hpt::BuildBox> tlasBoxes;
+  for (size_t i = 0; i < instBoxes.size(); i++)
+    if (instBoxes[i].lo[0] <= instBoxes[i].hi[0]) {
+      tlasInst.push_back((uint32_t)i);
+      tlasBoxes.push_back(instBoxes[i]);
+    }
+  hpt::BuildResult T = hpt::buildBVH(tlasBoxes, 1, 1);
+  std::vector<uint32_t> tlasOrder(T.order.size());
+  for (size_t k = 0; k < T.order.size(); k++)
+    tlasOrder[k] = tlasInst[T.order[k]];
+  const int tlasOff = (int)nNodes;
+  std::vector<hpt::DNode> nodes(nNodes + T.nodes.size());
+  std::vector<hpt::DPrim> prims(nPrims);
+  auto remap = [](hpt::DNode n, int nodeOff, int primOff, const std::vector<uint32_t> *order) {
+    for (int c = 0; c < 2; c++) {
+      // an empty child (box at +inf) is never entered: leave it alone
+      if ((c ? n.b1lo[0] : n.b0lo[0]) == hpt::kInf)
+        continue;
+      int32_t ch = n.child[c];
+      if (ch >= 0) {
+        n.child[c] = ch + nodeOff;
+      } else {
+        const uint32_t enc = ~(uint32_t)ch;
+        uint32_t first = enc >> 3;
+        const uint32_t cnt = enc & 7u;
+        first = order ? (*order)[first] : first + (uint32_t)primOff;
+        n.child[c] = (int32_t) ~((first << 3) | cnt);
+      }
+    }
+    return n;
+  };
+  for (size_t k = 0; k < uniqueBlas.size(); k++) {
+    const Blas &B = *uniqueBlas[k];
+    for (size_t i = 0; i < B.nodes.size(); i++)
+      nodes[blasNodeOff[k] + i] = remap(B.nodes[i], blasNodeOff[k], blasPrimOff[k], nullptr);
+    std::copy(B.prims.begin(), B.prims.end(), prims.begin() + blasPrimOff[k]);
+  }
+  for (size_t i = 0; i < T.nodes.size(); i++)
+    nodes[tlasOff + i] = remap(T.nodes[i], tlasOff, 0, &tlasOrder);
+
+  // lights (PathTracerData.cpp:32-82): world lights, uniform selection CDF
+  std::vector<hpt::DLight> dl;
+  for (auto *l : lights)
+    dl.push_back(l->l);
+  std::vector<float> cdf(dl.size(), 1.f);
+  if (!cdf.empty())
+    hpt::distribution1DCreate((int)cdf.size(), cdf.data());
+
+  dNodes.upload(nodes);
+  dPrims.upload(prims);
+  dInstances.upload(dinst);
+  dModels.upload(models);
+  dMaterials.upload(dmats);
+  dMedia.upload(media);
+  dLights.upload(dl);
+  dLightsCDF.upload(cdf);
+  scene.nodes = dNodes.as<const hpt::DNode>();
+  scene.prims = dPrims.as<const hpt::DPrim>();
+  scene.instances = dInstances.as<const hpt::DInstance>();
+  scene.models = dModels.as<const hpt::DModel>();
+  scene.materials = dMaterials.as<const hpt::DMaterial>();
+  scene.media = dMedia.as<const hpt::DMedium>();
+  scene.lights = dLights.as<const hpt::DLight>();
+  scene.lightsCDF = dLightsCDF.as<const float>();
+  scene.tlasRoot = tlasOff;
+  scene.numInstances = (int)instances.size();
+  scene.numLights = (int)dl.size();
+  scene.numMaterials = (int)dmats.size();
+  totalNodes = (int64_t)nodes.size();
+  totalPrims = (int64_t)prims.size();
+  for (int k = 0; k < 3; k++) {
+    bounds[k] = T.bounds.lo[k];
+    bounds[3 + k] = T.bounds.hi[k];
+  }
+  // the stack holds at most one entry per level of TLAS and BLAS plus the
+  // instance marker
+  if (maxDepth + T.maxDepth + 2 > hpt_stack_capacity())
+    throw std::runtime_error("BVH too deep for the traversal stack");
+  maxDepth = std::max(maxDepth, T.maxDepth);
+  built = true;
+}
+
+// ---- frame orchestration -------------------------------------------------------------------
+
+void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
+{
+  maxRec = std::max(1, maxRec);
+  const bool grow = slots > pathSlots || (size_t)maxRec * slots > recSlots || (aux && !auxAllocated);
+  if (!grow)
+    return;
+  HIP_CHECK(hipStreamSynchronize(stream));
+  pathSlots = std::max(slots, pathSlots);
+  recSlots = std::max((size_t)maxRec * pathSlots, recSlots);
+  auxAllocated = auxAllocated || aux;
+  const size_t S = pathSlots, R = recSlots;
+  // per-slot arrays: 9 x float4 + hitPrim + ids(uint4) + neeN (+ aux 16 floats)
+  const size_t perSlot = 9 * 16 + 4 + 16 + 4 + (auxAllocated ? 64 : 0);
+  pathBuf.ensure(S * perSlot);
+  recBuf.ensure(R * 64);
+  queueBuf.ensure((2 * S + R + S) * 4);
+  char *b = (char *)pathBuf.ptr;
+  auto take = [&](size_t bytes) {
+    char *p = b;
+    b += bytes;
+    return p;
+  };
+  paths.org = (float *)take(S * 16);
+  paths.dir = (float *)take(S * 16);
+  paths.thr = (float *)take(S * 16);
+  paths.contrib = (float *)take(S * 16);
+  paths.ld = (uint32_t *)take(S * 16);
+  paths.lastP = (float *)take(S * 16);
+  paths.lastNs = (float *)take(S * 16);
+  paths.hitA = (float *)take(S * 16);
+  paths.hitB = (float *)take(S * 16);
+  paths.ids = (uint32_t *)take(S * 16);
+  paths.hitPrim = (uint32_t *)take(S * 4);
+  paths.neeN = (uint32_t *)take(S * 4);
+  auxBase = auxAllocated ? (float *)take(S * 64) : nullptr;
+  paths.aux = auxBase;
+  char *r = (char *)recBuf.ptr;
+  paths.recOrg = (float *)r;
+  paths.recDir = (float *)(r + R * 16);
+  paths.recContrib = (float *)(r + R * 32);
+  paths.recResult = (float *)(r + R * 48);
+  uint32_t *q = (uint32_t *)queueBuf.ptr;
+  paths.queue[0] = q;
+  paths.queue[1] = q + S;
+  paths.shadowQueue = q + 2 * S;
+  paths.neeQueue = q + 2 * S + R;
+  paths.counters = (int32_t *)counters.ptr;
+  paths.overflow = (uint32_t *)overflow.ptr;
+  paths.stats = (unsigned long long *)stats.ptr;
+}
+
+ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, ManagedObject *co, ManagedObject *wo)
+{
+  if (fbo->managedType != OSP_FRAMEBUFFER || ro->managedType != OSP_RENDERER || co->managedType != OSP_CAMERA
+      || wo->managedType != OSP_WORLD)
+    throw std::runtime_error("ospRenderFrame: wrong object types");
+  FrameBuffer *fb = (FrameBuffer *)fbo;
+  Renderer *r = (Renderer *)ro;
+  Camera *cam = (Camera *)co;
+  World *w = (World *)wo;
+  if (!w->built)
+    throw std::runtime_error("ospRenderFrame: the world has not been committed");
+  if (r->commitCounter == 0 || cam->commitCounter == 0)
+    throw std::runtime_error("ospRenderFrame: renderer and camera must be committed");
+
+  auto *fut = new Future(this);
+  fut->fb.reset(fbo);
+  fut->renderer.reset(ro);
+  fut->camera.reset(co);
+  fut->world.reset(wo);
+  HIP_CHECK(hipEventCreate(&fut->start));
+  HIP_CHECK(hipEventCreate(&fut->rendered));
+  HIP_CHECK(hipEventCreate(&fut->finished));
+  HIP_CHECK(hipEventRecord(fut->start, stream));
+
+  // FrameBuffer::beginFrame (FrameBuffer.cpp:115-137)
+  const bool doAccum = (fb->channels & OSP_FB_ACCUM) != 0;
+  fb->frameID = doAccum ? fb->frameID + 1 : 0;
+  if (fb->variance.ptr) {
+    const bool evenFrame = (fb->frameID & 1) == 0;
+    if (evenFrame)
+      fb->pickOdd = (fb->mtGen() & 1u) != 0;
+    fb->accumulateVariance = evenFrame != fb->pickOdd;
+  }
+  fb->errorThreshold = r->errorThreshold;
+  const bool finishedAccum = fb->targetFrames && fb->frameID >= fb->targetFrames;
+
+  hpt::DFrame F;
+  std::memset(&F, 0, sizeof(F));
+  F.width = fb->w;
+  F.height = fb->h;
+  F.tilesX = (fb->w + 7) / 8;
+  F.tilesY = (fb->h + 7) / 8;
+  F.numPixPadded = F.tilesX * F.tilesY * 64;
+  F.spp = r->spp;
+  F.accumID = (uint32_t)fb->frameID;
+  F.targetFrames = fb->targetFrames;
+  F.projectedDepth = fb->projectedDepth ? 1 : 0;
+  F.channels = fb->channels;
+  F.accumulateVariance = fb->accumulateVariance ? 1 : 0;
+  cam->setup(F);
+  F.maxDepth = r->maxDepth;
+  F.rouletteDepth = r->rouletteDepth;
+  F.maxScatteringEvents = r->maxScatteringEvents;
+  // light sample counts (PathTracer.ispc:95-110)
+  const uint32_t numLights = (uint32_t)w->scene.numLights;
+  F.numFirstLightSamples = r->numLightSamples >= 0 && numLights > 0 ? (uint32_t)r->numLightSamples
+                                                                   : std::min<uint32_t>(1024u, numLights);
+  F.numIndirectLightSamples = r->limitIndirect && F.numFirstLightSamples >= 1 ? 1u : F.numFirstLightSamples;
+  F.minContribution = r->minContribution;
+  F.maxRadiance = r->maxRadiance;
+  std::memcpy(F.bgColor, r->bg, 16);
+  F.filterType = r->filterType;
+  F.filterWidth = r->filterWidth;
+  F.lutSize = r->lutSize;
+  F.lutCdfX = r->dLutX.as<const float>();
+  F.lutCdfY = r->dLutY.as<const float>();
+  // LDSampler bits (PathTracer.ispc:128-147)
+  uint32_t bits = 16, bitsLight = 16;
+  if (F.targetFrames) {
+    const uint32_t targetSpp = (uint32_t)F.targetFrames * (uint32_t)F.spp;
+    auto clz = [](uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; };
+    bits = 16 + 32 - clz(targetSpp - 1);
+    bitsLight = 48 - clz(targetSpp * F.numFirstLightSamples - 1);
+  }
+  F.idxMask = bits > 31 ? 0xffffffffu : ~(0xffffffffu >> bits);
+  F.idxMaskLight = bitsLight > 31 ? 0xffffffffu : ~(0xffffffffu >> bitsLight);
+  F.accRGBA = fb->accRGBA.as<float>();
+  F.accAux = fb->accAux.as<float>();
+  F.accZ = fb->accZ.as<float>();
+  F.accIDs = fb->accIDs.as<uint32_t>();
+  F.fbColor = fb->color.as<float>();
+  F.fbVariance = fb->variance.as<float>();
+  F.fbDepth = fb->depth.as<float>();
+  F.fbNormal = fb->normal.as<float>();
+  F.fbAlbedo = fb->albedo.as<float>();
+  F.fbFirstNormal = fb->firstNormal.as<float>();
+  F.fbPosition = fb->position.as<float>();
+  F.fbPrimID = fb->primID.as<uint32_t>();
+  F.fbObjID = fb->objID.as<uint32_t>();
+  F.fbInstID = fb->instID.as<uint32_t>();
+  F.taskVariance = fb->taskVar.as<float>();
+  F.maxRec = (int)std::max<uint32_t>(1u, F.numFirstLightSamples);
+
+  if (!finishedAccum) {
+    const bool aux = F.accAux != nullptr;
+    const size_t perSlotBytes = 9 * 16 + 40 + (aux ? 64 : 0) + (size_t)F.maxRec * 68 + 12;
+    const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
+    const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
+    ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
+    paths.aux = aux ? auxBase : nullptr;
+    F.residentThreads = residentTraceBlocks * hpt_trace_block_size();
+    const HptScene *S = reinterpret_cast<const HptScene *>(&w->scene);
+    const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);
+    if (travStats)
+      HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 16, 0u, stream));
+    for (int s0 = 0; s0 < F.spp; s0 += sppPerPass) {
+      F.passS0 = s0;
+      F.passCount = std::min(sppPerPass, F.spp - s0);
+      const HptFrame *Fp = reinterpret_cast<const HptFrame *>(&F);
+      launch("reset", [&]() { return hpt_reset_counters(P, -1, stream); });
+      launch("generate", [&]() { return hpt_generate(Fp, P, stream); });
+      int q = 0;
+      for (int b = 0; b <= F.maxDepth; b++) {
+        launch("reset", [&]() { return hpt_reset_counters(P, q, stream); });
+        launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
+        launch("shade", [&]() { return hpt_shade(S, Fp, P, q, shadeBlocks, stream); });
+        if (F.numFirstLightSamples > 0) {
+          launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, stream); });
+          launch("resolve", [&]() { return hpt_resolve(Fp, P, shadeBlocks, stream); });
+        }
+        q ^= 1;
+      }
+      launch("finish_pass", [&]() { return hpt_finish_pass(Fp, P, stream); });
+    }
+    HIP_CHECK(hipEventRecord(fut->rendered, stream));
+    const HptFrame *Fp = reinterpret_cast<const HptFrame *>(&F);
+    launch("fb_accumulate", [&]() { return hpt_fb_accumulate(Fp, stream); });
+    // LocalFrameBuffer::postProcess (LocalFB.cpp:459-478)
+    if (fb->variance.ptr && (fb->frameID & 1) == 1) {
+      const int ntask = F.tilesX * F.tilesY;
+      launch("variance", [&]() { return hpt_variance(Fp, stream); });
+      launch("variance_reduce", [&]() {
+        return hpt_variance_reduce(F.taskVariance, ntask, fb->errorThreshold, fb->varianceOut.as<float>(), stream);
+      });
+      HIP_CHECK(hipMemcpyAsync(fb->hostVariance, fb->varianceOut.ptr, sizeof(float), hipMemcpyDeviceToHost, stream));
+      fb->varianceValid = true;
+    }
+    if (fb->color.ptr && fb->format != OSP_FB_NONE) {
+      launch("color_convert", [&]() {
+        return hpt_color_convert(F.fbColor, F.width * F.height, (int)fb->format, fb->converted.ptr, stream);
+      });
+    }
+  } else {
+    HIP_CHECK(hipEventRecord(fut->rendered, stream));
+  }
+  HIP_CHECK(hipEventRecord(fut->finished, stream));
+  return fut;
+}
+
+int HipDevice::traceRays(World *w, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2)
+{
+  if (!w->built)
+    throw std::runtime_error("world not committed");
+  DevBuf dr, dt, di;
+  dr.ensure((size_t)std::max(n, 1) * 32);
+  dt.ensure((size_t)std::max(n, 1) * 12);
+  di.ensure((size_t)std::max(n, 1) * 12);
+  HIP_CHECK(hipMemcpy(dr.ptr, rays8, (size_t)n * 32, hipMemcpyHostToDevice));
+  HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 16, 0u, stream));
+  HIP_CHECK(hpt_trace_rays(reinterpret_cast<const HptScene *>(&w->scene), dr.as<float>(), n, dt.as<float>(),
+      di.as<uint32_t>(), (uint32_t *)overflow.ptr, residentTraceBlocks, (unsigned long long *)stats.ptr, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+  HIP_CHECK(hipMemcpy(outTUV, dt.ptr, (size_t)n * 12, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(outIDs, di.ptr, (size_t)n * 12, hipMemcpyDeviceToHost));
+  if (stats2) {
+    unsigned long long s[2];
+    HIP_CHECK(hipMemcpy(s, stats.ptr, 16, hipMemcpyDeviceToHost));
+    stats2[0] = s[0];
+    stats2[1] = s[1];
+  }
+  return n;
+}
+
+OSPPickResult HipDevice::pick(ManagedObject *fbo, ManagedObject *, ManagedObject *co, ManagedObject *wo, float x, float y)
+{
+  // Renderer_pick (render/Renderer.ispc:43-95): camera ray through the
+  // screen position, closest hit
+  OSPPickResult res;
+  std::memset(&res, 0, sizeof(res));
+  res.primID = hpt::kInvalidID;
+  World *w = (World *)wo;
+  Camera *cam = (Camera *)co;
+  (void)fbo;
+  hpt::DFrame F;
+  std::memset(&F, 0, sizeof(F));
+  cam->setup(F);
+  const vec3f dir = hpt::normalize(hpt::ld3(F.camDir00) + x * hpt::ld3(F.camDu) + y * hpt::ld3(F.camDv));
+  const float ray[8] = {F.camOrg[0], F.camOrg[1], F.camOrg[2], dir.x, dir.y, dir.z, F.nearClip, hpt::kInf};
+  float tuv[3];
+  uint32_t ids[3];
+  traceRays(w, ray, 1, tuv, ids, nullptr);
+  if (ids[0] == hpt::kInvalidID)
+    return res;
+  Instance *inst = w->instances[ids[0]];
+  GeometricModel *model = inst->group->models[ids[1]];
+  inst->refInc();
+  model->refInc();
+  res.hasHit = 1;
+  res.instance = (OSPInstance)inst;
+  res.model = (OSPGeometricModel)model;
+  res.primID = ids[2];
+  for (int k = 0; k < 3; k++)
+    res.worldPosition[k] = F.camOrg[k] + tuv[0] * (&dir.x)[k];
+  return res;
+}
+
+static HipDevice *current()
+{
+  Device *d = ospray_current_device();
+  return dynamic_cast<HipDevice *>(d);
+}
+
+} // namespace hipdev
+
+using namespace hipdev;
+
+extern "C" {
+
+// module entry point (modules/cpu/ModuleInit.cpp:24-54 pattern)
+OSPRAY_INTERFACE OSPError ospray_module_init_hip(int16_t major, int16_t minor, int16_t patch)
+{
+  (void)patch;
+  if (major != OSPRAY_VERSION_MAJOR || minor != OSPRAY_VERSION_MINOR)
+    return OSP_VERSION_MISMATCH;
+  ospray_register_device_type("hip", []() -> Device * { return new HipDevice(); });
+  return OSP_NO_ERROR;
+}
+
+OSPRAY_INTERFACE int ospHipGetKernelStats(OSPHipKernelStat *out, int maxEntries)
+{
+  HipDevice *d = current();
+  if (!d)
+    return -1;
+  d->collectTimers();
+  int n = 0;
+  for (auto &t : d->timers) {
+    if (n >= maxEntries)
+      break;
+    std::memset(out[n].name, 0, sizeof(out[n].name));
+    std::strncpy(out[n].name, t.name.c_str(), sizeof(out[n].name) - 1);
+    out[n].launches = t.launches;
+    out[n].totalMs = t.totalMs;
+    n++;
+  }
+  return n;
+}
+
+OSPRAY_INTERFACE int ospHipResetKernelStats(void)
+{
+  HipDevice *d = current();
+  if (!d)
+    return -1;
+  d->collectTimers();
+  d->timers.clear();
+  return 0;
+}
+
+OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out3[3])
+{
+  HipDevice *d = current();
+  if (!d)
+    return -1;
+  try {
+    unsigned long long s[3];
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    HIP_CHECK(hipMemcpy(s, d->stats.ptr, 24, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; k++)
+      out3[k] = s[k];
+  } catch (const std::exception &) {
+    return -2;
+  }
+  return 0;
+}
+
+OSPRAY_INTERFACE int ospHipTraceRays(OSPWorld world, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2)
+{
+  HipDevice *d = current();
+  if (!d || !world)
+    return -1;
+  try {
+    return d->traceRays((World *)world, rays8, n, outTUV, outIDs, stats2);
+  } catch (const std::exception &e) {
+    d->lastErrorCode = OSP_UNKNOWN_ERROR;
+    d->lastErrorMsg = e.what();
+    return -2;
+  }
+}
+
+OSPRAY_INTERFACE int ospHipGetDeviceInfo(char *name, int len, int *numCUs)
+{
+  HipDevice *d = current();
+  if (!d)
+    return -1;
+  if (name && len > 0) {
+    std::strncpy(name, d->gpuName.c_str(), len - 1);
+    name[len - 1] = 0;
+  }
+  if (numCUs)
+    *numCUs = d->numCUs;
+  return 0;
+}
+
+OSPRAY_INTERFACE int ospHipGetWorldInfo(OSPWorld world, int64_t out4[4])
+{
+  HipDevice *d = current();
+  if (!d || !world)
+    return -1;
+  World *w = (World *)world;
+  out4[0] = w->totalNodes;
+  out4[1] = w->totalPrims;
+  out4[2] = w->maxDepth;
+  out4[3] = w->scene.numInstances;
+  return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,864 @@
+// Wavefront path-tracing kernels for gfx950 and their C-ABI launch layer.
+//
+// One frame = for each pass of samples: generate -> (extend -> shade ->
+// shadow -> resolve) x (maxPathLength+1) -> finish_pass; then fb_accumulate,
+// variance (odd frames) and colour conversion.  This splits the reference's
+// per-pixel megakernel (PathTracer_renderTask, PathTracer.ispc:218-249;
+// samplePath, PathSampler.ispc:109-415) into coherent stages:
+//   extend  closest-hit traversal only (LDS stack, lean registers);
+//   shade   post-intersect, virtual lights, BSDF, NEE light samples (emits
+//           shadow-ray records and the next extension ray, compacted with
+//           one wave-wide ballot + atomic per queue);
+//   shadow  occlusion with opaque early exit, then the exact closest-hit
+//           transparency loop (TransparentShadow.ispc:18-85) when only
+//           non-opaque surfaces are in the way;
+//   resolve adds the NEE samples of a vertex in light-sample order, so the
+//           float sums match the reference's sequential order.
+// Queues live in HBM; counts are read by the kernels themselves, so no host
+// synchronisation is needed between stages (grid-stride loops over a fixed
+// resident grid).
+#include <hip/hip_runtime.h>
+
+#include "ospray_hip/hip_pt.h"
+#include "pt_shade.h"
+
+using namespace hpt;
+
+namespace {
+
+__device__ __forceinline__ int laneId() { return (int)__lane_id(); }
+
+// wave-aggregated queue append: one atomic per wave
+__device__ __forceinline__ int enqueue(int32_t *counter, bool pred)
+{
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0)
+    return -1;
+  const int lane = laneId();
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader)
+    base = atomicAdd(counter, __popcll(mask));
+  base = __shfl(base, leader);
+  return pred ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
+}
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ uint4 ld4u(const uint32_t *p) { return *reinterpret_cast<const uint4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
+{
+  *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+}
+__device__ __forceinline__ void st4u(uint32_t *p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+  *reinterpret_cast<uint4 *>(p) = make_uint4(a, b, c, d);
+}
+
+__device__ __forceinline__ int gridStride() { return (int)(gridDim.x * blockDim.x); }
+__device__ __forceinline__ int globalId() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
+
+// path slot <-> pixel: slots are 8x8 tiles, one wave per tile and sample
+__device__ __forceinline__ void slotPixel(const DFrame &F, int q, int &ix, int &iy)
+{
+  const int tile = q >> 6, w = q & 63;
+  ix = (tile % F.tilesX) * 8 + (w & 7);
+  iy = (tile / F.tilesX) * 8 + (w >> 3);
+}
+__device__ __forceinline__ int pixelSlot(const DFrame &F, int ix, int iy)
+{
+  return ((iy >> 3) * F.tilesX + (ix >> 3)) * 64 + (iy & 7) * 8 + (ix & 7);
+}
+
+__device__ __forceinline__ uint32_t clz32(uint32_t x) { return x ? (uint32_t)__clz((int)x) : 32u; }
+
+// pixel filter (PixelFilterDispatch.ispc:7-19, LUTPixelFilter.ispc:9-15,
+// Distribution2D.ispc:9-25)
+__device__ vec2f pixelFilterSample(const DFrame &F, vec2f s)
+{
+  if (F.filterType == 0)
+    return v2(0.f, 0.f);
+  if (F.filterType == 1)
+    return v2((s.x + -0.5f) * F.filterWidth, (s.y + -0.5f) * F.filterWidth);
+  const int n = F.lutSize;
+  const Sample1D sy = distribution1DSample(n, F.lutCdfY, 0, s.y);
+  const Sample1D sx = distribution1DSample(n, F.lutCdfX, sy.idx * n, s.x);
+  const float rcpN = 1.f / (float)n;
+  const vec2f uv = v2(((float)sx.idx + sx.frac) * rcpN, ((float)sy.idx + sy.frac) * rcpN);
+  return v2((uv.x + -0.5f) * F.filterWidth, (uv.y + -0.5f) * F.filterWidth);
+}
+
+} // namespace
+
+// ---- kernels ----------------------------------------------------------------------
+
+__global__ void k_reset(int32_t *counters, int n, int32_t keep)
+{
+  const int i = globalId();
+  if (i < n && i != keep)
+    counters[i] = 0;
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
+{
+  for (size_t i = globalId(); i < n; i += gridStride())
+    p[i] = v;
+}
+
+// PathTracer_renderPixel up to the first samplePath iteration
+// (PathTracer.ispc:75-170) for samples [passS0, passS0+passCount)
+__global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
+{
+  const int total = F.passCount * F.numPixPadded;
+  for (int i = globalId(); i < total; i += gridStride()) {
+    const int sLocal = i / F.numPixPadded;
+    const int q = i - sLocal * F.numPixPadded;
+    int ix, iy;
+    slotPixel(F, q, ix, iy);
+    const bool valid = ix < F.width && iy < F.height;
+    if (valid) {
+      const uint32_t spp = (uint32_t)F.spp;
+      const uint32_t s = (uint32_t)(F.passS0 + sLocal);
+      uint32_t seed = 0x49c5f37eu;
+      uint32_t sampleOffs = 0, sampleOffsLight = 0;
+      if (F.targetFrames) {
+        const uint32_t targetSpp = (uint32_t)F.targetFrames * spp;
+        sampleOffs = OwenScramble4(interleaveZOrder16((uint32_t)ix, (uint32_t)iy), seed) * targetSpp;
+        sampleOffsLight = sampleOffs * F.numFirstLightSamples;
+      } else {
+        seed ^= (uint32_t)F.width * (uint32_t)iy + (uint32_t)ix;
+      }
+      const uint32_t sampleID = F.accumID * spp + s;
+      // LDSampler_init (random.ih:283-294); the index masks are frame-uniform
+      LDSampler lds;
+      lds.revIndex = reverseBits(sampleOffs + sampleID);
+      lds.scramble = seed;
+      lds.baseIndexLight = sampleOffsLight + sampleID * F.numFirstLightSamples;
+      lds.idxMask = F.idxMask;
+      lds.idxMaskLight = F.idxMaskLight;
+      const vec2f center = v2(((float)ix + 0.5f) * (1.f / (float)F.width), ((float)iy + 0.5f) * (1.f / (float)F.height));
+      const vec2f pixelSample = ldGetNext5Samples(lds);
+      const vec2f pf = pixelFilterSample(F, pixelSample);
+      const vec2f screen = v2(center.x + pf.x * (1.f / (float)F.width), center.y + pf.y * (1.f / (float)F.height));
+      // PerspectiveCamera_initRay (PerspectiveCamera.ispc:16-64), default image region
+      const float sx = (1.f - screen.x) * 0.f + screen.x * 1.f;
+      const float sy = (1.f - screen.y) * 0.f + screen.y * 1.f;
+      const vec3f dir = normalize(ld3(F.camDir00) + sx * ld3(F.camDu) + sy * ld3(F.camDv));
+      const vec3f org = ld3(F.camOrg);
+      st4(P.org + 4 * (size_t)i, org.x, org.y, org.z, F.nearClip);
+      st4(P.dir + 4 * (size_t)i, dir.x, dir.y, dir.z, 0.f);
+      st4(P.thr + 4 * (size_t)i, 1.f, 1.f, 1.f, 1.f);
+      st4(P.contrib + 4 * (size_t)i, 0.f, 0.f, 0.f, kInf);
+      st4u(P.ld + 4 * (size_t)i, lds.revIndex, lds.scramble, lds.baseIndexLight,
+          FLAG_FIRST_BOUNCE_LIGHT | FLAG_STRAIGHT | FLAG_SPEC_TRANS_PATH);
+      st4(P.lastP + 4 * (size_t)i, org.x, org.y, org.z, kInf);
+      st4(P.lastNs + 4 * (size_t)i, dir.x, dir.y, dir.z, __uint_as_float(0u));
+      st4u(P.ids + 4 * (size_t)i, kInvalidID, kInvalidID, kInvalidID, 0u);
+      if (P.aux) {
+        float *a = P.aux + 16 * (size_t)i;
+        for (int k = 0; k < 16; k += 4)
+          st4(a + k, 0.f, 0.f, 0.f, 0.f);
+      }
+    }
+    const int slot = enqueue(&P.counters[0], valid);
+    if (valid)
+      P.queue[0][slot] = (uint32_t)i;
+  }
+}
+
+// closest hit of every queued path (the rtcIntersectV of
+// traceGeometryRayIntervals, World.ih:247-286)
+template <bool STATS>
+__global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int qin)
+{
+  __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
+  const int n = P.counters[qin];
+  uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
+  TraceStats st = {0, 0};
+  uint32_t rays = 0;
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t p = P.queue[qin][i];
+    const float4 o = ld4(P.org + 4 * (size_t)p);
+    const float4 d = ld4(P.dir + 4 * (size_t)p);
+    Hit h;
+    bool sawT;
+    traceRay<MODE_CLOSEST, STATS>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds, ovf, sawT, st);
+    st4(P.hitA + 4 * (size_t)p, h.hit ? h.t : kInf, h.u, h.v, __uint_as_float(h.hit ? h.inst : kInvalidID));
+    st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
+    P.hitPrim[p] = h.prim | (h.half << 31);
+    rays++;
+  }
+  if (STATS) {
+    atomicAdd(&P.stats[0], (unsigned long long)st.nodes);
+    atomicAdd(&P.stats[1], (unsigned long long)st.prims);
+    atomicAdd(&P.stats[2], (unsigned long long)rays);
+  }
+}
+
+// one iteration of samplePath (PathSampler.ispc:143-407) for every queued
+// path; NEE (NextEventEstimation.ispc:25-160) emits shadow-ray records
+__global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int qin)
+{
+  const int n = P.counters[qin];
+  const int qout = qin ^ 1;
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t p = P.queue[qin][i];
+    const size_t p4 = 4 * (size_t)p;
+    const float4 o4 = ld4(P.org + p4);
+    const float4 d4 = ld4(P.dir + p4);
+    float4 thr4 = ld4(P.thr + p4);
+    float4 con4 = ld4(P.contrib + p4);
+    uint4 ld4v = ld4u(P.ld + p4);
+    const float4 lp4 = ld4(P.lastP + p4);
+    const float4 ln4 = ld4(P.lastNs + p4);
+    const float4 hA = ld4(P.hitA + p4);
+    const float4 hB = ld4(P.hitB + p4);
+    const uint32_t hp = P.hitPrim[p];
+
+    vec3f rorg = v3(o4.x, o4.y, o4.z);
+    const vec3f rdir = v3(d4.x, d4.y, d4.z);
+    vec3f thr = v3(thr4.x, thr4.y, thr4.z);
+    float alpha = thr4.w;
+    vec3f contribution = v3(con4.x, con4.y, con4.z);
+    uint32_t flags = ld4v.w;
+    const uint32_t depth = flags & FLAG_DEPTH_MASK;
+    uint32_t scat = (flags >> FLAG_SCAT_SHIFT) & 0xffu;
+    uint32_t medium = flags >> FLAG_MEDIUM_SHIFT;
+    LDSampler lds;
+    lds.revIndex = ld4v.x;
+    lds.scramble = ld4v.y;
+    lds.baseIndexLight = ld4v.z;
+    lds.idxMask = F.idxMask;
+    lds.idxMaskLight = F.idxMaskLight;
+
+    Hit hit;
+    hit.hit = __float_as_uint(hA.w) != kInvalidID;
+    hit.t = hA.x;
+    hit.u = hA.y;
+    hit.v = hA.z;
+    hit.inst = __float_as_uint(hA.w);
+    hit.Ng = v3(hB.x, hB.y, hB.z);
+    hit.model = __float_as_uint(hB.w);
+    hit.prim = hp & 0x7fffffffu;
+    hit.half = hp >> 31;
+    const float rayT = hit.hit ? hit.t : kInf;
+
+    const bool straight = (flags & FLAG_STRAIGHT) != 0;
+    if (depth == 0)
+      con4.w = F.projectedDepth ? rayT * dot(rdir, ld3(F.camDirection)) : rayT;
+
+    // background (PathSampler.ispc:211-223)
+    if (!hit.hit && straight) {
+      alpha -= luminance(thr) * (1.0f - F.bgColor[3]);
+      const vec3f bg3 = thr * v3(F.bgColor[0], F.bgColor[1], F.bgColor[2]);
+      contribution = contribution + bg3;
+    }
+    // evaluateVirtualLights (VirtualLight.ispc:38-107)
+    if (S.numLights > 0) {
+      const vec3f lastP = v3(lp4.x, lp4.y, lp4.z);
+      const vec3f lastNs = v3(ln4.x, ln4.y, ln4.z);
+      const float lastPdf = lp4.w;
+      const uint32_t lastNLS = __float_as_uint(ln4.w);
+      vec3f L = v3(0.f);
+      const float lower = distance(lastP, rorg);
+      const float upper = !hit.hit ? kInf : distance(lastP, rorg + rayT * rdir);
+      const float selectionPDF = (float)lastNLS / (float)S.numLights;
+      for (int li = 0; li < S.numLights; li++) {
+        const DLight &l = S.lights[li];
+        if (!straight || l.visible) {
+          const LightEval le = lightEval(l, lastP, lastNs, rdir, lower, upper);
+          if (reduce_max(le.radiance) > 0.0f) {
+            float mis = 1.0f;
+            if (selectionPDF != 0.f)
+              mis = misHeuristic(lastPdf, selectionPDF * le.pdf);
+            L = L + thr * le.radiance * mis;
+          }
+        }
+      }
+      contribution = contribution + L;
+    }
+
+    bool cont = false;
+    vec3f newOrg = rorg, newDir = rdir;
+    float newLastPdf = lp4.w;
+    vec3f newLastP = v3(lp4.x, lp4.y, lp4.z), newLastNs = v3(ln4.x, ln4.y, ln4.z);
+    uint32_t newLastNLS = __float_as_uint(ln4.w);
+    uint32_t nNee = 0;
+    do {
+      if (!hit.hit)
+        break;
+      if (depth >= (uint32_t)F.maxDepth)
+        break;
+      DG dg;
+      postIntersect(S, rorg, rdir, hit, dg, true);
+      if (depth == 0) {
+        P.ids[p4 + 0] = dg.primID;
+        P.ids[p4 + 1] = dg.objID;
+        P.ids[p4 + 2] = dg.instID;
+        if (P.aux) {
+          float *a = P.aux + 16 * (size_t)p;
+          a[0] = dg.P.x;
+          a[1] = dg.P.y;
+          a[2] = dg.P.z;
+          a[3] = dg.Ns.x;
+          a[4] = dg.Ns.y;
+          a[5] = dg.Ns.z;
+        }
+      }
+      if (depth >= (uint32_t)F.maxDepth || scat >= (uint32_t)F.maxScatteringEvents)
+        break;
+      BSDF b;
+      if (!getBSDF(S, dg, S.media[medium], b))
+        break;
+      ldNextGroup(lds);
+      const vec3f wo = neg(rdir);
+      uint32_t pvNLS = 0;
+      if (b.scatteringType & S_SMOOTH) {
+        if (scat == 0 && P.aux) {
+          float *a = P.aux + 16 * (size_t)p;
+          const vec3f nn = thr * b.frame.vz;
+          const vec3f al = thr * b.albedo;
+          a[6] = nn.x;
+          a[7] = nn.y;
+          a[8] = nn.z;
+          a[9] = al.x;
+          a[10] = al.y;
+          a[11] = al.z;
+        }
+        const uint32_t nls = (flags & FLAG_FIRST_BOUNCE_LIGHT) ? F.numFirstLightSamples : F.numIndirectLightSamples;
+        pvNLS = nls;
+        nNee = nls;
+        for (uint32_t li = 0; li < nls; li++) {
+          const size_t r = (size_t)p * F.maxRec + li;
+          float ss;
+          const vec2f s = ldGet3LightSamples(lds, li, nls > 1, ss);
+          const Sample1D sel = distribution1DSample(S.numLights, S.lightsCDF, 0, ss);
+          const float selProb = sel.prob * (float)nls;
+          LightSample ls = lightSample(S.lights[sel.idx], dg.P, dg.Ng, dg.Ns, s);
+          ls.weight = ls.weight / selProb;
+          ls.pdf *= selProb;
+          st4(P.recResult + 4 * r, 0.f, 0.f, 0.f, 0.f);
+          bool emit = !(reduce_max(ls.weight) <= 0.0f || ls.pdf <= 0.0f);
+          EvalRes fe = {v3(0.f), 0.f};
+          if (emit) {
+            fe = bsdfEval(b, wo, ls.dir);
+            emit = !(reduce_max(fe.value) <= 0.0f);
+          }
+          if (emit) {
+            vec3f org = dg.P;
+            if (dot(dg.Ng, ls.dir) < 0.f)
+              org = org - (2.0f * dg.epsilon) * dg.Ng;
+            const vec3f throughput = thr * fe.value;
+            const vec3f unshaded = throughput * ls.weight;
+            st4(P.recOrg + 4 * r, org.x, org.y, org.z, ls.dist);
+            st4(P.recDir + 4 * r, ls.dir.x, ls.dir.y, ls.dir.z, misHeuristic(ls.pdf, fe.pdf));
+            st4(P.recContrib + 4 * r, unshaded.x, unshaded.y, unshaded.z, __uint_as_float(medium));
+          }
+          const int slot = enqueue(&P.counters[2], emit);
+          if (emit)
+            P.shadowQueue[slot] = (uint32_t)r;
+        }
+        flags &= ~FLAG_FIRST_BOUNCE_LIGHT;
+      }
+      // sampleDirection (PathSampler.ispc:72-107)
+      float ss;
+      uint32_t uu3;
+      const vec2f s = ldGetNext4Samples(lds, ss, uu3);
+      SampleRes fs = bsdfSample(b, wo, s, ss);
+      {
+        const bool sameSide = dot(fs.wi, dg.Ng) >= 0.f;
+        const bool reflection = (fs.type & S_REFLECTION) != 0;
+        if (sameSide != reflection)
+          fs.weight = v3(0.0f);
+      }
+      if (reduce_max(fs.weight) <= 0.0f || fs.pdf <= 0.0f)
+        break;
+      thr = thr * fs.weight;
+      if (depth >= (uint32_t)F.rouletteDepth) {
+        const float rr = ldFinalizeDim3(lds, uu3);
+        const float contProb = minf(reduce_max(thr), kMaxRouletteContProb);
+        if (rr > contProb)
+          break;
+        thr = thr * rcp(contProb);
+      }
+      const DMedium &cm = S.media[medium];
+      if (minf(minf(cm.attenuation[0], cm.attenuation[1]), cm.attenuation[2]) < 0.f) {
+        const vec3f a = ld3(cm.attenuation) * rayT;
+        thr = thr * v3(fexp(a.x), fexp(a.y), fexp(a.z));
+      }
+      vec3f rayOrg = dg.P;
+      if (fs.type & S_TRANSMISSION) {
+        rayOrg = rayOrg - (2.0f * dg.epsilon) * dg.Ng;
+        medium = selectNextMedium(S, dg, medium);
+      }
+      if (fs.type & ~SPEC_TRANS) {
+        newLastP = dg.P;
+        newLastNs = dg.Ns;
+        newLastPdf = fs.pdf;
+        newLastNLS = pvNLS;
+      }
+      if (!eq(rdir, fs.wi))
+        flags &= ~FLAG_STRAIGHT;
+      if (!(fs.type & SPEC_TRANS))
+        flags &= ~FLAG_SPEC_TRANS_PATH;
+      newOrg = rayOrg;
+      newDir = fs.wi;
+      const uint32_t nd = depth + 1;
+      if (fs.type & S_SMOOTH)
+        scat++;
+      flags = (flags & ~(FLAG_DEPTH_MASK | (0xffu << FLAG_SCAT_SHIFT) | (0xfffu << FLAG_MEDIUM_SHIFT))) | nd
+          | (scat << FLAG_SCAT_SHIFT) | (medium << FLAG_MEDIUM_SHIFT);
+      cont = reduce_max(thr) > F.minContribution;
+    } while (false);
+
+    st4(P.thr + p4, thr.x, thr.y, thr.z, alpha);
+    st4(P.contrib + p4, contribution.x, contribution.y, contribution.z, con4.w);
+    if (cont) {
+      st4(P.org + p4, newOrg.x, newOrg.y, newOrg.z, 0.f);
+      st4(P.dir + p4, newDir.x, newDir.y, newDir.z, 0.f);
+      st4u(P.ld + p4, lds.revIndex, lds.scramble, lds.baseIndexLight, flags);
+      st4(P.lastP + p4, newLastP.x, newLastP.y, newLastP.z, newLastPdf);
+      st4(P.lastNs + p4, newLastNs.x, newLastNs.y, newLastNs.z, __uint_as_float(newLastNLS));
+    }
+    if (nNee)
+      P.neeN[p] = nNee;
+    const int so = enqueue(&P.counters[qout], cont);
+    if (cont)
+      P.queue[qout][so] = p;
+    const int sn = enqueue(&P.counters[3], nNee > 0);
+    if (nNee > 0)
+      P.neeQueue[sn] = p;
+  }
+}
+
+// transparentShadow (TransparentShadow.ispc:18-85) per NEE record
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPaths P)
+{
+  __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
+  const int n = P.counters[2];
+  uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
+  TraceStats st = {0, 0};
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t r = P.shadowQueue[i];
+    const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
+    const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
+    const float4 c4 = ld4(P.recContrib + 4 * (size_t)r);
+    const vec3f org = v3(o4.x, o4.y, o4.z), dir = v3(d4.x, d4.y, d4.z);
+    vec3f lc = v3(c4.x, c4.y, c4.z);
+    uint32_t medium = __float_as_uint(c4.w);
+    Hit h;
+    bool sawTransparent;
+    const bool occluded = traceRay<MODE_OCCLUSION, false>(S, org, dir, 0.f, o4.w, h, lds, ovf, sawTransparent, st);
+    if (occluded)
+      continue;
+    if (sawTransparent) {
+      // only non-opaque surfaces in the way: exact closest-hit loop
+      uint32_t maxDepth = (uint32_t)F.maxDepth;
+      const float tOriginal = o4.w;
+      float t0 = 0.f;
+      for (;;) {
+        bool dummy;
+        traceRay<MODE_CLOSEST, false>(S, org, dir, t0, tOriginal, h, lds, ovf, dummy, st);
+        if (!h.hit)
+          break;
+        DG dg;
+        postIntersect(S, org, dir, h, dg, false);
+        const vec3f tr = getTransparency(S, dg, dir, S.media[medium]);
+        lc = lc * tr;
+        const DMedium &cm = S.media[medium];
+        if (!(cm.attenuation[0] == 0.f && cm.attenuation[1] == 0.f && cm.attenuation[2] == 0.f)) {
+          const vec3f a = ld3(cm.attenuation) * (h.t - t0);
+          lc = lc * v3(fexp(a.x), fexp(a.y), fexp(a.z));
+        }
+        if (reduce_max(lc) <= F.minContribution || --maxDepth <= 0) {
+          lc = v3(0.f);
+          break;
+        }
+        medium = selectNextMedium(S, dg, medium);
+        t0 = h.t + dg.epsilon;
+      }
+    }
+    if (reduce_max(lc) > 0) {
+      const vec3f res = lc * d4.w;
+      st4(P.recResult + 4 * (size_t)r, res.x, res.y, res.z, 1.f);
+    }
+  }
+}
+
+// sum of the NEE samples of a vertex in light-sample order, added to the
+// path contribution (NextEventEstimation.ispc:150-159, PathSampler.ispc:338)
+__global__ __launch_bounds__(256) void k_resolve(DFrame F, DPaths P)
+{
+  const int n = P.counters[3];
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t p = P.neeQueue[i];
+    const uint32_t nls = P.neeN[p];
+    vec3f L = v3(0.f);
+    for (uint32_t k = 0; k < nls; k++) {
+      const float4 r = ld4(P.recResult + 4 * ((size_t)p * F.maxRec + k));
+      if (r.w != 0.f)
+        L = L + v3(r.x, r.y, r.z);
+    }
+    float *c = P.contrib + 4 * (size_t)p;
+    c[0] = c[0] + L.x;
+    c[1] = c[1] + L.y;
+    c[2] = c[2] + L.z;
+  }
+}
+
+// end of samplePath + ScreenSample_accumulate over the pass's samples
+// (PathSampler.ispc:409-414, PathTracer.ispc:196-197, ScreenSample.ih:59-77)
+__global__ __launch_bounds__(256) void k_finish_pass(DFrame F, DPaths P)
+{
+  const int npix = F.width * F.height;
+  for (int pix = globalId(); pix < npix; pix += gridStride()) {
+    const int ix = pix % F.width, iy = pix / F.width;
+    const int q = pixelSlot(F, ix, iy);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float aux[12];
+    for (int k = 0; k < 12; k++)
+      aux[k] = 0.f;
+    float z = kInf;
+    uint32_t ids[3] = {kInvalidID, kInvalidID, kInvalidID};
+    if (F.passS0 > 0) {
+      acc = ld4(F.accRGBA + 4 * (size_t)pix);
+      if (F.accAux)
+        for (int k = 0; k < 12; k++)
+          aux[k] = F.accAux[12 * (size_t)pix + k];
+      z = F.accZ[pix];
+      for (int k = 0; k < 3; k++)
+        ids[k] = F.accIDs[3 * (size_t)pix + k];
+    }
+    for (int sl = 0; sl < F.passCount; sl++) {
+      const size_t p = (size_t)sl * F.numPixPadded + q;
+      const float4 c = ld4(P.contrib + 4 * p);
+      float alpha = P.thr[4 * p + 3];
+      vec3f rgb = v3(c.x, c.y, c.z);
+      if (rgb.x != rgb.x || rgb.y != rgb.y || rgb.z != rgb.z) {
+        rgb = v3(0.f);
+        alpha = 1.0f;
+      }
+      rgb = vmin(rgb, v3(F.maxRadiance));
+      acc.x = acc.x + rgb.x;
+      acc.y = acc.y + rgb.y;
+      acc.z = acc.z + rgb.z;
+      acc.w = acc.w + alpha;
+      if (F.accAux) {
+        const float *a = P.aux + 16 * p;
+        for (int k = 0; k < 12; k++)
+          aux[k] = aux[k] + a[k];
+      }
+      if (fabsf(c.w) < fabsf(z)) {
+        z = c.w;
+        ids[0] = P.ids[4 * p + 0];
+        ids[1] = P.ids[4 * p + 1];
+        ids[2] = P.ids[4 * p + 2];
+      }
+    }
+    st4(F.accRGBA + 4 * (size_t)pix, acc.x, acc.y, acc.z, acc.w);
+    if (F.accAux)
+      for (int k = 0; k < 12; k++)
+        F.accAux[12 * (size_t)pix + k] = aux[k];
+    F.accZ[pix] = z;
+    for (int k = 0; k < 3; k++)
+      F.accIDs[3 * (size_t)pix + k] = ids[k];
+  }
+}
+
+__device__ __forceinline__ float lerpAcc(float old, float nw, float f) { return (1.f - f) * old + f * nw; }
+
+// ScreenSample_normalize + LocalFB_accumulateSample (ScreenSample.ih:79-92,
+// fb/LocalFB.ispc:14-78)
+__global__ __launch_bounds__(256) void k_fb_accumulate(DFrame F)
+{
+  const int npix = F.width * F.height;
+  const float rspp = 1.0f / (float)F.spp;
+  const int accumID = (int)F.accumID;
+  const float accScale = 1.0f / (float)(accumID + 1);
+  const float accVarScale = 1.0f / (float)(accumID / 2 + 1);
+  for (int pix = globalId(); pix < npix; pix += gridStride()) {
+    const float4 a = ld4(F.accRGBA + 4 * (size_t)pix);
+    const float r = a.x * rspp, g = a.y * rspp, b = a.z * rspp, al = a.w * rspp;
+    if (F.fbColor) {
+      float *c = F.fbColor + 4 * (size_t)pix;
+      if (accScale > 0.8f)
+        st4(c, r, g, b, al);
+      else {
+        const float4 old = ld4(c);
+        st4(c, lerpAcc(old.x, r, accScale), lerpAcc(old.y, g, accScale), lerpAcc(old.z, b, accScale),
+            lerpAcc(old.w, al, accScale));
+      }
+    }
+    if (F.fbVariance && F.accumulateVariance) {
+      float *c = F.fbVariance + 4 * (size_t)pix;
+      if (accVarScale > 0.8f)
+        st4(c, r, g, b, al);
+      else {
+        const float4 old = ld4(c);
+        st4(c, lerpAcc(old.x, r, accVarScale), lerpAcc(old.y, g, accVarScale), lerpAcc(old.z, b, accVarScale),
+            lerpAcc(old.w, al, accVarScale));
+      }
+    }
+    const float z = F.accZ[pix];
+    bool updateIDs = accumID == 0;
+    if (F.fbDepth) {
+      if (accumID > 0 && fabsf(z) < fabsf(F.fbDepth[pix]))
+        updateIDs = true;
+      if (updateIDs)
+        F.fbDepth[pix] = z;
+    }
+    if (updateIDs) {
+      if (F.fbPrimID)
+        F.fbPrimID[pix] = F.accIDs[3 * (size_t)pix];
+      if (F.fbObjID)
+        F.fbObjID[pix] = F.accIDs[3 * (size_t)pix + 1];
+      if (F.fbInstID)
+        F.fbInstID[pix] = F.accIDs[3 * (size_t)pix + 2];
+    }
+    float *dsts[4] = {F.fbPosition, F.fbFirstNormal, F.fbNormal, F.fbAlbedo};
+    for (int k = 0; k < 4; k++) {
+      float *dst = dsts[k];
+      if (!dst)
+        continue;
+      for (int c = 0; c < 3; c++) {
+        const float v = F.accAux[12 * (size_t)pix + 3 * k + c] * rspp;
+        float *d = dst + 3 * (size_t)pix + c;
+        *d = accScale > 0.8f ? v : lerpAcc(*d, v, accScale);
+      }
+    }
+  }
+}
+
+// Variance_kernel (fb/frame_ops/Variance.ispc:11-59), one thread per task
+__global__ __launch_bounds__(256) void k_variance(DFrame F)
+{
+  const int tx = (F.width + 7) / 8, ty = (F.height + 7) / 8;
+  const int ntask = tx * ty;
+  for (int t = globalId(); t < ntask; t += gridStride()) {
+    const int bx = (t % tx) * 8, by = (t / tx) * 8;
+    float v = 0.f;
+    int cnt = 0;
+    for (int y = 0; y < 8; y++) {
+      const int fy = by + y;
+      if (fy >= F.height)
+        continue;
+      for (int x = 0; x < 8; x++) {
+        const int fx = bx + x;
+        if (fx >= F.width)
+          continue;
+        const size_t id = (size_t)fy * F.width + fx;
+        const float4 c = ld4(F.fbColor + 4 * id);
+        const float4 va = ld4(F.fbVariance + 4 * id);
+        const float den2 = (c.x + c.y + c.z) + (1.f - c.w);
+        if (den2 > 0.f) {
+          cnt++;
+          const float diff = fabsf(c.x - va.x) + fabsf(c.y - va.y) + fabsf(c.z - va.z) + fabsf(c.w - va.w);
+          v += diff * rsqrt(den2);
+        }
+      }
+    }
+    if (cnt)
+      v *= rcp((float)cnt);
+    F.taskVariance[t] = v * 64.f;
+  }
+}
+
+// LiveVarianceFrameOp::getAvgError (Variance.cpp:32-46) with an FP64 sum;
+// single block
+__global__ __launch_bounds__(256) void k_variance_reduce(const float *taskVar, int n, float threshold, float *out)
+{
+  __shared__ double sSum[256];
+  __shared__ float sMax[256];
+  __shared__ int sCnt[256];
+  double sum = 0.0;
+  float mx = 0.f;
+  int cnt = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float e = taskVar[i];
+    mx = maxf(mx, e);
+    if (e >= threshold) {
+      sum += (double)e;
+      cnt++;
+    }
+  }
+  sSum[threadIdx.x] = sum;
+  sMax[threadIdx.x] = mx;
+  sCnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sSum[threadIdx.x] += sSum[threadIdx.x + s];
+      sMax[threadIdx.x] = maxf(sMax[threadIdx.x], sMax[threadIdx.x + s]);
+      sCnt[threadIdx.x] += sCnt[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    out[0] = sCnt[0] ? (float)(sSum[0] / (double)sCnt[0]) : sMax[0];
+}
+
+__device__ __forceinline__ uint32_t cvtU8(float f) { return (uint32_t)(255.f * clampf(f) + 0.5f); }
+__device__ __forceinline__ float linearToSrgb(float x)
+{
+  x = clampf(x);
+  return x <= 0.0031308f ? 12.92f * x : 1.055f * fexp(flog(x) * (1.f / 2.4f)) - 0.055f;
+}
+
+// ColorConversion_kernel (fb/frame_ops/ColorConversion.ispc:11-36)
+__global__ __launch_bounds__(256) void k_color_convert(const float *color, int npix, int format, void *out)
+{
+  for (int i = globalId(); i < npix; i += gridStride()) {
+    const float4 v = ld4(color + 4 * (size_t)i);
+    if (format == 3) {
+      st4((float *)out + 4 * (size_t)i, maxf(v.x, 0.f), maxf(v.y, 0.f), maxf(v.z, 0.f), minf(maxf(v.w, 0.f), 1.f));
+    } else if (format == 1) {
+      ((uint32_t *)out)[i] = cvtU8(v.x) | (cvtU8(v.y) << 8) | (cvtU8(v.z) << 16) | (cvtU8(v.w) << 24);
+    } else {
+      ((uint32_t *)out)[i] = cvtU8(linearToSrgb(v.x)) | (cvtU8(linearToSrgb(v.y)) << 8)
+          | (cvtU8(linearToSrgb(v.z)) << 16) | (cvtU8(v.w) << 24);
+    }
+  }
+}
+
+// closest hits of an explicit ray list (first-hit parity checks and BVH
+// statistics): rays8 = org.xyz, dir.xyz, t0, t1; out: t,u,v + inst,model,prim
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DScene S, const float *rays8, int n, float *outTUV,
+    uint32_t *outIDs, uint32_t *ovfBase, unsigned long long *stats)
+{
+  __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
+  uint32_t *ovf = ovfBase + (size_t)globalId() * kOverflowDepth;
+  TraceStats st = {0, 0};
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const float *r = rays8 + 8 * (size_t)i;
+    Hit h;
+    bool sawT;
+    traceRay<MODE_CLOSEST, true>(S, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7], h, lds, ovf, sawT, st);
+    outTUV[3 * (size_t)i] = h.hit ? h.t : kInf;
+    outTUV[3 * (size_t)i + 1] = h.u;
+    outTUV[3 * (size_t)i + 2] = h.v;
+    outIDs[3 * (size_t)i] = h.hit ? h.inst : kInvalidID;
+    outIDs[3 * (size_t)i + 1] = h.hit ? h.model : kInvalidID;
+    outIDs[3 * (size_t)i + 2] = h.hit ? h.prim : kInvalidID;
+  }
+  if (stats) {
+    atomicAdd(&stats[0], (unsigned long long)st.nodes);
+    atomicAdd(&stats[1], (unsigned long long)st.prims);
+  }
+}
+
+// ---- C-ABI launch layer -------------------------------------------------------------
+
+namespace {
+inline int gridFor(size_t n, int block, int maxBlocks)
+{
+  size_t b = (n + block - 1) / block;
+  if (b < 1)
+    b = 1;
+  if (b > (size_t)maxBlocks)
+    b = maxBlocks;
+  return (int)b;
+}
+} // namespace
+
+extern "C" {
+
+hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s)
+{
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(8), 0, s, p.counters, 8, keepIndex);
+  return hipGetLastError();
+}
+
+hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_fill_u32, dim3(gridFor(n, 256, 4096)), dim3(256), 0, s, ptr, n, value);
+  return hipGetLastError();
+}
+
+hipError_t hpt_generate(const HptFrame *F, const HptPaths *P, hipStream_t s)
+{
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  const size_t n = (size_t)f.passCount * f.numPixPadded;
+  hipLaunchKernelGGL(k_generate, dim3(gridFor(n, 256, 16384)), dim3(256), 0, s, f, *reinterpret_cast<const DPaths *>(P));
+  return hipGetLastError();
+}
+
+hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int residentBlocks, int stats, hipStream_t s)
+{
+  const DScene &sc = *reinterpret_cast<const DScene *>(S);
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  if (stats)
+    hipLaunchKernelGGL(k_extend<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, p, queue);
+  else
+    hipLaunchKernelGGL(k_extend<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, p, queue);
+  return hipGetLastError();
+}
+
+hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
+      *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P), queue);
+  return hipGetLastError();
+}
+
+hipError_t hpt_shadow(const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_shadow, dim3(residentBlocks), dim3(kTraceBlock), 0, s, *reinterpret_cast<const DScene *>(S),
+      *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P));
+  return hipGetLastError();
+}
+
+hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DFrame *>(F),
+      *reinterpret_cast<const DPaths *>(P));
+  return hipGetLastError();
+}
+
+hipError_t hpt_finish_pass(const HptFrame *F, const HptPaths *P, hipStream_t s)
+{
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  hipLaunchKernelGGL(k_finish_pass, dim3(gridFor((size_t)f.width * f.height, 256, 16384)), dim3(256), 0, s, f,
+      *reinterpret_cast<const DPaths *>(P));
+  return hipGetLastError();
+}
+
+hipError_t hpt_fb_accumulate(const HptFrame *F, hipStream_t s)
+{
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  hipLaunchKernelGGL(k_fb_accumulate, dim3(gridFor((size_t)f.width * f.height, 256, 16384)), dim3(256), 0, s, f);
+  return hipGetLastError();
+}
+
+hipError_t hpt_variance(const HptFrame *F, hipStream_t s)
+{
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  const size_t nt = (size_t)((f.width + 7) / 8) * ((f.height + 7) / 8);
+  hipLaunchKernelGGL(k_variance, dim3(gridFor(nt, 256, 4096)), dim3(256), 0, s, f);
+  return hipGetLastError();
+}
+
+hipError_t hpt_variance_reduce(const float *taskVariance, int n, float threshold, float *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_variance_reduce, dim3(1), dim3(256), 0, s, taskVariance, n, threshold, out);
+  return hipGetLastError();
+}
+
+hipError_t hpt_color_convert(const float *color, int npix, int format, void *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_color_convert, dim3(gridFor((size_t)npix, 256, 16384)), dim3(256), 0, s, color, npix, format, out);
+  return hipGetLastError();
+}
+
+hipError_t hpt_trace_rays(const HptScene *S, const float *rays8, int n, float *outTUV, uint32_t *outIDs,
+    uint32_t *overflow, int residentBlocks, unsigned long long *stats, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_trace_rays, dim3(residentBlocks), dim3(kTraceBlock), 0, s, *reinterpret_cast<const DScene *>(S),
+      rays8, n, outTUV, outIDs, overflow, stats);
+  return hipGetLastError();
+}
+
+int hpt_trace_block_size() { return kTraceBlock; }
+int hpt_overflow_depth() { return kOverflowDepth; }
+int hpt_stack_capacity() { return kLdsDepth + kOverflowDepth; }
+
+} // extern "C"

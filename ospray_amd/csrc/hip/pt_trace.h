@@ -1,0 +1,334 @@
+// BVH traversal and ray/primitive tests of the hip path tracer (device).
+//
+// The reference hands these to Embree (rtcIntersectV,
+// modules/cpu/common/World.ih:29-52, 247-286).  Here:
+//   * two-level BVH2 (TLAS over instances, one BLAS per group) in 64-byte
+//     nodes holding both child boxes; one lane per ray, nearer child first;
+//   * the traversal stack lives in LDS (LDS_DEPTH entries per lane, laid out
+//     [depth][lane] so lanes hit distinct banks) and spills to a per-thread
+//     global overflow row beyond that;
+//   * triangles: watertight test (Woop, Benthin, Wald 2013) with an FP64
+//     fallback for zero edge functions; quads as two triangles (v0,v1,v3) and
+//     (v2,v3,v1); spheres: Intersect.ih:29-80;
+//   * closest hit = smallest t in (t0, tfar]; exact ties resolved by the
+//     smallest (instance, model, primitive, half), so results do not depend
+//     on BVH layout;
+//   * node boxes only cull, conservatively (far plane scaled by 1+4e-7).
+#pragma once
+
+#include "pt_math.h"
+#include "pt_types.h"
+
+namespace hpt {
+
+constexpr int kTraceBlock = 128;
+constexpr int kLdsDepth = 24;
+constexpr int kOverflowDepth = 72;
+constexpr int32_t kMarker = 0x7fffffff;
+
+struct Hit
+{
+  float t, u, v;
+  vec3f Ng;
+  uint32_t inst, model, prim, half;
+  bool hit;
+};
+
+HPT_HD bool keyLess(uint32_t i0, uint32_t m0, uint32_t p0, uint32_t h0, const Hit &b)
+{
+  if (i0 != b.inst)
+    return i0 < b.inst;
+  if (m0 != b.model)
+    return m0 < b.model;
+  if (p0 != b.prim)
+    return p0 < b.prim;
+  return h0 < b.half;
+}
+
+HPT_HD bool better(float t, uint32_t i, uint32_t m, uint32_t p, uint32_t h, const Hit &b)
+{
+  return !b.hit || t < b.t || (t == b.t && keyLess(i, m, p, h, b));
+}
+
+HPT_HD float safeInv(float d)
+{
+  const float ad = fabsf(d);
+  return 1.0f / (ad < 1e-18f ? copysignf(1e-18f, d) : d);
+}
+
+struct TriPre
+{
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+HPT_HD TriPre triPrecompute(vec3f dir)
+{
+  TriPre p;
+  const vec3f ad = vabs(dir);
+  p.kz = ad.x > ad.y ? (ad.x > ad.z ? 0 : 2) : (ad.y > ad.z ? 1 : 2);
+  p.kx = p.kz == 2 ? 0 : p.kz + 1;
+  p.ky = p.kx == 2 ? 0 : p.kx + 1;
+  if (comp(dir, p.kz) < 0.0f) {
+    const int t = p.kx;
+    p.kx = p.ky;
+    p.ky = t;
+  }
+  const float dz = comp(dir, p.kz);
+  p.Sx = comp(dir, p.kx) / dz;
+  p.Sy = comp(dir, p.ky) / dz;
+  p.Sz = 1.0f / dz;
+  return p;
+}
+
+HPT_HD bool intersectTriangle(vec3f org, float t0, const TriPre &pc, vec3f v0, vec3f v1, vec3f v2, float tmax,
+    float &t, float &u, float &v)
+{
+  const vec3f A = v0 - org;
+  const vec3f B = v1 - org;
+  const vec3f C = v2 - org;
+  const float Az_ = comp(A, pc.kz), Bz_ = comp(B, pc.kz), Cz_ = comp(C, pc.kz);
+  const float Ax = comp(A, pc.kx) - pc.Sx * Az_;
+  const float Ay = comp(A, pc.ky) - pc.Sy * Az_;
+  const float Bx = comp(B, pc.kx) - pc.Sx * Bz_;
+  const float By = comp(B, pc.ky) - pc.Sy * Bz_;
+  const float Cx = comp(C, pc.kx) - pc.Sx * Cz_;
+  const float Cy = comp(C, pc.ky) - pc.Sy * Cz_;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+    return false;
+  const float det = U + V + W;
+  if (det == 0.0f)
+    return false;
+  const float Az = pc.Sz * Az_;
+  const float Bz = pc.Sz * Bz_;
+  const float Cz = pc.Sz * Cz_;
+  const float T = U * Az + V * Bz + W * Cz;
+  const float rcpDet = 1.0f / det;
+  const float tt = T * rcpDet;
+  if (!(tt > t0) || !(tt <= tmax))
+    return false;
+  t = tt;
+  u = V * rcpDet;
+  v = W * rcpDet;
+  return true;
+}
+
+HPT_HD bool intersectSphere(vec3f org, vec3f d, float t0, vec3f center, float radius, float tmax, float &t, vec3f &N)
+{
+  const float rd2 = 1.0f / dot(d, d);
+  const vec3f CO = center - org;
+  const float projCO = dot(CO, d) * rd2;
+  const vec3f perp = CO - projCO * d;
+  const float l2 = dot(perp, perp);
+  const float r2 = radius * radius;
+  if (l2 > r2)
+    return false;
+  const float td = sqrtf((r2 - l2) * rd2);
+  const float tEntry = projCO - td;
+  const float tExit = projCO + td;
+  if (tEntry > t0 && tEntry <= tmax) {
+    t = tEntry;
+    N = (-td) * d - perp;
+    return true;
+  }
+  if (tExit > t0 && tExit <= tmax) {
+    t = tExit;
+    N = td * d - perp;
+    return true;
+  }
+  return false;
+}
+
+struct TraceStats
+{
+  uint32_t nodes, prims;
+};
+
+__device__ __forceinline__ void loadNode(const DNode *nodes, int idx, float4 &a, float4 &b, float4 &c, int4 &d)
+{
+  const float4 *np = reinterpret_cast<const float4 *>(nodes + idx);
+  a = np[0];
+  b = np[1];
+  c = np[2];
+  d = reinterpret_cast<const int4 *>(np)[3];
+}
+
+__device__ __forceinline__ float slabNear(float lox, float loy, float loz, float hix, float hiy, float hiz,
+    vec3f o, vec3f inv, float t0, float tb)
+{
+  const float x0 = (lox - o.x) * inv.x, x1 = (hix - o.x) * inv.x;
+  const float y0 = (loy - o.y) * inv.y, y1 = (hiy - o.y) * inv.y;
+  const float z0 = (loz - o.z) * inv.z, z1 = (hiz - o.z) * inv.z;
+  const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), t0));
+  const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb)) * 1.0000004f;
+  return tn <= tf ? tn : kInf;
+}
+
+// Traversal modes: MODE_CLOSEST returns the closest hit; MODE_OCCLUSION
+// returns as soon as a primitive of an opaque model is hit and otherwise
+// reports in `sawTransparent` whether any (non-opaque) primitive was hit.
+enum TraceMode
+{
+  MODE_CLOSEST = 0,
+  MODE_OCCLUSION = 1
+};
+
+template <int MODE, bool STATS>
+__device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
+    uint32_t *lds, uint32_t *ovf, bool &sawTransparent, TraceStats &st)
+{
+  const int lane = threadIdx.x;
+  best.hit = false;
+  best.t = tfar;
+  best.u = best.v = 0.f;
+  best.Ng = v3(0.f);
+  best.inst = best.model = best.prim = kInvalidID;
+  best.half = 0;
+  sawTransparent = false;
+  const vec3f winv = v3(safeInv(wdir.x), safeInv(wdir.y), safeInv(wdir.z));
+  vec3f o = worg, d = wdir, inv = winv;
+  TriPre pc = {0, 1, 2, 0.f, 0.f, 0.f};
+  int inst = -1;
+  int modelOffset = 0;
+  int sp = 0;
+  int cur = S.tlasRoot;
+  // callers check sp < kLdsDepth + kOverflowDepth before pushing
+  auto push = [&](int v) {
+    if (sp < kLdsDepth)
+      lds[sp * kTraceBlock + lane] = (uint32_t)v;
+    else
+      ovf[sp - kLdsDepth] = (uint32_t)v;
+    sp++;
+  };
+  auto pop = [&]() -> int {
+    sp--;
+    if (sp < kLdsDepth)
+      return (int)lds[sp * kTraceBlock + lane];
+    return (int)ovf[sp - kLdsDepth];
+  };
+  for (;;) {
+    if (cur >= 0 && cur != kMarker) {
+      float4 a, b, c;
+      int4 ch;
+      loadNode(S.nodes, cur, a, b, c, ch);
+      if (STATS)
+        st.nodes++;
+      const float tb = best.t;
+      const float tn0 = slabNear(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, t0, tb);
+      const float tn1 = slabNear(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, t0, tb);
+      const bool h0 = tn0 != kInf, h1 = tn1 != kInf;
+      if (h0 && h1) {
+        int nearC = ch.x, farC = ch.y;
+        if (tn1 < tn0) {
+          nearC = ch.y;
+          farC = ch.x;
+        }
+        if (sp >= kLdsDepth + kOverflowDepth) // stack full: cannot happen for builder-bounded trees
+          return best.hit;
+        push(farC);
+        cur = nearC;
+      } else if (h0) {
+        cur = ch.x;
+      } else if (h1) {
+        cur = ch.y;
+      } else {
+        if (sp == 0)
+          break;
+        cur = pop();
+      }
+      continue;
+    }
+    if (cur == kMarker) {
+      o = worg;
+      d = wdir;
+      inv = winv;
+      inst = -1;
+      if (sp == 0)
+        break;
+      cur = pop();
+      continue;
+    }
+    const uint32_t enc = ~(uint32_t)cur;
+    const uint32_t first = enc >> 3;
+    const uint32_t count = (enc & 7u) + 1u;
+    if (inst < 0) {
+      // TLAS leaf: one instance; continue in its BLAS with the object ray
+      inst = (int)first;
+      const DInstance &I = S.instances[inst];
+      const linear3f rl = {{I.rcp[0], I.rcp[1], I.rcp[2]}, {I.rcp[3], I.rcp[4], I.rcp[5]}, {I.rcp[6], I.rcp[7], I.rcp[8]}};
+      o = mul(rl, worg) + v3(I.rcp[9], I.rcp[10], I.rcp[11]);
+      d = mul(rl, wdir);
+      inv = v3(safeInv(d.x), safeInv(d.y), safeInv(d.z));
+      pc = triPrecompute(d);
+      modelOffset = I.modelOffset;
+      if (sp >= kLdsDepth + kOverflowDepth)
+        return best.hit;
+      push(kMarker);
+      cur = I.blasRoot;
+      continue;
+    }
+    for (uint32_t k = 0; k < count; k++) {
+      const DPrim &P = S.prims[first + k];
+      const float4 pa = reinterpret_cast<const float4 *>(&P)[0];
+      const float4 pb = reinterpret_cast<const float4 *>(&P)[1];
+      const float4 pcv = reinterpret_cast<const float4 *>(&P)[2];
+      const uint32_t primID = __float_as_uint(pa.w);
+      const uint32_t mk = __float_as_uint(pb.w);
+      const uint32_t model = mk & 0x0fffffffu;
+      const uint32_t kind = mk >> 28;
+      if (STATS)
+        st.prims++;
+      float t, u = 0.f, v = 0.f;
+      vec3f Ng;
+      bool h;
+      uint32_t half = 0;
+      if (kind == PRIM_SPHERE) {
+        h = intersectSphere(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, best.t, t, Ng);
+      } else {
+        const vec3f A = v3(pa.x, pa.y, pa.z), B = v3(pb.x, pb.y, pb.z), C = v3(pcv.x, pcv.y, pcv.z);
+        h = intersectTriangle(o, t0, pc, A, B, C, best.t, t, u, v);
+        if (h) {
+          Ng = cross(B - A, C - A);
+          if (kind == PRIM_QUAD1) {
+            half = 1;
+            u = 1.0f - u;
+            v = 1.0f - v;
+          }
+        }
+      }
+      if (!h)
+        continue;
+      if (MODE == MODE_OCCLUSION) {
+        if (S.models[modelOffset + model].opaque)
+          return true;
+        sawTransparent = true;
+        continue;
+      }
+      if (better(t, (uint32_t)inst, model, primID, half, best)) {
+        best.hit = true;
+        best.t = t;
+        best.u = u;
+        best.v = v;
+        best.Ng = Ng;
+        best.inst = (uint32_t)inst;
+        best.model = model;
+        best.prim = primID;
+        best.half = half;
+      }
+    }
+    if (sp == 0)
+      break;
+    cur = pop();
+  }
+  return best.hit;
+}
+
+} // namespace hpt

@@ -1,0 +1,224 @@
+// Device-side data layout of the hip path tracer (plain structs shared by
+// host set-up code and kernels).
+//
+// HBM layout of a committed world:
+//   DNode[]     BVH2 nodes, 64 B each, both child boxes in one node so a
+//               traversal step is one 64-byte fetch; TLAS and every BLAS share
+//               the array.  child >= 0: inner node; child < 0: leaf,
+//               ~child = first << 3 | (count - 1).
+//   DPrim[]     48 B per primitive (triangle, quad half or sphere); vertices
+//               are stored in object space next to the ids so a leaf is a
+//               contiguous run.
+//   DInstance[] 128 B: inverse and forward affine transform (columns),
+//               BLAS root, model table offset, user id.
+//   DModel[]    per geometric model of a group: geometry arrays for
+//               post-intersect, material, user id.
+//   materials, media, lights, light CDF: small tables.
+#pragma once
+
+#include <cstdint>
+
+namespace hpt {
+
+enum PrimKind : uint32_t
+{
+  PRIM_TRIANGLE = 0,
+  PRIM_QUAD0 = 1, // (v0, v1, v3)
+  PRIM_QUAD1 = 2, // (v2, v3, v1), reports (1-u, 1-v)
+  PRIM_SPHERE = 3
+};
+
+struct DNode
+{
+  float b0lo[3], b0hi[3];
+  float b1lo[3], b1hi[3];
+  int32_t child[2];
+  int32_t pad[2];
+};
+static_assert(sizeof(DNode) == 64, "node size");
+
+struct DPrim
+{
+  // triangles / quad halves: v0, primID | v1, model | kind<<28 | v2, 0
+  // spheres: centre, primID | radius,0,0, model | kind<<28 | 0
+  float a[3];
+  uint32_t primID;
+  float b[3];
+  uint32_t modelKind;
+  float c[3];
+  uint32_t pad;
+};
+static_assert(sizeof(DPrim) == 48, "prim size");
+
+struct DInstance
+{
+  float rcp[12]; // inverse transform: columns vx, vy, vz, p
+  float xfm[12]; // forward transform
+  int32_t blasRoot; // node index of the group's BLAS root
+  int32_t modelOffset; // first DModel of the group
+  uint32_t userID; // kInvalidID if unset
+  float epsScale; // max(|xfm.vx.x|, |xfm.vy.y|, |xfm.vz.z|)
+  int32_t pad[4];
+};
+static_assert(sizeof(DInstance) == 128, "instance size");
+
+enum GeomType : int32_t
+{
+  GEOM_TRIANGLES = 0,
+  GEOM_QUADS = 1,
+  GEOM_SPHERES = 2
+};
+
+struct DModel
+{
+  const float *vertex; // vec3f
+  const uint32_t *index; // vec3ui / vec4ui
+  const float *normal; // vec3f or null
+  const float *spheres; // vec4f centre + radius
+  int32_t geomType;
+  int32_t material; // -1: none
+  uint32_t userID;
+  int32_t invertNormals;
+  int32_t opaque; // material transparency is identically zero
+  int32_t pad[3];
+};
+
+enum MaterialType : int32_t
+{
+  MAT_OBJ = 0,
+  MAT_METAL = 1,
+  MAT_GLASS = 2
+};
+
+struct DMaterial
+{
+  int32_t type;
+  float Kd[3], Tf[3], d; // obj
+  float eta[3], k[3], roughness; // metal
+  int32_t inside, outside; // glass: indices into the media table
+};
+
+struct DMedium
+{
+  float attenuation[3];
+  float ior;
+};
+
+enum LightType : int32_t
+{
+  LIGHT_QUAD = 0,
+  LIGHT_DISTANT = 1,
+  LIGHT_AMBIENT = 2
+};
+
+struct DLight
+{
+  int32_t type;
+  int32_t visible;
+  float radiance[3];
+  float position[3], edge1[3], edge2[3], nnormal[3];
+  float ppdf;
+  float frame[9]; // distant: vx, vy, vz
+  float irradiance[3];
+  float cosAngle, pdf;
+};
+
+struct DScene
+{
+  const DNode *nodes;
+  const DPrim *prims;
+  const DInstance *instances;
+  const DModel *models;
+  const DMaterial *materials;
+  const DMedium *media;
+  const DLight *lights;
+  const float *lightsCDF;
+  int32_t tlasRoot;
+  int32_t numInstances;
+  int32_t numLights;
+  int32_t numMaterials;
+};
+
+// Path state of one pass, structure of arrays indexed by path slot.
+struct DPaths
+{
+  float *org; // 4 per path: org.xyz, t0
+  float *dir; // 4: dir.xyz, (unused)
+  float *thr; // 4: throughput.xyz, alpha
+  float *contrib; // 4: contribution.xyz, z (primary depth)
+  uint32_t *ld; // 4: revIndex, scramble, baseIndexLight, flags
+  float *lastP; // 4: last vertex P.xyz, pdf_w
+  float *lastNs; // 4: last vertex Ns.xyz, numLightSamples (bits)
+  float *hitA; // 4: t, u, v, inst (bits)
+  float *hitB; // 4: Ng.xyz (object space), model (bits)
+  uint32_t *hitPrim; // prim | half << 31
+  uint32_t *ids; // 4: primID, geomID, instID, medium
+  float *aux; // 16: position.xyz, firstNormal.xyz, normal.xyz, albedo.xyz, pad
+  uint32_t *neeN; // light samples of the pending NEE vertex
+  // NEE shadow-ray records, slot = path * maxRec + i
+  float *recOrg; // 4: org.xyz, tmax
+  float *recDir; // 4: dir.xyz, mis weight
+  float *recContrib; // 4: unshaded contribution.xyz, medium (bits)
+  float *recResult; // 4: L.xyz, valid flag
+  // queues and counters
+  uint32_t *queue[2]; // extension queues (ping-pong)
+  uint32_t *shadowQueue;
+  uint32_t *neeQueue;
+  int32_t *counters; // [0..1] extension, [2] shadow, [3] nee, [4..7] spare
+  uint32_t *overflow; // traversal stack overflow, per resident thread
+  unsigned long long *stats; // [0] node visits, [1] primitive tests, [2] rays
+};
+
+// path flags in ld[3]
+constexpr uint32_t FLAG_DEPTH_MASK = 0xffu; // bits 0-7
+constexpr uint32_t FLAG_SCAT_SHIFT = 8u; // bits 8-15 scattering events
+constexpr uint32_t FLAG_FIRST_BOUNCE_LIGHT = 1u << 16;
+constexpr uint32_t FLAG_STRAIGHT = 1u << 17;
+constexpr uint32_t FLAG_SPEC_TRANS_PATH = 1u << 18;
+constexpr uint32_t FLAG_MEDIUM_SHIFT = 20u; // bits 20-31 medium index
+
+struct DFrame
+{
+  // image and pass
+  int32_t width, height;
+  int32_t tilesX, tilesY, numPixPadded;
+  int32_t spp, passS0, passCount; // samples [passS0, passS0+passCount)
+  uint32_t accumID;
+  int32_t targetFrames;
+  int32_t projectedDepth;
+  uint32_t channels; // OSPFrameBufferChannel mask
+  int32_t accumulateVariance;
+  int32_t maxRec;
+  int32_t residentThreads; // overflow stack rows
+  uint32_t idxMask, idxMaskLight; // LDSampler index masks
+  // camera (PerspectiveCamera set-up, host side)
+  float camOrg[3], camDir00[3], camDu[3], camDv[3], camDirection[3];
+  float nearClip;
+  // renderer
+  int32_t maxDepth, rouletteDepth, maxScatteringEvents;
+  uint32_t numFirstLightSamples, numIndirectLightSamples;
+  float minContribution, maxRadiance;
+  float bgColor[4];
+  int32_t filterType; // 0 point, 1 box, 2 LUT
+  float filterWidth;
+  int32_t lutSize;
+  const float *lutCdfX;
+  const float *lutCdfY;
+  // per-pixel sample accumulators across passes (row-major)
+  float *accRGBA; // 4: rgb sum, alpha sum
+  float *accAux; // 12: position, firstNormal, normal, albedo sums
+  float *accZ;
+  uint32_t *accIDs; // 3
+  // framebuffer buffers (row-major, null if the channel is absent)
+  float *fbColor; // vec4f
+  float *fbVariance; // vec4f
+  float *fbDepth;
+  float *fbNormal; // vec3f
+  float *fbAlbedo;
+  float *fbFirstNormal;
+  float *fbPosition;
+  uint32_t *fbPrimID, *fbObjID, *fbInstID;
+  float *taskVariance;
+};
+
+} // namespace hpt

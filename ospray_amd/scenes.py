@@ -1,0 +1,346 @@
+"""Seeded synthetic scenes of the benchmark configurations.
+
+All randomness comes from PCG32 (O'Neill's XSH-RR generator, seeded with
+pcg32_srandom(seed, 54)); uniform floats are (u32 >> 8) * 2^-24.
+
+  c0(): 512x512, 1 spp, maxPathLength 5; 64 icospheres (subdivision 3) with
+        radius U[0.2,0.6], centre U[-5,5]^3 and Lambert kd U[0.1,0.9]^3 plus
+        a 20x20 ground quad at y=-6; one quad light (intensity 10) and an
+        ambient light (0.2).  Seed 42.
+  c1(): 1920x1080, 4 spp, maxPathLength 8; 1448^2 fBm heightfield over
+        [-50,50]^2 (4,187,618 triangles, 5 octaves, amplitude 2) plus 256
+        instances of one subdivision-4 icosphere; 60/25/15 % diffuse / metal
+        (roughness U[0,0.5]) / glass (eta 1.5) per instance; a sun
+        (distant, 0.53 deg) and 4 quad lights.  Seed 43.
+
+A Scene is a plain description (numpy arrays and dicts); `to_ospray` builds
+it through the C API and returns the committed handles.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+_MASK64 = (1 << 64) - 1
+
+
+class Pcg32:
+    MULT = 6364136223846793005
+
+    def __init__(self, seed: int, seq: int = 54):
+        self.state = 0
+        self.inc = ((seq << 1) | 1) & _MASK64
+        self.next_u32()
+        self.state = (self.state + seed) & _MASK64
+        self.next_u32()
+
+    def next_u32(self) -> int:
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & _MASK64
+        xorshifted = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xorshifted >> rot) | (xorshifted << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        u = (self.next_u32() >> 8) * (1.0 / 16777216.0)
+        return lo + (hi - lo) * u
+
+
+@dataclasses.dataclass
+class Geometry:
+    type: str  # "triangles" | "quads" | "spheres"
+    vertex: Optional[np.ndarray] = None  # (N,3) float32
+    index: Optional[np.ndarray] = None  # (M,3|4) uint32
+    normal: Optional[np.ndarray] = None  # (N,3) float32
+    spheres: Optional[np.ndarray] = None  # (N,4) float32 centre+radius
+
+    @property
+    def num_prims(self) -> int:
+        return int(self.spheres.shape[0] if self.type == "spheres" else self.index.shape[0])
+
+
+@dataclasses.dataclass
+class Model:
+    geometry: int
+    material: int = -1
+    user_id: Optional[int] = None
+
+
+@dataclasses.dataclass
+class Instance:
+    group: int
+    xfm: tuple = (1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0)
+    user_id: Optional[int] = None
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    width: int
+    height: int
+    geometries: list
+    models: list
+    groups: list  # list of lists of model indices
+    instances: list
+    materials: list  # dicts with "type" + OSPRay parameters
+    lights: list  # dicts with "type" + OSPRay parameters
+    camera: dict
+    renderer: dict
+
+    @property
+    def num_triangles(self) -> int:
+        n = 0
+        for inst in self.instances:
+            for m in self.groups[inst.group]:
+                g = self.geometries[self.models[m].geometry]
+                n += g.num_prims * (2 if g.type == "quads" else 1) if g.type != "spheres" else 0
+        return n
+
+
+# ---- geometry generators -----------------------------------------------------------------
+
+def icosphere(subdiv: int):
+    """Unit icosphere: (vertices (N,3) float64, triangles (M,3) int)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, np.float64) / np.linalg.norm(p) for p in verts]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                cache[key] = len(v) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(v), np.array(faces, np.int64)
+
+
+def _fade(t):
+    return t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+
+
+class GradientNoise2D:
+    """Seeded 2D gradient (Perlin-style) noise, vectorised."""
+
+    def __init__(self, rng: Pcg32):
+        perm = list(range(256))
+        for i in range(255, 0, -1):  # Fisher-Yates with PCG draws
+            j = rng.next_u32() % (i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        self.perm = np.array(perm + perm, np.int64)
+        ang = np.array([rng.uniform(0.0, 2.0 * math.pi) for _ in range(256)])
+        self.grad = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+
+    def __call__(self, x, y):
+        xi = np.floor(x).astype(np.int64)
+        yi = np.floor(y).astype(np.int64)
+        xf, yf = x - xi, y - yi
+        xi &= 255
+        yi &= 255
+        p = self.perm
+
+        def g(ix, iy, dx, dy):
+            h = p[p[ix] + iy] & 255
+            gr = self.grad[h]
+            return gr[..., 0] * dx + gr[..., 1] * dy
+
+        n00 = g(xi, yi, xf, yf)
+        n10 = g(xi + 1, yi, xf - 1, yf)
+        n01 = g(xi, yi + 1, xf, yf - 1)
+        n11 = g(xi + 1, yi + 1, xf - 1, yf - 1)
+        u, v = _fade(xf), _fade(yf)
+        a = n00 + u * (n10 - n00)
+        b = n01 + u * (n11 - n01)
+        return a + v * (b - a)
+
+
+def fbm(noise, x, y, octaves=5, lacunarity=2.0, gain=0.5, freq=1.0 / 20.0):
+    s = np.zeros_like(x)
+    amp = 1.0
+    for _ in range(octaves):
+        s += amp * noise(x * freq, y * freq)
+        freq *= lacunarity
+        amp *= gain
+    return s
+
+
+def heightfield(n: int, extent: float, amplitude: float, noise):
+    """n x n vertex grid over [-extent, extent]^2, y = amplitude * fBm."""
+    c = np.linspace(-extent, extent, n)
+    X, Z = np.meshgrid(c, c)  # Z varies along rows (j), X along columns (i)
+    Y = amplitude * fbm(noise, X, Z)
+    vert = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    i = np.arange(n - 1)
+    j = np.arange(n - 1)
+    I, J = np.meshgrid(i, j)
+    v00 = (J * n + I).ravel()
+    v10 = v00 + 1
+    v01 = v00 + n
+    v11 = v01 + 1
+    # (v00, v01, v11) and (v00, v11, v10): geometric normals point to +y
+    tri = np.empty((2 * v00.size, 3), np.uint32)
+    tri[0::2] = np.stack([v00, v01, v11], axis=1)
+    tri[1::2] = np.stack([v00, v11, v10], axis=1)
+    return vert, tri
+
+
+# ---- configurations --------------------------------------------------------------------
+
+def _look_at(pos, target=(0.0, 0.0, 0.0)):
+    return tuple(float(t - p) for p, t in zip(pos, target))
+
+
+def c0(width: int = 512, height: int = 512, spp: int = 1, max_path_length: int = 5, seed: int = 42) -> Scene:
+    rng = Pcg32(seed)
+    unit, faces = icosphere(3)
+    idx = faces.astype(np.uint32)
+    geoms, models, mats = [], [], []
+    for i in range(64):
+        r = rng.uniform(0.2, 0.6)
+        c = np.array([rng.uniform(-5, 5), rng.uniform(-5, 5), rng.uniform(-5, 5)])
+        kd = (rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9))
+        geoms.append(Geometry("triangles", vertex=(c + r * unit).astype(np.float32), index=idx,
+                              normal=unit.astype(np.float32)))
+        mats.append({"type": "obj", "kd": kd})
+        models.append(Model(geometry=i, material=i))
+    ground = np.array([[-10, -6, -10], [10, -6, -10], [10, -6, 10], [-10, -6, 10]], np.float32)
+    geoms.append(Geometry("quads", vertex=ground, index=np.array([[0, 1, 2, 3]], np.uint32)))
+    mats.append({"type": "obj"})
+    models.append(Model(geometry=64, material=64))
+    lights = [
+        {"type": "quad", "position": (-2.0, 8.0, -2.0), "edge1": (0.0, 0.0, 4.0), "edge2": (4.0, 0.0, 0.0),
+         "intensity": 10.0},
+        {"type": "ambient", "intensity": 0.2},
+    ]
+    pos = (0.0, 4.0, -20.0)
+    return Scene("c0", width, height, geoms, models, [list(range(65))], [Instance(0)], mats, lights,
+                 camera={"position": pos, "direction": _look_at(pos), "up": (0.0, 1.0, 0.0), "fovy": 45.0,
+                         "aspect": width / height},
+                 renderer={"pixelSamples": spp, "maxPathLength": max_path_length})
+
+
+def c1(width: int = 1920, height: int = 1080, spp: int = 4, max_path_length: int = 8, seed: int = 43,
+       grid: int = 1448, instances: int = 256) -> Scene:
+    rng = Pcg32(seed)
+    noise = GradientNoise2D(rng)
+    vert, tri = heightfield(grid, 50.0, 2.0, noise)
+    unit, faces = icosphere(4)
+    geoms = [Geometry("triangles", vertex=vert, index=tri),
+             Geometry("triangles", vertex=unit.astype(np.float32), index=faces.astype(np.uint32),
+                      normal=unit.astype(np.float32))]
+    mats = [{"type": "obj", "kd": (0.55, 0.5, 0.45)}]
+    models = [Model(geometry=0, material=0)]
+    groups = [[0]]
+    insts = [Instance(0)]
+    for k in range(instances):
+        u = rng.uniform()
+        if u < 0.60:
+            mats.append({"type": "obj", "kd": (rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9))})
+        elif u < 0.85:
+            mats.append({"type": "metal", "roughness": rng.uniform(0.0, 0.5)})
+        else:
+            mats.append({"type": "glass", "eta": 1.5})
+        s = rng.uniform(0.5, 2.5)
+        x, z = rng.uniform(-45, 45), rng.uniform(-45, 45)
+        y = float(2.0 * fbm(noise, np.array([x]), np.array([z]))[0]) + s + rng.uniform(0.0, 3.0)
+        models.append(Model(geometry=1, material=len(mats) - 1))
+        groups.append([len(models) - 1])
+        insts.append(Instance(len(groups) - 1, xfm=(s, 0, 0, 0, s, 0, 0, 0, s, x, y, z)))
+    d = np.array([-0.3, -1.0, 0.4])
+    lights = [{"type": "distant", "direction": tuple(float(v) for v in d / np.linalg.norm(d)), "intensity": 2.5,
+               "angularDiameter": 0.53, "color": (1.0, 0.95, 0.85)}]
+    for k, (qx, qz) in enumerate([(-20, -20), (20, -20), (-20, 20), (20, 20)]):
+        lights.append({"type": "quad", "position": (qx - 3.0, 25.0, qz - 3.0), "edge1": (0.0, 0.0, 6.0),
+                       "edge2": (6.0, 0.0, 0.0), "intensity": 4.0,
+                       "color": (1.0, 0.9 + 0.03 * k, 0.8 + 0.05 * k)})
+    pos = (0.0, 20.0, -60.0)
+    return Scene("c1", width, height, geoms, models, groups, insts, mats, lights,
+                 camera={"position": pos, "direction": _look_at(pos), "up": (0.0, 1.0, 0.0), "fovy": 40.0,
+                         "aspect": width / height},
+                 renderer={"pixelSamples": spp, "maxPathLength": max_path_length})
+
+
+# ---- OSPRay construction ---------------------------------------------------------------------
+
+@dataclasses.dataclass
+class OspScene:
+    world: object
+    camera: object
+    renderer: object
+    handles: list
+
+
+def to_ospray(scene: Scene, renderer_params: Optional[dict] = None) -> OspScene:
+    """Build the scene through the OSPRay C API (device must be current)."""
+    from . import api as A
+    from . import capi as K
+
+    keep = []
+    geo_h = []
+    for g in scene.geometries:
+        if g.type == "spheres":
+            pos = A.shared_data(np.ascontiguousarray(g.spheres[:, :3], np.float32))
+            rad = A.shared_data(np.ascontiguousarray(g.spheres[:, 3], np.float32))
+            h = A.new("geometry", "sphere", {"sphere.position": pos, "sphere.radius": rad})
+            keep += [pos, rad]
+        else:
+            params = {"vertex.position": A.shared_data(g.vertex), "index": A.shared_data(g.index)}
+            if g.normal is not None:
+                params["vertex.normal"] = A.shared_data(g.normal)
+            h = A.new("geometry", "mesh", params)
+            keep += list(params.values())
+        geo_h.append(h)
+    mat_h = []
+    for m in scene.materials:
+        p = {k: v for k, v in m.items() if k != "type"}
+        mat_h.append(A.new("material", m["type"], p))
+    model_h = []
+    for m in scene.models:
+        p = {"geometry": geo_h[m.geometry]}
+        if m.material >= 0:
+            p["material"] = mat_h[m.material]
+        if m.user_id is not None:
+            p["id"] = A.UInt(m.user_id)
+        model_h.append(A.new("model", params=p))
+    group_h = []
+    for grp in scene.groups:
+        arr = A.object_data([model_h[i] for i in grp], K.OSP_GEOMETRIC_MODEL)
+        group_h.append(A.new("group", params={"geometry": arr}))
+        keep.append(arr)
+    inst_h = []
+    for inst in scene.instances:
+        p = {"group": group_h[inst.group], "transform": A.Affine3f(inst.xfm)}
+        if inst.user_id is not None:
+            p["id"] = A.UInt(inst.user_id)
+        inst_h.append(A.new("instance", params=p))
+    light_h = []
+    for l in scene.lights:
+        p = {k: v for k, v in l.items() if k != "type"}
+        light_h.append(A.new("light", l["type"], p))
+    world_params = {"instance": A.object_data(inst_h, K.OSP_INSTANCE)}
+    if light_h:
+        world_params["light"] = A.object_data(light_h, K.OSP_LIGHT)
+    world = A.new("world", params=world_params)
+    keep += list(world_params.values())
+    cam = A.new("camera", "perspective", dict(scene.camera))
+    rp = dict(scene.renderer)
+    if renderer_params:
+        rp.update(renderer_params)
+    ren = A.new("renderer", "pathtracer", rp)
+    handles = keep + geo_h + mat_h + model_h + group_h + inst_h + light_h
+    return OspScene(world, cam, ren, handles)
