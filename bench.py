@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Path samples/s of the hip `pathtracer` at 1080p.
+
+Workload: the 4.2M-triangle scene at 1920x1080, 4 spp, maxPathLength 8
+(seeded fBm heightfield 1448^2 + 256 icosphere instances, 60/25/15 %
+diffuse/metal/glass, a sun and 4 quad lights; ospray_amd/scenes.py c1()).
+A step is one ospRenderFrame + ospWait of that frame through the OSPRay C
+API, the scene already committed and resident in HBM.  Prints one JSON line.
+
+  value       whole-job path samples/s (1920*1080*4 per step / wall time)
+  roofline    the dominant kernel (closest-hit traversal of the extension or
+              the shadow rays) against MI355X's 8 TB/s: algorithmic bytes per
+              launch = 64 B per BVH node visit + 48 B per primitive test +
+              56 B per ray (32 B ray in, 24 B hit out), with node and
+              primitive counts from the device's traversal counters on one
+              extra frame, over the kernel's mean launch duration from HIP
+              events recorded on the device stream during the timed steps
+  cpu_baseline the CPU oracle (a C++ restatement of the reference path
+              tracer) on a bounded sample of the same frame's rows, all host
+              threads up to 16
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "path samples/s at 1080p; plain vs neural-cache time-to-equal-RMSE"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, PRIM_BYTES, RAY_BYTES = 64, 48, 56
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c1.json")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(sc, target_s: float) -> dict:
+    """Oracle samples/s on evenly spaced 8-row bands of the frame (about
+    target_s seconds of CPU work)."""
+    from oracle import orc
+
+    t = time.perf_counter()
+    osc = orc.OracleScene(sc)
+    log(f"oracle BVH built in {time.perf_counter() - t:.1f} s")
+    cam, ren = orc.make_camera(sc.camera), orc.make_renderer(sc.renderer)
+    fb = orc.OracleFrameBuffer(sc.width, sc.height, 3, 1 | 4)
+    nt = host_threads()
+    nbands = (sc.height + 7) // 8
+    spp = sc.renderer["pixelSamples"]
+
+    def band(b):
+        y0 = b * 8
+        y1 = min(sc.height, y0 + 8)
+        t0 = time.perf_counter()
+        fb.render(osc, cam, ren, nt, (y0, y1))
+        return time.perf_counter() - t0, (y1 - y0) * sc.width * spp
+
+    # calibrate on the middle band, then spread the sample over the frame
+    dt, _ = band(nbands // 2)
+    count = int(max(2, min(nbands, target_s / max(dt, 1e-4))))
+    picks = sorted({int(round(i * (nbands - 1) / (count - 1))) for i in range(count)})
+    total_t = total_n = 0
+    for b in picks:
+        dt, n = band(b)
+        total_t += dt
+        total_n += n
+    log(f"oracle: {total_n} samples in {total_t:.1f} s on {nt} threads")
+    return {"value": total_n / total_t, "unit": "samples/s", "cores": nt, "kind": "port",
+            "sample": f"{len(picks)} of {nbands} 8-row bands of the same 1920x1080 4-spp frame "
+                      f"({total_n} path samples, {total_t:.1f} s), oracle C++ restatement, {nt} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample size (0: skip)")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the frame would be split by "
+              "tiles or sample ranges across devices; that path is not built)", file=sys.stderr)
+        return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+
+    from ospray_amd import api as A
+    from ospray_amd import capi as K
+    from ospray_amd import scenes
+
+    t = time.perf_counter()
+    sc = scenes.c1(width=args.width, height=args.height, spp=args.spp)
+    ntris = sum(g.num_prims for g in sc.geometries[:1]) + len(sc.instances[1:]) * sc.geometries[1].num_prims
+    log(f"scene generated in {time.perf_counter() - t:.1f} s ({ntris} instanced triangles)")
+
+    A.init("hip", profileKernels=True)
+    name, cus = A.device_info()
+    log(f"device {name}, {cus} CUs")
+    t = time.perf_counter()
+    s = scenes.to_ospray(sc)
+    log(f"scene committed (host BVH build + upload) in {time.perf_counter() - t:.1f} s")
+    ch = K.OSP_FB_COLOR | K.OSP_FB_ACCUM | K.OSP_FB_VARIANCE | K.OSP_FB_DEPTH
+    fb = A.framebuffer(sc.width, sc.height, K.OSP_FB_RGBA32F, ch)
+    L = K.lib()
+
+    def frame():
+        f = L.ospRenderFrame(fb.h, s.renderer.h, s.camera.h, s.world.h)
+        if not f:
+            A.check()
+            raise RuntimeError("ospRenderFrame failed")
+        L.ospWait(f, K.OSP_TASK_FINISHED)
+        L.ospRelease(f)
+
+    for i in range(args.warmup):
+        frame()
+        log(f"warmup {i + 1}/{args.warmup}")
+    A.check()
+    A.reset_kernel_stats()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        frame()
+    elapsed = time.perf_counter() - t0
+    A.check()
+    kstats = A.kernel_stats()
+    log(f"timed {args.steps} frames in {elapsed:.3f} s")
+
+    samples = sc.width * sc.height * args.spp
+    value = samples * args.steps / elapsed
+
+    # traversal counts of one more frame (not timed)
+    dev = L.ospGetCurrentDevice()
+    A.set_device_param(dev, "traversalStats", True)
+    L.ospDeviceCommit(dev)
+    frame()
+    tr = A.traversal_stats()
+    A.set_device_param(dev, "traversalStats", False)
+    L.ospDeviceCommit(dev)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    import numpy as np
+
+    if not np.isfinite(col).all():
+        raise RuntimeError("non-finite pixels in the rendered frame")
+    log(f"traversal counters: {tr}")
+
+    kern = {}
+    for k, key in (("extend", 0), ("shadow", 3)):
+        nodes, prims, rays = tr[key:key + 3]
+        launches, ms = kstats.get(k, (0, 0.0))
+        if not launches:
+            continue
+        per_frame_bytes = NODE_BYTES * nodes + PRIM_BYTES * prims + RAY_BYTES * rays
+        ms_frame = ms / args.steps
+        kern[k] = {"ms_per_frame": ms_frame, "launches_per_frame": launches / args.steps,
+                   "bytes_per_frame": per_frame_bytes, "rays_per_frame": rays,
+                   "nodes_per_ray": nodes / max(rays, 1), "prims_per_ray": prims / max(rays, 1),
+                   "gbs": per_frame_bytes / (ms_frame * 1e-3) / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["ms_per_frame"])
+    d = kern[dom]
+    lpf = d["launches_per_frame"]
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            tj = json.load(fh)
+        if tj.get("kernel") == dom:
+            traffic = tj.get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}",
+                "bytes_per_launch": int(d["bytes_per_frame"] / lpf),
+                "launch_ms": round(d["ms_per_frame"] / lpf, 4)}
+    rays_total = sum(kern[k]["rays_per_frame"] for k in kern)
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(sc, args.cpu_seconds)
+
+    ms_step = elapsed / args.steps * 1e3
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"plain pathtracer {sc.width}x{sc.height} {args.spp}spp maxPathLength 8, "
+                                  "fBm heightfield 1448^2 (4,187,618 tris) + 256 icosphere instances, "
+                                  "sun + 4 quad lights",
+                      "width": sc.width, "height": sc.height, "spp": args.spp, "max_path_length": 8,
+                      "instanced_triangles": int(ntris), "device": name},
+           "rays_per_s": round(rays_total * args.steps / elapsed, 1),
+           "kernels_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats.items())},
+           "traversal": {k: {"nodes_per_ray": round(v["nodes_per_ray"], 2), "prims_per_ray": round(v["prims_per_ray"], 2),
+                             "rays_per_frame": v["rays_per_frame"], "gbs": round(v["gbs"], 1)}
+                         for k, v in kern.items()},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -26,9 +26,11 @@ typedef struct
 OSPRAY_INTERFACE int ospHipGetKernelStats(OSPHipKernelStat *out, int maxEntries);
 OSPRAY_INTERFACE int ospHipResetKernelStats(void);
 
-/* Node visits, primitive tests and rays of the extension-ray traversals,
- * counted while the device parameter "traversalStats" (bool) is set. */
-OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out3[3]);
+/* Traversal counters of the last frame rendered while the device parameter
+ * "traversalStats" (bool) was set: node visits, primitive tests and
+ * traversals of the extension rays (out6[0..2]) and of the shadow rays
+ * (out6[3..5]). */
+OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out6[6]);
 
 /* Closest hits of n rays (org.xyz, dir.xyz, t0, t1) against a committed
  * world, traced by the device's BVH kernels.  outTUV: t,u,v; outIDs:

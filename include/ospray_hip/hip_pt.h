@@ -37,8 +37,11 @@ HPT_API hipError_t hpt_generate(const HptFrame *F, const HptPaths *P, hipStream_
 HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int residentBlocks, int stats, hipStream_t s);
 /* One samplePath bounce (PathSampler.ispc:143-407) for queue `queue`. */
 HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
-/* transparentShadow of every NEE record (TransparentShadow.ispc:18-85). */
-HPT_API hipError_t hpt_shadow(const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, hipStream_t s);
+/* transparentShadow of every NEE record (TransparentShadow.ispc:18-85);
+ * stats != 0 adds node visits, primitive tests and traversals to
+ * P->stats[3..5]. */
+HPT_API hipError_t hpt_shadow(
+    const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s);
 /* Ordered sum of the NEE samples per vertex (NextEventEstimation.ispc:150-159). */
 HPT_API hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s);
 /* ScreenSample_accumulate over the pass (PathTracer.ispc:170-197). */

@@ -37,21 +37,27 @@ def init(device: str = "hip", **params) -> int:
     if not dev:
         raise OSPRayError(f"ospNewDevice('{device}') failed")
     for k, v in params.items():
-        if isinstance(v, bool):
-            x = C.c_int(int(v))
-            L.ospDeviceSetParam(dev, _b(k), K.OSP_BOOL, C.byref(x))
-        elif isinstance(v, int):
-            x = C.c_int(v)
-            L.ospDeviceSetParam(dev, _b(k), K.OSP_INT, C.byref(x))
-        elif isinstance(v, str):
-            L.ospDeviceSetParam(dev, _b(k), K.OSP_STRING, C.c_char_p(_b(v)))
-        else:
-            raise TypeError(f"device parameter {k}: unsupported type")
+        set_device_param(dev, k, v)
     L.ospDeviceSetErrorCallback(dev, _on_error, None)
     L.ospDeviceCommit(dev)
     L.ospSetCurrentDevice(dev)
     check()
     return dev
+
+
+def set_device_param(dev: int, name: str, v) -> None:
+    """ospDeviceSetParam for bool, int and string values."""
+    L = K.lib()
+    if isinstance(v, bool):
+        x = C.c_int(int(v))
+        L.ospDeviceSetParam(dev, _b(name), K.OSP_BOOL, C.byref(x))
+    elif isinstance(v, int):
+        x = C.c_int(v)
+        L.ospDeviceSetParam(dev, _b(name), K.OSP_INT, C.byref(x))
+    elif isinstance(v, str):
+        L.ospDeviceSetParam(dev, _b(name), K.OSP_STRING, C.c_char_p(_b(v)))
+    else:
+        raise TypeError(f"device parameter {name}: unsupported type")
 
 
 _errors: list = []
@@ -283,7 +289,9 @@ def reset_kernel_stats() -> None:
 
 
 def traversal_stats() -> tuple:
-    out = (C.c_uint64 * 3)()
+    """(node visits, primitive tests, traversals) of the extension rays, then
+    of the shadow rays, of the last frame rendered with "traversalStats"."""
+    out = (C.c_uint64 * 6)()
     K.ext().ospHipGetTraversalStats(out)
     return tuple(int(x) for x in out)
 

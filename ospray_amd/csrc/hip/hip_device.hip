@@ -691,8 +691,20 @@ class HipDevice : public Device
 
   void commitDevice() override
   {
+    deviceID = getInt("deviceID", 0);
+    profile = getBool("profileKernels", false);
+    travStats = getBool("traversalStats", false);
+    const int budgetMB = getInt("pathBudgetMB", 16384);
+    pathBudgetBytes = (size_t)std::max(64, budgetMB) << 20;
+  }
+
+  // GPU state is created on first use, so the object layer works (and
+  // reports errors) on a host without a GPU
+  int deviceID = 0;
+  void gpu()
+  {
     if (!stream) {
-      int dev = getInt("deviceID", 0);
+      const int dev = deviceID;
       HIP_CHECK(hipSetDevice(dev));
       HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       hipDeviceProp_t prop;
@@ -707,10 +719,15 @@ class HipDevice : public Device
       HIP_CHECK(hipMemset(stats.ptr, 0, 64));
       overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
     }
-    profile = getBool("profileKernels", false);
-    travStats = getBool("traversalStats", false);
-    const int budgetMB = getInt("pathBudgetMB", 16384);
-    pathBudgetBytes = (size_t)std::max(64, budgetMB) << 20;
+  }
+
+  void commitObject(ManagedObject *obj) override
+  {
+    // objects that own device memory need the GPU
+    const OSPDataType t = obj->managedType;
+    if (t == OSP_GEOMETRY || t == OSP_WORLD || t == OSP_RENDERER)
+      gpu();
+    Device::commitObject(obj);
   }
 
   // ---- timing ----
@@ -830,6 +847,7 @@ class HipDevice : public Device
   {
     if (w <= 0 || h <= 0)
       throw std::runtime_error("framebuffer has invalid size. Dimensions must be greater than 0");
+    gpu();
     return new FrameBuffer(this, w, h, f, ch);
   }
 
@@ -1389,6 +1407,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   World *w = (World *)wo;
   if (!w->built)
     throw std::runtime_error("ospRenderFrame: the world has not been committed");
+  gpu();
   if (r->commitCounter == 0 || cam->commitCounter == 0)
     throw std::runtime_error("ospRenderFrame: renderer and camera must be committed");
 
@@ -1495,7 +1514,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
         launch("shade", [&]() { return hpt_shade(S, Fp, P, q, shadeBlocks, stream); });
         if (F.numFirstLightSamples > 0) {
-          launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, stream); });
+          launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
           launch("resolve", [&]() { return hpt_resolve(Fp, P, shadeBlocks, stream); });
         }
         q ^= 1;
@@ -1531,6 +1550,7 @@ int HipDevice::traceRays(World *w, const float *rays8, int n, float *outTUV, uin
 {
   if (!w->built)
     throw std::runtime_error("world not committed");
+  gpu();
   DevBuf dr, dt, di;
   dr.ensure((size_t)std::max(n, 1) * 32);
   dt.ensure((size_t)std::max(n, 1) * 12);
@@ -1635,17 +1655,19 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void)
   return 0;
 }
 
-OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out3[3])
+OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out6[6])
 {
   HipDevice *d = current();
   if (!d)
     return -1;
+  if (!d->stream)
+    return -2;
   try {
-    unsigned long long s[3];
+    unsigned long long s[6];
     HIP_CHECK(hipStreamSynchronize(d->stream));
-    HIP_CHECK(hipMemcpy(s, d->stats.ptr, 24, hipMemcpyDeviceToHost));
-    for (int k = 0; k < 3; k++)
-      out3[k] = s[k];
+    HIP_CHECK(hipMemcpy(s, d->stats.ptr, sizeof(s), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; k++)
+      out6[k] = s[k];
   } catch (const std::exception &) {
     return -2;
   }

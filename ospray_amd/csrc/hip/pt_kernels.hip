@@ -43,6 +43,16 @@ __device__ __forceinline__ int enqueue(int32_t *counter, bool pred)
   return pred ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
 }
 
+// wave-reduced counter add: one atomic per wave; every lane of the wave calls it
+__device__ __forceinline__ void waveAdd(unsigned long long *counter, uint32_t v)
+{
+  unsigned long long s = v;
+  for (int o = 32; o > 0; o >>= 1)
+    s += __shfl_xor(s, o);
+  if (laneId() == 0)
+    atomicAdd(counter, s);
+}
+
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
 __device__ __forceinline__ uint4 ld4u(const uint32_t *p) { return *reinterpret_cast<const uint4 *>(p); }
 __device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
@@ -188,9 +198,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
     rays++;
   }
   if (STATS) {
-    atomicAdd(&P.stats[0], (unsigned long long)st.nodes);
-    atomicAdd(&P.stats[1], (unsigned long long)st.prims);
-    atomicAdd(&P.stats[2], (unsigned long long)rays);
+    waveAdd(&P.stats[0], st.nodes);
+    waveAdd(&P.stats[1], st.prims);
+    waveAdd(&P.stats[2], rays);
   }
 }
 
@@ -431,12 +441,14 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
 }
 
 // transparentShadow (TransparentShadow.ispc:18-85) per NEE record
+template <bool STATS>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPaths P)
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[2];
   uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
   TraceStats st = {0, 0};
+  uint32_t rays = 0;
   for (int i = globalId(); i < n; i += gridStride()) {
     const uint32_t r = P.shadowQueue[i];
     const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
@@ -447,7 +459,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPat
     uint32_t medium = __float_as_uint(c4.w);
     Hit h;
     bool sawTransparent;
-    const bool occluded = traceRay<MODE_OCCLUSION, false>(S, org, dir, 0.f, o4.w, h, lds, ovf, sawTransparent, st);
+    const bool occluded = traceRay<MODE_OCCLUSION, STATS>(S, org, dir, 0.f, o4.w, h, lds, ovf, sawTransparent, st);
+    rays++;
     if (occluded)
       continue;
     if (sawTransparent) {
@@ -457,7 +470,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPat
       float t0 = 0.f;
       for (;;) {
         bool dummy;
-        traceRay<MODE_CLOSEST, false>(S, org, dir, t0, tOriginal, h, lds, ovf, dummy, st);
+        traceRay<MODE_CLOSEST, STATS>(S, org, dir, t0, tOriginal, h, lds, ovf, dummy, st);
+        rays++;
         if (!h.hit)
           break;
         DG dg;
@@ -481,6 +495,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPat
       const vec3f res = lc * d4.w;
       st4(P.recResult + 4 * (size_t)r, res.x, res.y, res.z, 1.f);
     }
+  }
+  if (STATS) {
+    waveAdd(&P.stats[3], st.nodes);
+    waveAdd(&P.stats[4], st.prims);
+    waveAdd(&P.stats[5], rays);
   }
 }
 
@@ -740,8 +759,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DScene S, const floa
     outIDs[3 * (size_t)i + 2] = h.hit ? h.prim : kInvalidID;
   }
   if (stats) {
-    atomicAdd(&stats[0], (unsigned long long)st.nodes);
-    atomicAdd(&stats[1], (unsigned long long)st.prims);
+    waveAdd(&stats[0], st.nodes);
+    waveAdd(&stats[1], st.prims);
   }
 }
 
@@ -800,10 +819,16 @@ hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, in
   return hipGetLastError();
 }
 
-hipError_t hpt_shadow(const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, hipStream_t s)
+hipError_t hpt_shadow(
+    const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_shadow, dim3(residentBlocks), dim3(kTraceBlock), 0, s, *reinterpret_cast<const DScene *>(S),
-      *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P));
+  const DScene &sc = *reinterpret_cast<const DScene *>(S);
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  if (stats)
+    hipLaunchKernelGGL(k_shadow<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
+  else
+    hipLaunchKernelGGL(k_shadow<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
   return hipGetLastError();
 }
 

@@ -223,7 +223,8 @@ def c0(width: int = 512, height: int = 512, spp: int = 1, max_path_length: int =
     mats.append({"type": "obj"})
     models.append(Model(geometry=64, material=64))
     lights = [
-        {"type": "quad", "position": (-2.0, 8.0, -2.0), "edge1": (0.0, 0.0, 4.0), "edge2": (4.0, 0.0, 0.0),
+        # emission side is edge1 x edge2 (QuadLight.ispc:23-25): -y, onto the scene
+        {"type": "quad", "position": (-2.0, 8.0, -2.0), "edge1": (4.0, 0.0, 0.0), "edge2": (0.0, 0.0, 4.0),
          "intensity": 10.0},
         {"type": "ambient", "intensity": 0.2},
     ]
@@ -265,8 +266,8 @@ def c1(width: int = 1920, height: int = 1080, spp: int = 4, max_path_length: int
     lights = [{"type": "distant", "direction": tuple(float(v) for v in d / np.linalg.norm(d)), "intensity": 2.5,
                "angularDiameter": 0.53, "color": (1.0, 0.95, 0.85)}]
     for k, (qx, qz) in enumerate([(-20, -20), (20, -20), (-20, 20), (20, 20)]):
-        lights.append({"type": "quad", "position": (qx - 3.0, 25.0, qz - 3.0), "edge1": (0.0, 0.0, 6.0),
-                       "edge2": (6.0, 0.0, 0.0), "intensity": 4.0,
+        lights.append({"type": "quad", "position": (qx - 3.0, 25.0, qz - 3.0), "edge1": (6.0, 0.0, 0.0),
+                       "edge2": (0.0, 0.0, 6.0), "intensity": 4.0,
                        "color": (1.0, 0.9 + 0.03 * k, 0.8 + 0.05 * k)})
     pos = (0.0, 20.0, -60.0)
     return Scene("c1", width, height, geoms, models, groups, insts, mats, lights,

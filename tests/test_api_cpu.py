@@ -1,0 +1,123 @@
+"""The C-ABI boundary on the CPU: exported symbols, module loading, the
+object layer and error behaviour (ospray/api/API.cpp:45-66, 86-153;
+ospray/common/OSPCommon.cpp:749-782; OSPCommon.h:126-133).  No GPU work is
+launched here; commits that need device memory are not exercised."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from ospray_amd import capi as K
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header, macro):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(macro + r"\s+[\w\s\*]+?\b(\w+)\s*\(", text)))
+
+
+def _exports(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_core_exports_every_declared_function():
+    names = _declared("ospray/ospray.h", "OSPRAY_INTERFACE") + _declared("ospray/ospray_util.h", "OSPRAY_INTERFACE")
+    assert len(names) == 77  # ospray.h (51) + ospray_util.h (26) of OSPRay 3.3
+    exported = _exports(os.path.join(K.LIBDIR, "libospray.so"))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+
+
+def test_module_exports_every_declared_function():
+    names = _declared("ospray/ospray_hip.h", "OSPRAY_INTERFACE") + _declared("ospray_hip/hip_pt.h", "HPT_API")
+    assert "hpt_extend" in names and "ospHipTraceRays" in names
+    exported = _exports(os.path.join(K.LIBDIR, "libospray_module_hip.so"))
+    missing = [n for n in names + ["ospray_module_init_hip"] if n not in exported]
+    assert not missing, missing
+
+
+def test_module_version_check():
+    lib = C.CDLL(os.path.join(K.LIBDIR, "libospray_module_hip.so"), mode=C.RTLD_GLOBAL)
+    init = lib.ospray_module_init_hip
+    init.restype = C.c_int
+    init.argtypes = [C.c_int16, C.c_int16, C.c_int16]
+    assert init(2, 0, 0) == K.OSP_VERSION_MISMATCH
+    assert init(3, 2, 0) == K.OSP_VERSION_MISMATCH
+    assert init(3, 3, 0) == K.OSP_NO_ERROR
+
+
+def _run(code):
+    """Run a snippet in a fresh interpreter (the C API keeps global state)."""
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_calls_without_device_fail_cleanly():
+    out = _run(
+        "from ospray_amd import capi as K\n"
+        "L = K.lib()\n"
+        "print(L.ospNewCamera(b'perspective') is None)\n"
+        "print(L.ospGetVariance(None))\n"
+        "print(L.ospLoadModule(b'doesnotexist'))\n"
+        "print(L.ospNewDevice(b'nosuchdevice') is None)\n")
+    lines = out.split()
+    assert lines == ["True", "inf", str(K.OSP_UNKNOWN_ERROR), "True"]
+
+
+def test_object_layer_and_errors():
+    out = _run(
+        "from ospray_amd import api as A, capi as K\n"
+        "A.init('hip')\n"
+        "L = K.lib()\n"
+        "for kind, t in [('camera','perspective'), ('material','obj'), ('material','metal'), ('material','glass'),"
+        " ('light','quad'), ('light','distant'), ('light','ambient')]:\n"
+        "    A.new(kind, t, {})\n"
+        "for kind, t in [('camera','orthographic'), ('renderer','scivis'), ('geometry','curve'), ('light','hdri')]:\n"
+        "    try:\n"
+        "        A.new(kind, t, {})\n"
+        "        print('accepted', kind, t)\n"
+        "    except A.OSPRayError as e:\n"
+        "        print('rejected', 'Could not find' in str(e))\n"
+        "try:\n"
+        "    A.new('material', 'obj', {'ks': (0.5, 0.5, 0.5)})\n"
+        "except A.OSPRayError as e:\n"
+        "    print('ks', 'not supported' in str(e))\n"
+        "cam = A.new('camera', 'perspective', {'fovy': 30.0})\n"
+        "L.ospRelease(cam.h)\n"
+        "print('property', L.ospDeviceGetProperty(L.ospGetCurrentDevice(), 1))\n")
+    lines = out.strip().splitlines()
+    assert lines[:4] == ["rejected True"] * 4
+    assert lines[4] == "ks True"
+    assert lines[5] == "property 3"
+
+
+def test_framebuffer_variance_requires_accum():
+    # ospNewFrameBuffer drops VARIANCE without ACCUM (API.cpp:568-582); the
+    # check happens before any device call, so it is visible in the handle's
+    # channel mask only through behaviour: here just that creation of a
+    # framebuffer of invalid size fails with an error.
+    out = _run(
+        "from ospray_amd import api as A, capi as K\n"
+        "A.init('hip')\n"
+        "h = K.lib().ospNewFrameBuffer(0, 10, K.OSP_FB_RGBA8, K.OSP_FB_COLOR)\n"
+        "print(h is None)\n"
+        "try:\n"
+        "    A.check()\n"
+        "except A.OSPRayError as e:\n"
+        "    print('invalid size' in str(e))\n")
+    assert out.split() == ["True", "True"]
+
+
+def test_c_application_compiles_against_headers(tmp_path):
+    """An application written only against include/ospray/*.h compiles and
+    links against libospray.so (the drop-in boundary)."""
+    src = os.path.join(ROOT, "tests", "apps", "pathtracer_app.c")
+    exe = str(tmp_path / "app")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-o", exe, src, f"-I{ROOT}/include", f"-L{K.LIBDIR}",
+                    "-lospray", f"-Wl,-rpath,{K.LIBDIR}", "-lm"], check=True)
+    assert os.path.exists(exe)
