@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -33,7 +34,7 @@ sys.path.insert(0, ROOT)
 METRIC = "path samples/s at 1080p; plain vs neural-cache time-to-equal-RMSE"
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, PRIM_BYTES, RAY_BYTES = 64, 48, 56
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c1.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_pathtracer_1080p.md")
 
 
 def log(msg: str) -> None:
@@ -69,19 +70,40 @@ def cpu_baseline(sc, target_s: float) -> dict:
         fb.render(osc, cam, ren, nt, (y0, y1))
         return time.perf_counter() - t0, (y1 - y0) * sc.width * spp
 
-    # calibrate on the middle band, then spread the sample over the frame
+    # calibrate on the middle band, then spread the sample over the frame;
+    # whole frames are repeated (successive accumulation frames) while the
+    # sample is shorter than target_s
     dt, _ = band(nbands // 2)
-    count = int(max(2, min(nbands, target_s / max(dt, 1e-4))))
-    picks = sorted({int(round(i * (nbands - 1) / (count - 1))) for i in range(count)})
+    count = int(max(2, target_s / max(dt, 1e-4)))
+    frames, count = divmod(count, nbands) if count >= nbands else (0, count)
+    picks = list(range(nbands)) * frames
+    if count >= 2:
+        picks += sorted({int(round(i * (nbands - 1) / (count - 1))) for i in range(count)})
     total_t = total_n = 0
     for b in picks:
         dt, n = band(b)
         total_t += dt
         total_n += n
     log(f"oracle: {total_n} samples in {total_t:.1f} s on {nt} threads")
-    return {"value": total_n / total_t, "unit": "samples/s", "cores": nt, "kind": "port",
-            "sample": f"{len(picks)} of {nbands} 8-row bands of the same 1920x1080 4-spp frame "
-                      f"({total_n} path samples, {total_t:.1f} s), oracle C++ restatement, {nt} threads"}
+    return {"value": round(total_n / total_t, 1), "unit": "samples/s", "cores": nt, "kind": "port",
+            "sample": f"{len(picks)} 8-row bands ({frames} whole frames + {len(picks) - frames * nbands} bands spread "
+                      f"over the frame) of the 1920x1080 4-spp frame: {total_n} path samples in {total_t:.1f} s; "
+                      f"oracle C++ restatement of the reference path tracer, {nt} threads"}
+
+
+def traffic_per_launch(kernel: str):
+    """Measured HBM bytes per launch of `kernel` from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    pat = re.compile(r"^\|\s*" + re.escape(kernel) + r"\s*\|.*\|\s*hbm_bytes_per_launch=(\d+)\s*\|\s*$")
+    with open(TRAFFIC_FILE) as fh:
+        for line in fh:
+            m = pat.match(line.strip())
+            if m:
+                return int(m.group(1))
+    return None
 
 
 def main() -> int:
@@ -93,6 +115,8 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=4)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only time the plain copy kernel on BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the frame would be split by "
@@ -105,6 +129,13 @@ def main() -> int:
     from ospray_amd import api as A
     from ospray_amd import capi as K
     from ospray_amd import scenes
+
+    if args.copy_calibration:
+        A.init("hip")
+        ms = A.copy_bandwidth(args.copy_calibration, 10)
+        print(json.dumps({"copy_bytes": args.copy_calibration, "ms": ms,
+                          "gbs": 2 * args.copy_calibration / (ms * 1e-3) / 1e9}), flush=True)
+        return 0
 
     t = time.perf_counter()
     sc = scenes.c1(width=args.width, height=args.height, spp=args.spp)
@@ -175,17 +206,14 @@ def main() -> int:
     dom = max(kern, key=lambda k: kern[k]["ms_per_frame"])
     d = kern[dom]
     lpf = d["launches_per_frame"]
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as fh:
-            tj = json.load(fh)
-        if tj.get("kernel") == dom:
-            traffic = tj.get("bytes_per_launch")
+    traffic = traffic_per_launch(f"k_{dom}")
     roofline = {"bound": "hbm", "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}",
                 "bytes_per_launch": int(d["bytes_per_frame"] / lpf),
                 "launch_ms": round(d["ms_per_frame"] / lpf, 4)}
     rays_total = sum(kern[k]["rays_per_frame"] for k in kern)
+    copy_ms = A.copy_bandwidth(1 << 30, 10)
+    roofline["copy_kernel_gbs"] = round(2 * (1 << 30) / (copy_ms * 1e-3) / 1e9, 1)
 
     cpu = None
     if args.cpu_seconds > 0:

@@ -44,6 +44,11 @@ OSPRAY_INTERFACE int ospHipTraceRays(
 OSPRAY_INTERFACE int ospHipGetDeviceInfo(char *name, int len, int *numCUs);
 OSPRAY_INTERFACE int ospHipGetWorldInfo(OSPWorld world, int64_t out4[4]);
 
+/* Mean duration in ms of `reps` device-to-device copies of `bytes` (rounded
+ * down to a multiple of 16) by the device's plain copy kernel: the
+ * achievable-bandwidth reference for the roofline figures. */
+OSPRAY_INTERFACE int ospHipCopyBandwidth(size_t bytes, int reps, float *msPerCopy);
+
 #ifdef __cplusplus
 }
 #endif

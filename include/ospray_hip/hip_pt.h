@@ -30,6 +30,9 @@ typedef struct HptPaths HptPaths; /* wavefront path state, queues and counters *
  * vertices; all are set to 0 except counter `keepIndex` (-1: none). */
 HPT_API hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s);
 HPT_API hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s);
+/* dst <- src, bytes a multiple of 16: a plain streaming copy that calibrates
+ * bandwidth figures and HBM counters (no reference counterpart). */
+HPT_API hipError_t hpt_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 
 /* Camera rays of the pass's samples (PathTracer.ispc:75-170). */
 HPT_API hipError_t hpt_generate(const HptFrame *F, const HptPaths *P, hipStream_t s);

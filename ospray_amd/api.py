@@ -322,3 +322,12 @@ def world_info(world: Handle) -> tuple:
     out = (C.c_int64 * 4)()
     K.ext().ospHipGetWorldInfo(world.h, out)
     return tuple(int(x) for x in out)
+
+
+def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
+    """Mean ms of one device-to-device copy of nbytes by the plain copy kernel."""
+    ms = C.c_float(0.0)
+    if K.ext().ospHipCopyBandwidth(nbytes, reps, C.byref(ms)) != 0:
+        check()
+        raise OSPRayError("ospHipCopyBandwidth failed")
+    return float(ms.value)

@@ -1674,6 +1674,16 @@ OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out6[6])
   return 0;
 }
 
+// errors of the extension entry points reach the application like those of
+// the C API: last error code and message, and the error callback
+static void extensionError(HipDevice *d, const char *msg)
+{
+  d->lastErrorCode = OSP_UNKNOWN_ERROR;
+  d->lastErrorMsg = msg;
+  if (d->errorFcn)
+    d->errorFcn(d->errorUserData, OSP_UNKNOWN_ERROR, msg);
+}
+
 OSPRAY_INTERFACE int ospHipTraceRays(OSPWorld world, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2)
 {
   HipDevice *d = current();
@@ -1682,8 +1692,7 @@ OSPRAY_INTERFACE int ospHipTraceRays(OSPWorld world, const float *rays8, int n, 
   try {
     return d->traceRays((World *)world, rays8, n, outTUV, outIDs, stats2);
   } catch (const std::exception &e) {
-    d->lastErrorCode = OSP_UNKNOWN_ERROR;
-    d->lastErrorMsg = e.what();
+    extensionError(d, e.what());
     return -2;
   }
 }
@@ -1693,6 +1702,11 @@ OSPRAY_INTERFACE int ospHipGetDeviceInfo(char *name, int len, int *numCUs)
   HipDevice *d = current();
   if (!d)
     return -1;
+  try {
+    d->gpu();
+  } catch (const std::exception &) {
+    return -2;
+  }
   if (name && len > 0) {
     std::strncpy(name, d->gpuName.c_str(), len - 1);
     name[len - 1] = 0;
@@ -1700,6 +1714,46 @@ OSPRAY_INTERFACE int ospHipGetDeviceInfo(char *name, int len, int *numCUs)
   if (numCUs)
     *numCUs = d->numCUs;
   return 0;
+}
+
+OSPRAY_INTERFACE int ospHipCopyBandwidth(size_t bytes, int reps, float *msPerCopy)
+{
+  HipDevice *d = current();
+  if (!d || reps < 1 || bytes < 16 || !msPerCopy)
+    return -1;
+  void *src = nullptr, *dst = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  int rc = 0;
+  try {
+    d->gpu();
+    bytes &= ~(size_t)15;
+    HIP_CHECK(hipMalloc(&src, bytes));
+    HIP_CHECK(hipMalloc(&dst, bytes));
+    HIP_CHECK(hpt_fill_u32((uint32_t *)src, bytes / 4, 0x3f800000u, d->stream));
+    HIP_CHECK(hpt_copy(dst, src, bytes, d->stream)); // untimed first touch
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    HIP_CHECK(hipEventRecord(a, d->stream));
+    for (int i = 0; i < reps; i++)
+      HIP_CHECK(hpt_copy(dst, src, bytes, d->stream));
+    HIP_CHECK(hipEventRecord(b, d->stream));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    *msPerCopy = ms / reps;
+  } catch (const std::exception &e) {
+    extensionError(d, e.what());
+    rc = -2;
+  }
+  if (a)
+    (void)hipEventDestroy(a);
+  if (b)
+    (void)hipEventDestroy(b);
+  if (src)
+    (void)hipFree(src);
+  if (dst)
+    (void)hipFree(dst);
+  return rc;
 }
 
 OSPRAY_INTERFACE int ospHipGetWorldInfo(OSPWorld world, int64_t out4[4])

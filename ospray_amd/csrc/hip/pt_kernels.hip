@@ -114,6 +114,14 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
     p[i] = v;
 }
 
+// plain device copy, 16 B per lane and iteration (bandwidth and HBM-counter
+// calibration)
+__global__ __launch_bounds__(256) void k_copy(uint4 *dst, const uint4 *src, size_t n16)
+{
+  for (size_t i = globalId(); i < n16; i += gridStride())
+    dst[i] = src[i];
+}
+
 // PathTracer_renderPixel up to the first samplePath iteration
 // (PathTracer.ispc:75-170) for samples [passS0, passS0+passCount)
 __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
@@ -790,6 +798,15 @@ hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s)
 hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s)
 {
   hipLaunchKernelGGL(k_fill_u32, dim3(gridFor(n, 256, 4096)), dim3(256), 0, s, ptr, n, value);
+  return hipGetLastError();
+}
+
+hipError_t hpt_copy(void *dst, const void *src, size_t bytes, hipStream_t s)
+{
+  if (bytes % 16 != 0)
+    return hipErrorInvalidValue;
+  const size_t n16 = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(gridFor(n16, 256, 16384)), dim3(256), 0, s, (uint4 *)dst, (const uint4 *)src, n16);
   return hipGetLastError();
 }
 
