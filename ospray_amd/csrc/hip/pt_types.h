@@ -166,7 +166,7 @@ struct DPaths
   uint32_t *neeQueue;
   int32_t *counters; // [0..1] extension, [2] shadow, [3] nee, [4..7] spare
   uint32_t *overflow; // traversal stack overflow, per resident thread
-  unsigned long long *stats; // [0] node visits, [1] primitive tests, [2] rays
+  unsigned long long *stats; // node visits, primitive tests, traversals: [0..2] extension, [3..5] shadow
 };
 
 // path flags in ld[3]
