@@ -27,7 +27,8 @@ typedef struct HptPaths HptPaths; /* wavefront path state, queues and counters *
 #define HPT_API __attribute__((visibility("default")))
 
 /* Queue counters: [0],[1] extension queues, [2] shadow records, [3] NEE
- * vertices; all are set to 0 except counter `keepIndex` (-1: none). */
+ * vertices, [4],[5] extension/shadow work fetch, [6] transparency queue;
+ * all are set to 0 except counter `keepIndex` (-1: none). */
 HPT_API hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s);
 HPT_API hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s);
 /* dst <- src, bytes a multiple of 16: a plain streaming copy that calibrates
@@ -40,7 +41,9 @@ HPT_API hipError_t hpt_generate(const HptFrame *F, const HptPaths *P, hipStream_
 HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int residentBlocks, int stats, hipStream_t s);
 /* One samplePath bounce (PathSampler.ispc:143-407) for queue `queue`. */
 HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
-/* transparentShadow of every NEE record (TransparentShadow.ispc:18-85);
+/* transparentShadow of every NEE record (TransparentShadow.ispc:18-85):
+ * an occlusion pass, then the closest-hit transparency loop for the records
+ * that hit only non-opaque surfaces;
  * stats != 0 adds node visits, primitive tests and traversals to
  * P->stats[3..5]. */
 HPT_API hipError_t hpt_shadow(

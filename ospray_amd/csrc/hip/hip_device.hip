@@ -694,6 +694,12 @@ class HipDevice : public Device
     deviceID = getInt("deviceID", 0);
     profile = getBool("profileKernels", false);
     travStats = getBool("traversalStats", false);
+    traceBlocksPerCU = std::max(1, std::min(64, getInt("traceBlocksPerCU", traceBlocksPerCU)));
+    shadeBlocksPerCU = std::max(1, std::min(64, getInt("shadeBlocksPerCU", shadeBlocksPerCU)));
+    if (stream) {
+      HIP_CHECK(hipStreamSynchronize(stream));
+      sizeGrids();
+    }
     const int budgetMB = getInt("pathBudgetMB", 16384);
     pathBudgetBytes = (size_t)std::max(64, budgetMB) << 20;
   }
@@ -711,14 +717,22 @@ class HipDevice : public Device
       HIP_CHECK(hipGetDeviceProperties(&prop, dev));
       numCUs = prop.multiProcessorCount;
       gpuName = prop.gcnArchName;
-      residentTraceBlocks = std::max(1, numCUs) * 8;
-      shadeBlocks = std::max(1, numCUs) * 8;
       HIP_CHECK(hipMalloc(&counters.ptr, 64));
       counters.bytes = 64;
       stats.ensure(64);
       HIP_CHECK(hipMemset(stats.ptr, 0, 64));
-      overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
+      sizeGrids();
     }
+  }
+
+  // resident grids of the bounce kernels (grid-stride loops) and the
+  // traversal-stack overflow rows they need
+  int traceBlocksPerCU = 12, shadeBlocksPerCU = 8;
+  void sizeGrids()
+  {
+    residentTraceBlocks = std::max(1, numCUs) * traceBlocksPerCU;
+    shadeBlocks = std::max(1, numCUs) * shadeBlocksPerCU;
+    overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
   }
 
   void commitObject(ManagedObject *obj) override
@@ -1360,7 +1374,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   const size_t perSlot = 9 * 16 + 4 + 16 + 4 + (auxAllocated ? 64 : 0);
   pathBuf.ensure(S * perSlot);
   recBuf.ensure(R * 64);
-  queueBuf.ensure((2 * S + R + S) * 4);
+  queueBuf.ensure((3 * S + 2 * R) * 4);
   char *b = (char *)pathBuf.ptr;
   auto take = [&](size_t bytes) {
     char *p = b;
@@ -1391,6 +1405,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.queue[1] = q + S;
   paths.shadowQueue = q + 2 * S;
   paths.neeQueue = q + 2 * S + R;
+  paths.transQueue = q + 3 * S + R;
   paths.counters = (int32_t *)counters.ptr;
   paths.overflow = (uint32_t *)overflow.ptr;
   paths.stats = (unsigned long long *)stats.ptr;
@@ -1492,11 +1507,12 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
 
   if (!finishedAccum) {
     const bool aux = F.accAux != nullptr;
-    const size_t perSlotBytes = 9 * 16 + 40 + (aux ? 64 : 0) + (size_t)F.maxRec * 68 + 12;
+    const size_t perSlotBytes = 9 * 16 + 40 + (aux ? 64 : 0) + (size_t)F.maxRec * 72 + 12;
     const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
     const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
     ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
     paths.aux = aux ? auxBase : nullptr;
+    paths.overflow = (uint32_t *)overflow.ptr;
     F.residentThreads = residentTraceBlocks * hpt_trace_block_size();
     const HptScene *S = reinterpret_cast<const HptScene *>(&w->scene);
     const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);

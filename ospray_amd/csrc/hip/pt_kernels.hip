@@ -183,6 +183,17 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
   }
 }
 
+// Work distribution of the traversal kernels: each wave takes batches of
+// 64 consecutive queue entries (neighbouring pixels' rays, which share node
+// fetches) with one atomic per batch, until the queue is exhausted.
+__device__ __forceinline__ int nextBatch(int32_t *counter)
+{
+  int base = 0;
+  if (laneId() == 0)
+    base = atomicAdd(counter, 64);
+  return __shfl(base, 0);
+}
+
 // closest hit of every queued path (the rtcIntersectV of
 // traceGeometryRayIntervals, World.ih:247-286)
 template <bool STATS>
@@ -193,17 +204,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
   uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
-  for (int i = globalId(); i < n; i += gridStride()) {
-    const uint32_t p = P.queue[qin][i];
-    const float4 o = ld4(P.org + 4 * (size_t)p);
-    const float4 d = ld4(P.dir + 4 * (size_t)p);
-    Hit h;
-    bool sawT;
-    traceRay<MODE_CLOSEST, STATS>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds, ovf, sawT, st);
-    st4(P.hitA + 4 * (size_t)p, h.hit ? h.t : kInf, h.u, h.v, __uint_as_float(h.hit ? h.inst : kInvalidID));
-    st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
-    P.hitPrim[p] = h.prim | (h.half << 31);
-    rays++;
+  for (int base = nextBatch(&P.counters[4]); base < n; base = nextBatch(&P.counters[4])) {
+    const int i = base + laneId();
+    if (i < n) {
+      const uint32_t p = P.queue[qin][i];
+      const float4 o = ld4(P.org + 4 * (size_t)p);
+      const float4 d = ld4(P.dir + 4 * (size_t)p);
+      Hit h;
+      bool sawT;
+      traceRay<MODE_CLOSEST, STATS>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds, ovf, sawT, st);
+      st4(P.hitA + 4 * (size_t)p, h.hit ? h.t : kInf, h.u, h.v, __uint_as_float(h.hit ? h.inst : kInvalidID));
+      st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
+      P.hitPrim[p] = h.prim | (h.half << 31);
+      rays++;
+    }
   }
   if (STATS) {
     waveAdd(&P.stats[0], st.nodes);
@@ -448,7 +462,10 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
   }
 }
 
-// transparentShadow (TransparentShadow.ispc:18-85) per NEE record
+// transparentShadow (TransparentShadow.ispc:18-85) per NEE record, first
+// pass: occlusion traversal with early exit on an opaque primitive.
+// Unoccluded records get their result; records that hit only non-opaque
+// primitives go to the transparency queue (k_shadow_trans).
 template <bool STATS>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPaths P)
 {
@@ -457,47 +474,87 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPat
   uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
+  for (int base = nextBatch(&P.counters[5]); base < n; base = nextBatch(&P.counters[5])) {
+    const int i = base + laneId();
+    bool toTrans = false;
+    uint32_t r = 0;
+    if (i < n) {
+      r = P.shadowQueue[i];
+      const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
+      const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
+      Hit h;
+      bool sawT;
+      const bool occluded =
+          traceRay<MODE_OCCLUSION, STATS>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 0.f, o4.w, h, lds, ovf, sawT, st);
+      rays++;
+      if (!occluded) {
+        if (sawT) {
+          toTrans = true;
+        } else {
+          const float4 c4 = ld4(P.recContrib + 4 * (size_t)r);
+          const vec3f lc = v3(c4.x, c4.y, c4.z);
+          if (reduce_max(lc) > 0) {
+            const vec3f res = lc * d4.w;
+            st4(P.recResult + 4 * (size_t)r, res.x, res.y, res.z, 1.f);
+          }
+        }
+      }
+    }
+    const int slot = enqueue(&P.counters[6], toTrans);
+    if (toTrans)
+      P.transQueue[slot] = r;
+  }
+  if (STATS) {
+    waveAdd(&P.stats[3], st.nodes);
+    waveAdd(&P.stats[4], st.prims);
+    waveAdd(&P.stats[5], rays);
+  }
+}
+
+// transparentShadow second pass: the exact closest-hit loop through
+// non-opaque surfaces (TransparentShadow.ispc:36-84) for the records of the
+// transparency queue
+template <bool STATS>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F, DPaths P)
+{
+  __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
+  const int n = P.counters[6];
+  uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
+  TraceStats st = {0, 0};
+  uint32_t rays = 0;
   for (int i = globalId(); i < n; i += gridStride()) {
-    const uint32_t r = P.shadowQueue[i];
+    const uint32_t r = P.transQueue[i];
     const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
     const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
     const float4 c4 = ld4(P.recContrib + 4 * (size_t)r);
     const vec3f org = v3(o4.x, o4.y, o4.z), dir = v3(d4.x, d4.y, d4.z);
     vec3f lc = v3(c4.x, c4.y, c4.z);
     uint32_t medium = __float_as_uint(c4.w);
+    uint32_t maxDepth = (uint32_t)F.maxDepth;
+    const float tOriginal = o4.w;
+    float t0 = 0.f;
     Hit h;
-    bool sawTransparent;
-    const bool occluded = traceRay<MODE_OCCLUSION, STATS>(S, org, dir, 0.f, o4.w, h, lds, ovf, sawTransparent, st);
-    rays++;
-    if (occluded)
-      continue;
-    if (sawTransparent) {
-      // only non-opaque surfaces in the way: exact closest-hit loop
-      uint32_t maxDepth = (uint32_t)F.maxDepth;
-      const float tOriginal = o4.w;
-      float t0 = 0.f;
-      for (;;) {
-        bool dummy;
-        traceRay<MODE_CLOSEST, STATS>(S, org, dir, t0, tOriginal, h, lds, ovf, dummy, st);
-        rays++;
-        if (!h.hit)
-          break;
-        DG dg;
-        postIntersect(S, org, dir, h, dg, false);
-        const vec3f tr = getTransparency(S, dg, dir, S.media[medium]);
-        lc = lc * tr;
-        const DMedium &cm = S.media[medium];
-        if (!(cm.attenuation[0] == 0.f && cm.attenuation[1] == 0.f && cm.attenuation[2] == 0.f)) {
-          const vec3f a = ld3(cm.attenuation) * (h.t - t0);
-          lc = lc * v3(fexp(a.x), fexp(a.y), fexp(a.z));
-        }
-        if (reduce_max(lc) <= F.minContribution || --maxDepth <= 0) {
-          lc = v3(0.f);
-          break;
-        }
-        medium = selectNextMedium(S, dg, medium);
-        t0 = h.t + dg.epsilon;
+    for (;;) {
+      bool dummy;
+      traceRay<MODE_CLOSEST, STATS>(S, org, dir, t0, tOriginal, h, lds, ovf, dummy, st);
+      rays++;
+      if (!h.hit)
+        break;
+      DG dg;
+      postIntersect(S, org, dir, h, dg, false);
+      const vec3f tr = getTransparency(S, dg, dir, S.media[medium]);
+      lc = lc * tr;
+      const DMedium &cm = S.media[medium];
+      if (!(cm.attenuation[0] == 0.f && cm.attenuation[1] == 0.f && cm.attenuation[2] == 0.f)) {
+        const vec3f a = ld3(cm.attenuation) * (h.t - t0);
+        lc = lc * v3(fexp(a.x), fexp(a.y), fexp(a.z));
       }
+      if (reduce_max(lc) <= F.minContribution || --maxDepth <= 0) {
+        lc = v3(0.f);
+        break;
+      }
+      medium = selectNextMedium(S, dg, medium);
+      t0 = h.t + dg.epsilon;
     }
     if (reduce_max(lc) > 0) {
       const vec3f res = lc * d4.w;
@@ -842,10 +899,13 @@ hipError_t hpt_shadow(
   const DScene &sc = *reinterpret_cast<const DScene *>(S);
   const DFrame &f = *reinterpret_cast<const DFrame *>(F);
   const DPaths &p = *reinterpret_cast<const DPaths *>(P);
-  if (stats)
+  if (stats) {
     hipLaunchKernelGGL(k_shadow<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
-  else
+    hipLaunchKernelGGL(k_shadow_trans<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
+  } else {
     hipLaunchKernelGGL(k_shadow<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
+    hipLaunchKernelGGL(k_shadow_trans<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
+  }
   return hipGetLastError();
 }
 

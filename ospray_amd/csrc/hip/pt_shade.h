@@ -410,12 +410,12 @@ HPT_HD SampleRes bsdfSample(const BSDF &b, vec3f wo, vec2f s, float ss)
     if (eq(res.weight, v3(0.f)) || res.pdf == 0.f)
       return zero;
     if (res.type & S_SPECULAR) {
-      res.weight = res.weight * (b.importanceSum * rcp(b.importance[choice]));
+      res.weight = res.weight * (b.importanceSum * rcp(choice == 0 ? b.importance[0] : b.importance[1]));
       return res;
     }
     EvalRes e;
     e.value = res.weight * res.pdf;
-    res.pdf *= b.importance[choice];
+    res.pdf *= choice == 0 ? b.importance[0] : b.importance[1];
     e.pdf = res.pdf;
     const vec3f wi = res.wi;
     if (b.importance[0] > 0.f && choice != 0) {

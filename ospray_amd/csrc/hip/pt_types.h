@@ -164,7 +164,10 @@ struct DPaths
   uint32_t *queue[2]; // extension queues (ping-pong)
   uint32_t *shadowQueue;
   uint32_t *neeQueue;
-  int32_t *counters; // [0..1] extension, [2] shadow, [3] nee, [4..7] spare
+  uint32_t *transQueue; // shadow records that hit only non-opaque surfaces
+  // [0..1] extension queues, [2] shadow records, [3] NEE vertices, [4] and
+  // [5] extension/shadow work fetch, [6] transparency queue, [7] spare
+  int32_t *counters;
   uint32_t *overflow; // traversal stack overflow, per resident thread
   unsigned long long *stats; // node visits, primitive tests, traversals: [0..2] extension, [3..5] shadow
 };
