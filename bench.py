@@ -10,7 +10,8 @@ API, the scene already committed and resident in HBM.  Prints one JSON line.
   value       whole-job path samples/s (1920*1080*4 per step / wall time)
   roofline    the dominant kernel (closest-hit traversal of the extension or
               the shadow rays) against MI355X's 8 TB/s: algorithmic bytes per
-              launch = 64 B per BVH node visit + 48 B per primitive test +
+              launch = 128 B per BVH node visit (4-wide node) + 48 B per
+              primitive test +
               56 B per ray (32 B ray in, 24 B hit out), with node and
               primitive counts from the device's traversal counters on one
               extra frame, over the kernel's mean launch duration from HIP
@@ -33,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "path samples/s at 1080p; plain vs neural-cache time-to-equal-RMSE"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, PRIM_BYTES, RAY_BYTES = 64, 48, 56
+NODE_BYTES, PRIM_BYTES, RAY_BYTES = 128, 48, 56
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_pathtracer_1080p.md")
 
 

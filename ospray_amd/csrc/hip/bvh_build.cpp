@@ -1,4 +1,4 @@
-// Binned-SAH BVH2 builder (host) producing hpt::DNode child-pair nodes.
+// Binned-SAH BVH builder (host) producing 4-wide hpt::DNode4 nodes.
 #include "bvh_build.h"
 
 #include <algorithm>
@@ -193,13 +193,14 @@ BuildResult buildBVH(const std::vector<BuildBox> &boxes, int maxLeaf, int nthrea
   R.order.resize(boxes.size());
   for (size_t i = 0; i < boxes.size(); i++)
     R.order[i] = (uint32_t)i;
-  DNode emptyNode;
-  for (int k = 0; k < 3; k++) {
-    emptyNode.b0lo[k] = emptyNode.b0hi[k] = kInfF;
-    emptyNode.b1lo[k] = emptyNode.b1hi[k] = kInfF;
+  DNode4 emptyNode;
+  for (int c = 0; c < 4; c++) {
+    emptyNode.lox[c] = emptyNode.hix[c] = kInfF;
+    emptyNode.loy[c] = emptyNode.hiy[c] = kInfF;
+    emptyNode.loz[c] = emptyNode.hiz[c] = kInfF;
+    emptyNode.child[c] = (int32_t)~0u;
+    emptyNode.pad[c] = 0;
   }
-  emptyNode.child[0] = emptyNode.child[1] = (int32_t)~0u;
-  emptyNode.pad[0] = emptyNode.pad[1] = 0;
   if (boxes.empty()) {
     R.nodes.push_back(emptyNode);
     R.bounds = emptyBox();
@@ -278,67 +279,86 @@ BuildResult buildBVH(const std::vector<BuildBox> &boxes, int maxLeaf, int nthrea
     return {tree, idx};
   };
 
-  // flatten depth-first into child-pair nodes
+  // collapse into 4-wide nodes, emitted depth-first
   const auto root = resolve(-1, 0);
   R.bounds = get(root.first, root.second).box;
-  struct Emit
-  {
-    int tree;
-    int32_t idx;
-    int32_t out;
-  };
-  auto setChild = [&](DNode &n, int c, const TNode &t) {
-    float *lo = c ? n.b1lo : n.b0lo;
-    float *hi = c ? n.b1hi : n.b0hi;
-    for (int k = 0; k < 3; k++) {
-      lo[k] = t.box.lo[k];
-      hi[k] = t.box.hi[k];
-    }
+  using Ref = std::pair<int, int32_t>;
+  auto setChild = [&](DNode4 &n, int c, const TNode &t) {
+    n.lox[c] = t.box.lo[0];
+    n.loy[c] = t.box.lo[1];
+    n.loz[c] = t.box.lo[2];
+    n.hix[c] = t.box.hi[0];
+    n.hiy[c] = t.box.hi[1];
+    n.hiz[c] = t.box.hi[2];
   };
   auto leafCode = [](uint32_t first, uint32_t count) -> int32_t {
     return (int32_t) ~((first << 3) | (count - 1u));
   };
+  auto area = [](const BuildBox &b) {
+    const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+  };
   const TNode &r0 = get(root.first, root.second);
   if (r0.left < 0) {
-    DNode n = emptyNode;
+    DNode4 n = emptyNode;
     setChild(n, 0, r0);
     n.child[0] = leafCode(r0.first, r0.count);
     R.nodes.push_back(n);
     R.maxDepth = 1;
     return R;
   }
-  R.nodes.reserve(boxes.size());
+  struct Emit
+  {
+    Ref ref;
+    int32_t out;
+    int need; // stack entries pushed above this node
+  };
+  R.nodes.reserve(boxes.size() / 2 + 1);
   std::vector<Emit> stack;
   R.nodes.push_back(emptyNode);
-  stack.push_back({root.first, root.second, 0});
+  stack.push_back({root, 0, 0});
   while (!stack.empty()) {
     const Emit e = stack.back();
     stack.pop_back();
-    const TNode &tn = get(e.tree, e.idx);
-    const std::pair<int, int32_t> kids[2] = {resolve(e.tree, tn.left), resolve(e.tree, tn.right)};
-    DNode n = emptyNode;
-    int32_t pushOut[2] = {-1, -1};
-    for (int c = 0; c < 2; c++) {
+    const TNode &tn = get(e.ref.first, e.ref.second);
+    Ref kids[4] = {resolve(e.ref.first, tn.left), resolve(e.ref.first, tn.right), {-1, -1}, {-1, -1}};
+    int nk = 2;
+    while (nk < 4) {
+      int best = -1;
+      float bestArea = -1.f;
+      for (int c = 0; c < nk; c++) {
+        const TNode &t = get(kids[c].first, kids[c].second);
+        if (t.left >= 0 && area(t.box) > bestArea) {
+          bestArea = area(t.box);
+          best = c;
+        }
+      }
+      if (best < 0)
+        break;
+      const TNode &t = get(kids[best].first, kids[best].second);
+      const int tree = kids[best].first;
+      kids[best] = resolve(tree, t.left);
+      kids[nk++] = resolve(tree, t.right);
+    }
+    const int need = e.need + (nk - 1);
+    R.stackNeed = std::max(R.stackNeed, need);
+    DNode4 n = emptyNode;
+    int32_t slot[4] = {-1, -1, -1, -1};
+    for (int c = 0; c < nk; c++) {
       const TNode &ch = get(kids[c].first, kids[c].second);
       setChild(n, c, ch);
       if (ch.left < 0) {
         n.child[c] = leafCode(ch.first, ch.count);
       } else {
-        pushOut[c] = (int32_t)R.nodes.size() + (c == 1 && pushOut[0] >= 0 ? 1 : 0);
+        slot[c] = (int32_t)R.nodes.size();
+        R.nodes.push_back(emptyNode);
+        n.child[c] = slot[c];
       }
     }
-    // reserve slots: children nodes are emitted depth-first; allocate now
-    for (int c = 0; c < 2; c++)
-      if (pushOut[c] >= 0) {
-        const int32_t idx = (int32_t)R.nodes.size();
-        R.nodes.push_back(emptyNode);
-        n.child[c] = idx;
-        pushOut[c] = idx;
-      }
     R.nodes[e.out] = n;
-    for (int c = 1; c >= 0; c--)
-      if (pushOut[c] >= 0)
-        stack.push_back({kids[c].first, kids[c].second, pushOut[c]});
+    for (int c = nk - 1; c >= 0; c--)
+      if (slot[c] >= 0)
+        stack.push_back({kids[c], slot[c], need});
   }
   R.maxDepth = maxDepth + 1;
   return R;

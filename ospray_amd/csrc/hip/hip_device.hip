@@ -456,10 +456,11 @@ struct Light : public ManagedObject
 // BLAS of one list of geometries (shared by groups with the same list)
 struct Blas
 {
-  std::vector<hpt::DNode> nodes;
+  std::vector<hpt::DNode4> nodes;
   std::vector<hpt::DPrim> prims;
   hpt::BuildBox bounds;
   int maxDepth = 0;
+  int stackNeed = 0;
 };
 
 struct World : public ManagedObject
@@ -473,7 +474,7 @@ struct World : public ManagedObject
   hpt::DScene scene{};
   bool built = false;
   int64_t totalNodes = 0, totalPrims = 0;
-  int maxDepth = 0;
+  int maxDepth = 0, stackNeed = 0;
   float bounds[6] = {0, 0, 0, 0, 0, 0};
   World(HipDevice *d) : dev(d) { managedType = OSP_WORLD; }
   std::string toString() const override { return "ospray::World"; }
@@ -1074,6 +1075,7 @@ static std::shared_ptr<Blas> buildBlas(const std::vector<Geometry *> &geoms)
   B->nodes = std::move(R.nodes);
   B->bounds = R.bounds;
   B->maxDepth = R.maxDepth;
+  B->stackNeed = R.stackNeed;
   B->prims.resize(R.order.size());
   for (size_t k = 0; k < R.order.size(); k++) {
     const Item &it = items[R.order[k]];
@@ -1224,12 +1226,14 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   std::vector<int> blasNodeOff(uniqueBlas.size()), blasPrimOff(uniqueBlas.size());
   size_t nNodes = 0, nPrims = 0;
   maxDepth = 0;
+  int blasStackNeed = 0;
   for (size_t k = 0; k < uniqueBlas.size(); k++) {
     blasNodeOff[k] = (int)nNodes;
     blasPrimOff[k] = (int)nPrims;
     nNodes += uniqueBlas[k]->nodes.size();
     nPrims += uniqueBlas[k]->prims.size();
     maxDepth = std::max(maxDepth, uniqueBlas[k]->maxDepth);
+    blasStackNeed = std::max(blasStackNeed, uniqueBlas[k]->stackNeed);
   }
   // instances and TLAS
   std::vector<hpt::DInstance> dinst(instances.size());
@@ -1286,12 +1290,12 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   for (size_t k = 0; k < T.order.size(); k++)
     tlasOrder[k] = tlasInst[T.order[k]];
   const int tlasOff = (int)nNodes;
-  std::vector<hpt::DNode> nodes(nNodes + T.nodes.size());
+  std::vector<hpt::DNode4> nodes(nNodes + T.nodes.size());
   std::vector<hpt::DPrim> prims(nPrims);
-  auto remap = [](hpt::DNode n, int nodeOff, int primOff, const std::vector<uint32_t> *order) {
-    for (int c = 0; c < 2; c++) {
-      // an empty child (box at +inf) is never entered: leave it alone
-      if ((c ? n.b1lo[0] : n.b0lo[0]) == hpt::kInf)
+  auto remap = [](hpt::DNode4 n, int nodeOff, int primOff, const std::vector<uint32_t> *order) {
+    for (int c = 0; c < 4; c++) {
+      // an unused slot (box at +inf) is never entered: leave it alone
+      if (n.lox[c] == hpt::kInf)
         continue;
       int32_t ch = n.child[c];
       if (ch >= 0) {
@@ -1331,7 +1335,7 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   dMedia.upload(media);
   dLights.upload(dl);
   dLightsCDF.upload(cdf);
-  scene.nodes = dNodes.as<const hpt::DNode>();
+  scene.nodes = dNodes.as<const hpt::DNode4>();
   scene.prims = dPrims.as<const hpt::DPrim>();
   scene.instances = dInstances.as<const hpt::DInstance>();
   scene.models = dModels.as<const hpt::DModel>();
@@ -1349,10 +1353,11 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
     bounds[k] = T.bounds.lo[k];
     bounds[3 + k] = T.bounds.hi[k];
   }
-  // the stack holds at most one entry per level of TLAS and BLAS plus the
-  // instance marker
-  if (maxDepth + T.maxDepth + 2 > hpt_stack_capacity())
+  // the traversal stack holds the TLAS entries, the instance marker and the
+  // BLAS entries of one root-to-leaf path
+  if (T.stackNeed + 1 + blasStackNeed > hpt_stack_capacity())
     throw std::runtime_error("BVH too deep for the traversal stack");
+  stackNeed = T.stackNeed + 1 + blasStackNeed;
   maxDepth = std::max(maxDepth, T.maxDepth);
   built = true;
 }
@@ -1370,11 +1375,12 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   recSlots = std::max((size_t)maxRec * pathSlots, recSlots);
   auxAllocated = auxAllocated || aux;
   const size_t S = pathSlots, R = recSlots;
-  // per-slot arrays: 9 x float4 + hitPrim + ids(uint4) + neeN (+ aux 16 floats)
-  const size_t perSlot = 9 * 16 + 4 + 16 + 4 + (auxAllocated ? 64 : 0);
+  // per-slot arrays: 9 x float4 + hitPrim + ids(uint4) + neeN + NEE vertex
+  // record (6 x float4) (+ aux 16 floats)
+  const size_t perSlot = 9 * 16 + 4 + 16 + 4 + 6 * 16 + (auxAllocated ? 64 : 0);
   pathBuf.ensure(S * perSlot);
   recBuf.ensure(R * 64);
-  queueBuf.ensure((3 * S + 2 * R) * 4);
+  queueBuf.ensure((3 * S + 3 * R) * 4);
   char *b = (char *)pathBuf.ptr;
   auto take = [&](size_t bytes) {
     char *p = b;
@@ -1393,6 +1399,12 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.ids = (uint32_t *)take(S * 16);
   paths.hitPrim = (uint32_t *)take(S * 4);
   paths.neeN = (uint32_t *)take(S * 4);
+  paths.vtxP = (float *)take(S * 16);
+  paths.vtxNg = (float *)take(S * 16);
+  paths.vtxNs = (float *)take(S * 16);
+  paths.vtxThr = (float *)take(S * 16);
+  paths.vtxWo = (float *)take(S * 16);
+  paths.vtxLd = (uint32_t *)take(S * 16);
   auxBase = auxAllocated ? (float *)take(S * 64) : nullptr;
   paths.aux = auxBase;
   char *r = (char *)recBuf.ptr;
@@ -1406,6 +1418,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.shadowQueue = q + 2 * S;
   paths.neeQueue = q + 2 * S + R;
   paths.transQueue = q + 3 * S + R;
+  paths.neeSampleQueue = q + 3 * S + 2 * R;
   paths.counters = (int32_t *)counters.ptr;
   paths.overflow = (uint32_t *)overflow.ptr;
   paths.stats = (unsigned long long *)stats.ptr;
@@ -1507,7 +1520,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
 
   if (!finishedAccum) {
     const bool aux = F.accAux != nullptr;
-    const size_t perSlotBytes = 9 * 16 + 40 + (aux ? 64 : 0) + (size_t)F.maxRec * 72 + 12;
+    const size_t perSlotBytes = 9 * 16 + 40 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 76 + 12;
     const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
     const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
     ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
@@ -1530,6 +1543,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
         launch("shade", [&]() { return hpt_shade(S, Fp, P, q, shadeBlocks, stream); });
         if (F.numFirstLightSamples > 0) {
+          launch("nee", [&]() { return hpt_nee(S, Fp, P, shadeBlocks, stream); });
           launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
           launch("resolve", [&]() { return hpt_resolve(Fp, P, shadeBlocks, stream); });
         }

@@ -53,6 +53,28 @@ __device__ __forceinline__ void waveAdd(unsigned long long *counter, uint32_t v)
     atomicAdd(counter, s);
 }
 
+// wave-aggregated append of `count` entries per lane (inclusive scan over
+// the wave, one atomic); every lane of the wave calls it.  Returns the
+// lane's first slot.
+__device__ __forceinline__ int enqueueN(int32_t *counter, uint32_t count)
+{
+  const int lane = laneId();
+  uint32_t x = count;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o)
+      x += y;
+  }
+  const uint32_t total = __shfl(x, 63);
+  if (total == 0)
+    return 0;
+  int base = 0;
+  if (lane == 63)
+    base = atomicAdd(counter, (int)total);
+  base = __shfl(base, 63);
+  return base + (int)(x - count);
+}
+
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
 __device__ __forceinline__ uint4 ld4u(const uint32_t *p) { return *reinterpret_cast<const uint4 *>(p); }
 __device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
@@ -227,238 +249,290 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
 }
 
 // one iteration of samplePath (PathSampler.ispc:143-407) for every queued
-// path; NEE (NextEventEstimation.ispc:25-160) emits shadow-ray records
+// path.  At a smooth vertex it stores the NEE vertex record and appends the
+// vertex's light-sample slots to the NEE sample queue; k_nee evaluates them
+// (NextEventEstimation.ispc:25-160).  The loop runs over whole waves so the
+// appends can scan across the wave.
 __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int qin)
 {
   const int n = P.counters[qin];
   const int qout = qin ^ 1;
-  for (int i = globalId(); i < n; i += gridStride()) {
-    const uint32_t p = P.queue[qin][i];
-    const size_t p4 = 4 * (size_t)p;
-    const float4 o4 = ld4(P.org + p4);
-    const float4 d4 = ld4(P.dir + p4);
-    float4 thr4 = ld4(P.thr + p4);
-    float4 con4 = ld4(P.contrib + p4);
-    uint4 ld4v = ld4u(P.ld + p4);
-    const float4 lp4 = ld4(P.lastP + p4);
-    const float4 ln4 = ld4(P.lastNs + p4);
-    const float4 hA = ld4(P.hitA + p4);
-    const float4 hB = ld4(P.hitB + p4);
-    const uint32_t hp = P.hitPrim[p];
-
-    vec3f rorg = v3(o4.x, o4.y, o4.z);
-    const vec3f rdir = v3(d4.x, d4.y, d4.z);
-    vec3f thr = v3(thr4.x, thr4.y, thr4.z);
-    float alpha = thr4.w;
-    vec3f contribution = v3(con4.x, con4.y, con4.z);
-    uint32_t flags = ld4v.w;
-    const uint32_t depth = flags & FLAG_DEPTH_MASK;
-    uint32_t scat = (flags >> FLAG_SCAT_SHIFT) & 0xffu;
-    uint32_t medium = flags >> FLAG_MEDIUM_SHIFT;
-    LDSampler lds;
-    lds.revIndex = ld4v.x;
-    lds.scramble = ld4v.y;
-    lds.baseIndexLight = ld4v.z;
-    lds.idxMask = F.idxMask;
-    lds.idxMaskLight = F.idxMaskLight;
-
-    Hit hit;
-    hit.hit = __float_as_uint(hA.w) != kInvalidID;
-    hit.t = hA.x;
-    hit.u = hA.y;
-    hit.v = hA.z;
-    hit.inst = __float_as_uint(hA.w);
-    hit.Ng = v3(hB.x, hB.y, hB.z);
-    hit.model = __float_as_uint(hB.w);
-    hit.prim = hp & 0x7fffffffu;
-    hit.half = hp >> 31;
-    const float rayT = hit.hit ? hit.t : kInf;
-
-    const bool straight = (flags & FLAG_STRAIGHT) != 0;
-    if (depth == 0)
-      con4.w = F.projectedDepth ? rayT * dot(rdir, ld3(F.camDirection)) : rayT;
-
-    // background (PathSampler.ispc:211-223)
-    if (!hit.hit && straight) {
-      alpha -= luminance(thr) * (1.0f - F.bgColor[3]);
-      const vec3f bg3 = thr * v3(F.bgColor[0], F.bgColor[1], F.bgColor[2]);
-      contribution = contribution + bg3;
-    }
-    // evaluateVirtualLights (VirtualLight.ispc:38-107)
-    if (S.numLights > 0) {
-      const vec3f lastP = v3(lp4.x, lp4.y, lp4.z);
-      const vec3f lastNs = v3(ln4.x, ln4.y, ln4.z);
-      const float lastPdf = lp4.w;
-      const uint32_t lastNLS = __float_as_uint(ln4.w);
-      vec3f L = v3(0.f);
-      const float lower = distance(lastP, rorg);
-      const float upper = !hit.hit ? kInf : distance(lastP, rorg + rayT * rdir);
-      const float selectionPDF = (float)lastNLS / (float)S.numLights;
-      for (int li = 0; li < S.numLights; li++) {
-        const DLight &l = S.lights[li];
-        if (!straight || l.visible) {
-          const LightEval le = lightEval(l, lastP, lastNs, rdir, lower, upper);
-          if (reduce_max(le.radiance) > 0.0f) {
-            float mis = 1.0f;
-            if (selectionPDF != 0.f)
-              mis = misHeuristic(lastPdf, selectionPDF * le.pdf);
-            L = L + thr * le.radiance * mis;
-          }
-        }
-      }
-      contribution = contribution + L;
-    }
-
+  for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
+    const int i = w0 + laneId();
     bool cont = false;
-    vec3f newOrg = rorg, newDir = rdir;
-    float newLastPdf = lp4.w;
-    vec3f newLastP = v3(lp4.x, lp4.y, lp4.z), newLastNs = v3(ln4.x, ln4.y, ln4.z);
-    uint32_t newLastNLS = __float_as_uint(ln4.w);
     uint32_t nNee = 0;
-    do {
-      if (!hit.hit)
-        break;
-      if (depth >= (uint32_t)F.maxDepth)
-        break;
-      DG dg;
-      postIntersect(S, rorg, rdir, hit, dg, true);
-      if (depth == 0) {
-        P.ids[p4 + 0] = dg.primID;
-        P.ids[p4 + 1] = dg.objID;
-        P.ids[p4 + 2] = dg.instID;
-        if (P.aux) {
-          float *a = P.aux + 16 * (size_t)p;
-          a[0] = dg.P.x;
-          a[1] = dg.P.y;
-          a[2] = dg.P.z;
-          a[3] = dg.Ns.x;
-          a[4] = dg.Ns.y;
-          a[5] = dg.Ns.z;
-        }
-      }
-      if (depth >= (uint32_t)F.maxDepth || scat >= (uint32_t)F.maxScatteringEvents)
-        break;
-      BSDF b;
-      if (!getBSDF(S, dg, S.media[medium], b))
-        break;
-      ldNextGroup(lds);
-      const vec3f wo = neg(rdir);
-      uint32_t pvNLS = 0;
-      if (b.scatteringType & S_SMOOTH) {
-        if (scat == 0 && P.aux) {
-          float *a = P.aux + 16 * (size_t)p;
-          const vec3f nn = thr * b.frame.vz;
-          const vec3f al = thr * b.albedo;
-          a[6] = nn.x;
-          a[7] = nn.y;
-          a[8] = nn.z;
-          a[9] = al.x;
-          a[10] = al.y;
-          a[11] = al.z;
-        }
-        const uint32_t nls = (flags & FLAG_FIRST_BOUNCE_LIGHT) ? F.numFirstLightSamples : F.numIndirectLightSamples;
-        pvNLS = nls;
-        nNee = nls;
-        for (uint32_t li = 0; li < nls; li++) {
-          const size_t r = (size_t)p * F.maxRec + li;
-          float ss;
-          const vec2f s = ldGet3LightSamples(lds, li, nls > 1, ss);
-          const Sample1D sel = distribution1DSample(S.numLights, S.lightsCDF, 0, ss);
-          const float selProb = sel.prob * (float)nls;
-          LightSample ls = lightSample(S.lights[sel.idx], dg.P, dg.Ng, dg.Ns, s);
-          ls.weight = ls.weight / selProb;
-          ls.pdf *= selProb;
-          st4(P.recResult + 4 * r, 0.f, 0.f, 0.f, 0.f);
-          bool emit = !(reduce_max(ls.weight) <= 0.0f || ls.pdf <= 0.0f);
-          EvalRes fe = {v3(0.f), 0.f};
-          if (emit) {
-            fe = bsdfEval(b, wo, ls.dir);
-            emit = !(reduce_max(fe.value) <= 0.0f);
-          }
-          if (emit) {
-            vec3f org = dg.P;
-            if (dot(dg.Ng, ls.dir) < 0.f)
-              org = org - (2.0f * dg.epsilon) * dg.Ng;
-            const vec3f throughput = thr * fe.value;
-            const vec3f unshaded = throughput * ls.weight;
-            st4(P.recOrg + 4 * r, org.x, org.y, org.z, ls.dist);
-            st4(P.recDir + 4 * r, ls.dir.x, ls.dir.y, ls.dir.z, misHeuristic(ls.pdf, fe.pdf));
-            st4(P.recContrib + 4 * r, unshaded.x, unshaded.y, unshaded.z, __uint_as_float(medium));
-          }
-          const int slot = enqueue(&P.counters[2], emit);
-          if (emit)
-            P.shadowQueue[slot] = (uint32_t)r;
-        }
-        flags &= ~FLAG_FIRST_BOUNCE_LIGHT;
-      }
-      // sampleDirection (PathSampler.ispc:72-107)
-      float ss;
-      uint32_t uu3;
-      const vec2f s = ldGetNext4Samples(lds, ss, uu3);
-      SampleRes fs = bsdfSample(b, wo, s, ss);
-      {
-        const bool sameSide = dot(fs.wi, dg.Ng) >= 0.f;
-        const bool reflection = (fs.type & S_REFLECTION) != 0;
-        if (sameSide != reflection)
-          fs.weight = v3(0.0f);
-      }
-      if (reduce_max(fs.weight) <= 0.0f || fs.pdf <= 0.0f)
-        break;
-      thr = thr * fs.weight;
-      if (depth >= (uint32_t)F.rouletteDepth) {
-        const float rr = ldFinalizeDim3(lds, uu3);
-        const float contProb = minf(reduce_max(thr), kMaxRouletteContProb);
-        if (rr > contProb)
-          break;
-        thr = thr * rcp(contProb);
-      }
-      const DMedium &cm = S.media[medium];
-      if (minf(minf(cm.attenuation[0], cm.attenuation[1]), cm.attenuation[2]) < 0.f) {
-        const vec3f a = ld3(cm.attenuation) * rayT;
-        thr = thr * v3(fexp(a.x), fexp(a.y), fexp(a.z));
-      }
-      vec3f rayOrg = dg.P;
-      if (fs.type & S_TRANSMISSION) {
-        rayOrg = rayOrg - (2.0f * dg.epsilon) * dg.Ng;
-        medium = selectNextMedium(S, dg, medium);
-      }
-      if (fs.type & ~SPEC_TRANS) {
-        newLastP = dg.P;
-        newLastNs = dg.Ns;
-        newLastPdf = fs.pdf;
-        newLastNLS = pvNLS;
-      }
-      if (!eq(rdir, fs.wi))
-        flags &= ~FLAG_STRAIGHT;
-      if (!(fs.type & SPEC_TRANS))
-        flags &= ~FLAG_SPEC_TRANS_PATH;
-      newOrg = rayOrg;
-      newDir = fs.wi;
-      const uint32_t nd = depth + 1;
-      if (fs.type & S_SMOOTH)
-        scat++;
-      flags = (flags & ~(FLAG_DEPTH_MASK | (0xffu << FLAG_SCAT_SHIFT) | (0xfffu << FLAG_MEDIUM_SHIFT))) | nd
-          | (scat << FLAG_SCAT_SHIFT) | (medium << FLAG_MEDIUM_SHIFT);
-      cont = reduce_max(thr) > F.minContribution;
-    } while (false);
+    uint32_t p = 0;
+    if (i < n) {
+      p = P.queue[qin][i];
+      const size_t p4 = 4 * (size_t)p;
+      const float4 o4 = ld4(P.org + p4);
+      const float4 d4 = ld4(P.dir + p4);
+      float4 thr4 = ld4(P.thr + p4);
+      float4 con4 = ld4(P.contrib + p4);
+      uint4 ld4v = ld4u(P.ld + p4);
+      const float4 lp4 = ld4(P.lastP + p4);
+      const float4 ln4 = ld4(P.lastNs + p4);
+      const float4 hA = ld4(P.hitA + p4);
+      const float4 hB = ld4(P.hitB + p4);
+      const uint32_t hp = P.hitPrim[p];
 
-    st4(P.thr + p4, thr.x, thr.y, thr.z, alpha);
-    st4(P.contrib + p4, contribution.x, contribution.y, contribution.z, con4.w);
-    if (cont) {
-      st4(P.org + p4, newOrg.x, newOrg.y, newOrg.z, 0.f);
-      st4(P.dir + p4, newDir.x, newDir.y, newDir.z, 0.f);
-      st4u(P.ld + p4, lds.revIndex, lds.scramble, lds.baseIndexLight, flags);
-      st4(P.lastP + p4, newLastP.x, newLastP.y, newLastP.z, newLastPdf);
-      st4(P.lastNs + p4, newLastNs.x, newLastNs.y, newLastNs.z, __uint_as_float(newLastNLS));
+      vec3f rorg = v3(o4.x, o4.y, o4.z);
+      const vec3f rdir = v3(d4.x, d4.y, d4.z);
+      vec3f thr = v3(thr4.x, thr4.y, thr4.z);
+      float alpha = thr4.w;
+      vec3f contribution = v3(con4.x, con4.y, con4.z);
+      uint32_t flags = ld4v.w;
+      const uint32_t depth = flags & FLAG_DEPTH_MASK;
+      uint32_t scat = (flags >> FLAG_SCAT_SHIFT) & 0xffu;
+      uint32_t medium = flags >> FLAG_MEDIUM_SHIFT;
+      LDSampler lds;
+      lds.revIndex = ld4v.x;
+      lds.scramble = ld4v.y;
+      lds.baseIndexLight = ld4v.z;
+      lds.idxMask = F.idxMask;
+      lds.idxMaskLight = F.idxMaskLight;
+
+      Hit hit;
+      hit.hit = __float_as_uint(hA.w) != kInvalidID;
+      hit.t = hA.x;
+      hit.u = hA.y;
+      hit.v = hA.z;
+      hit.inst = __float_as_uint(hA.w);
+      hit.Ng = v3(hB.x, hB.y, hB.z);
+      hit.model = __float_as_uint(hB.w);
+      hit.prim = hp & 0x7fffffffu;
+      hit.half = hp >> 31;
+      const float rayT = hit.hit ? hit.t : kInf;
+
+      const bool straight = (flags & FLAG_STRAIGHT) != 0;
+      if (depth == 0)
+        con4.w = F.projectedDepth ? rayT * dot(rdir, ld3(F.camDirection)) : rayT;
+
+      // background (PathSampler.ispc:211-223)
+      if (!hit.hit && straight) {
+        alpha -= luminance(thr) * (1.0f - F.bgColor[3]);
+        const vec3f bg3 = thr * v3(F.bgColor[0], F.bgColor[1], F.bgColor[2]);
+        contribution = contribution + bg3;
+      }
+      // evaluateVirtualLights (VirtualLight.ispc:38-107)
+      if (S.numLights > 0) {
+        const vec3f lastP = v3(lp4.x, lp4.y, lp4.z);
+        const vec3f lastNs = v3(ln4.x, ln4.y, ln4.z);
+        const float lastPdf = lp4.w;
+        const uint32_t lastNLS = __float_as_uint(ln4.w);
+        vec3f L = v3(0.f);
+        const float lower = distance(lastP, rorg);
+        const float upper = !hit.hit ? kInf : distance(lastP, rorg + rayT * rdir);
+        const float selectionPDF = (float)lastNLS / (float)S.numLights;
+        for (int li = 0; li < S.numLights; li++) {
+          const DLight &l = S.lights[li];
+          if (!straight || l.visible) {
+            const LightEval le = lightEval(l, lastP, lastNs, rdir, lower, upper);
+            if (reduce_max(le.radiance) > 0.0f) {
+              float mis = 1.0f;
+              if (selectionPDF != 0.f)
+                mis = misHeuristic(lastPdf, selectionPDF * le.pdf);
+              L = L + thr * le.radiance * mis;
+            }
+          }
+        }
+        contribution = contribution + L;
+      }
+
+      vec3f newOrg = rorg, newDir = rdir;
+      float newLastPdf = lp4.w;
+      vec3f newLastP = v3(lp4.x, lp4.y, lp4.z), newLastNs = v3(ln4.x, ln4.y, ln4.z);
+      uint32_t newLastNLS = __float_as_uint(ln4.w);
+      do {
+        if (!hit.hit)
+          break;
+        if (depth >= (uint32_t)F.maxDepth)
+          break;
+        DG dg;
+        postIntersect(S, rorg, rdir, hit, dg, true);
+        if (depth == 0) {
+          P.ids[p4 + 0] = dg.primID;
+          P.ids[p4 + 1] = dg.objID;
+          P.ids[p4 + 2] = dg.instID;
+          if (P.aux) {
+            float *a = P.aux + 16 * (size_t)p;
+            a[0] = dg.P.x;
+            a[1] = dg.P.y;
+            a[2] = dg.P.z;
+            a[3] = dg.Ns.x;
+            a[4] = dg.Ns.y;
+            a[5] = dg.Ns.z;
+          }
+        }
+        if (depth >= (uint32_t)F.maxDepth || scat >= (uint32_t)F.maxScatteringEvents)
+          break;
+        BSDF b;
+        if (!getBSDF(S, dg, S.media[medium], b))
+          break;
+        ldNextGroup(lds);
+        const vec3f wo = neg(rdir);
+        uint32_t pvNLS = 0;
+        if (b.scatteringType & S_SMOOTH) {
+          if (scat == 0 && P.aux) {
+            float *a = P.aux + 16 * (size_t)p;
+            const vec3f nn = thr * b.frame.vz;
+            const vec3f al = thr * b.albedo;
+            a[6] = nn.x;
+            a[7] = nn.y;
+            a[8] = nn.z;
+            a[9] = al.x;
+            a[10] = al.y;
+            a[11] = al.z;
+          }
+          const uint32_t nls = (flags & FLAG_FIRST_BOUNCE_LIGHT) ? F.numFirstLightSamples : F.numIndirectLightSamples;
+          pvNLS = nls;
+          nNee = nls;
+          if (nls) {
+            // NEE vertex record: everything k_nee needs of this vertex
+            st4(P.vtxP + p4, dg.P.x, dg.P.y, dg.P.z, dg.epsilon);
+            st4(P.vtxNg + p4, dg.Ng.x, dg.Ng.y, dg.Ng.z, __int_as_float(dg.material));
+            st4(P.vtxNs + p4, dg.Ns.x, dg.Ns.y, dg.Ns.z, __uint_as_float(medium));
+            st4(P.vtxThr + p4, thr.x, thr.y, thr.z, __uint_as_float(nls));
+            st4(P.vtxWo + p4, wo.x, wo.y, wo.z, 0.f);
+            st4u(P.vtxLd + p4, lds.revIndex, lds.scramble, lds.baseIndexLight, 0u);
+          }
+          flags &= ~FLAG_FIRST_BOUNCE_LIGHT;
+        }
+        // sampleDirection (PathSampler.ispc:72-107)
+        float ss;
+        uint32_t uu3;
+        const vec2f s = ldGetNext4Samples(lds, ss, uu3);
+        SampleRes fs = bsdfSample(b, wo, s, ss);
+        {
+          const bool sameSide = dot(fs.wi, dg.Ng) >= 0.f;
+          const bool reflection = (fs.type & S_REFLECTION) != 0;
+          if (sameSide != reflection)
+            fs.weight = v3(0.0f);
+        }
+        if (reduce_max(fs.weight) <= 0.0f || fs.pdf <= 0.0f)
+          break;
+        thr = thr * fs.weight;
+        if (depth >= (uint32_t)F.rouletteDepth) {
+          const float rr = ldFinalizeDim3(lds, uu3);
+          const float contProb = minf(reduce_max(thr), kMaxRouletteContProb);
+          if (rr > contProb)
+            break;
+          thr = thr * rcp(contProb);
+        }
+        const DMedium &cm = S.media[medium];
+        if (minf(minf(cm.attenuation[0], cm.attenuation[1]), cm.attenuation[2]) < 0.f) {
+          const vec3f a = ld3(cm.attenuation) * rayT;
+          thr = thr * v3(fexp(a.x), fexp(a.y), fexp(a.z));
+        }
+        vec3f rayOrg = dg.P;
+        if (fs.type & S_TRANSMISSION) {
+          rayOrg = rayOrg - (2.0f * dg.epsilon) * dg.Ng;
+          medium = selectNextMedium(S, dg, medium);
+        }
+        if (fs.type & ~SPEC_TRANS) {
+          newLastP = dg.P;
+          newLastNs = dg.Ns;
+          newLastPdf = fs.pdf;
+          newLastNLS = pvNLS;
+        }
+        if (!eq(rdir, fs.wi))
+          flags &= ~FLAG_STRAIGHT;
+        if (!(fs.type & SPEC_TRANS))
+          flags &= ~FLAG_SPEC_TRANS_PATH;
+        newOrg = rayOrg;
+        newDir = fs.wi;
+        const uint32_t nd = depth + 1;
+        if (fs.type & S_SMOOTH)
+          scat++;
+        flags = (flags & ~(FLAG_DEPTH_MASK | (0xffu << FLAG_SCAT_SHIFT) | (0xfffu << FLAG_MEDIUM_SHIFT))) | nd
+            | (scat << FLAG_SCAT_SHIFT) | (medium << FLAG_MEDIUM_SHIFT);
+        cont = reduce_max(thr) > F.minContribution;
+      } while (false);
+
+      st4(P.thr + p4, thr.x, thr.y, thr.z, alpha);
+      st4(P.contrib + p4, contribution.x, contribution.y, contribution.z, con4.w);
+      if (cont) {
+        st4(P.org + p4, newOrg.x, newOrg.y, newOrg.z, 0.f);
+        st4(P.dir + p4, newDir.x, newDir.y, newDir.z, 0.f);
+        st4u(P.ld + p4, lds.revIndex, lds.scramble, lds.baseIndexLight, flags);
+        st4(P.lastP + p4, newLastP.x, newLastP.y, newLastP.z, newLastPdf);
+        st4(P.lastNs + p4, newLastNs.x, newLastNs.y, newLastNs.z, __uint_as_float(newLastNLS));
+      }
+      if (nNee)
+        P.neeN[p] = nNee;
     }
-    if (nNee)
-      P.neeN[p] = nNee;
     const int so = enqueue(&P.counters[qout], cont);
     if (cont)
       P.queue[qout][so] = p;
     const int sn = enqueue(&P.counters[3], nNee > 0);
     if (nNee > 0)
       P.neeQueue[sn] = p;
+    // light-sample slots of this vertex: records p*maxRec + [0, nNee)
+    const int first = enqueueN(&P.counters[7], nNee);
+    for (uint32_t k = 0; k < nNee; k++)
+      P.neeSampleQueue[first + k] = p * (uint32_t)F.maxRec + k;
+  }
+}
+
+// One NEE light sample per thread (NextEventEstimation.ispc:54-146): pick a
+// light, sample it, evaluate the BSDF and write the shadow-ray record;
+// records that can contribute go to the shadow queue.
+__global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P)
+{
+  const int n = P.counters[7];
+  for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
+    const int i = w0 + laneId();
+    bool emit = false;
+    uint32_t r = 0;
+    if (i < n) {
+      r = P.neeSampleQueue[i];
+      const uint32_t p = r / (uint32_t)F.maxRec;
+      const uint32_t li = r - p * (uint32_t)F.maxRec;
+      const size_t p4 = 4 * (size_t)p;
+      const float4 vP = ld4(P.vtxP + p4);
+      const float4 vNg = ld4(P.vtxNg + p4);
+      const float4 vNs = ld4(P.vtxNs + p4);
+      const float4 vThr = ld4(P.vtxThr + p4);
+      const uint4 vLd = ld4u(P.vtxLd + p4);
+      const vec3f Pv = v3(vP.x, vP.y, vP.z), Ng = v3(vNg.x, vNg.y, vNg.z), Ns = v3(vNs.x, vNs.y, vNs.z);
+      const uint32_t nls = __float_as_uint(vThr.w);
+      LDSampler lds;
+      lds.revIndex = vLd.x;
+      lds.scramble = vLd.y;
+      lds.baseIndexLight = vLd.z;
+      lds.idxMask = F.idxMask;
+      lds.idxMaskLight = F.idxMaskLight;
+      float ss;
+      const vec2f s = ldGet3LightSamples(lds, li, nls > 1, ss);
+      const Sample1D sel = distribution1DSample(S.numLights, S.lightsCDF, 0, ss);
+      const float selProb = sel.prob * (float)nls;
+      LightSample ls = lightSample(S.lights[sel.idx], Pv, Ng, Ns, s);
+      ls.weight = ls.weight / selProb;
+      ls.pdf *= selProb;
+      st4(P.recResult + 4 * (size_t)r, 0.f, 0.f, 0.f, 0.f);
+      emit = !(reduce_max(ls.weight) <= 0.0f || ls.pdf <= 0.0f);
+      EvalRes fe = {v3(0.f), 0.f};
+      if (emit) {
+        const float4 vWo = ld4(P.vtxWo + p4);
+        BSDF b;
+        getEvalBSDF(S, __float_as_int(vNg.w), Ns, b);
+        fe = bsdfEval(b, v3(vWo.x, vWo.y, vWo.z), ls.dir);
+        emit = !(reduce_max(fe.value) <= 0.0f);
+      }
+      if (emit) {
+        vec3f org = Pv;
+        if (dot(Ng, ls.dir) < 0.f)
+          org = org - (2.0f * vP.w) * Ng;
+        const vec3f throughput = v3(vThr.x, vThr.y, vThr.z) * fe.value;
+        const vec3f unshaded = throughput * ls.weight;
+        st4(P.recOrg + 4 * (size_t)r, org.x, org.y, org.z, ls.dist);
+        st4(P.recDir + 4 * (size_t)r, ls.dir.x, ls.dir.y, ls.dir.z, misHeuristic(ls.pdf, fe.pdf));
+        st4(P.recContrib + 4 * (size_t)r, unshaded.x, unshaded.y, unshaded.z, vNs.w);
+      }
+    }
+    const int slot = enqueue(&P.counters[2], emit);
+    if (emit)
+      P.shadowQueue[slot] = r;
   }
 }
 
@@ -890,6 +964,13 @@ hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, in
 {
   hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
       *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P), queue);
+  return hipGetLastError();
+}
+
+hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_nee, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
+      *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P));
   return hipGetLastError();
 }
 

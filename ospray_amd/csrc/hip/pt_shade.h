@@ -256,6 +256,56 @@ HPT_HD vec3f ggxSampleVisible(vec2f a, vec3f wo, float &pdf, vec2f s)
 }
 
 // Material_dispatch_getBSDF (render/MaterialDispatch.ispc:23-72)
+// the lobes of an OBJ material: Lambert (kd) and specular transmission
+// (1 - d, tf) (OBJ.ispc:55-98, MultiBSDF.ih:11-40)
+HPT_HD void objLobes(const DMaterial &m, float colorW, BSDF &b)
+{
+  b.kind = 1;
+  b.albedo = v3(0.f);
+  b.scatteringType = 0;
+  b.importanceSum = 0.f;
+  b.weight[0] = b.weight[1] = 0.f;
+  b.importance[0] = b.importance[1] = 0.f;
+  const float d = m.d * colorW;
+  const vec3f Kd = ld3(m.Kd) * d * v3(1.f, 1.f, 1.f);
+  if (reduce_max(Kd) > 0.0f) {
+    b.childAlbedo[0] = Kd;
+    b.albedo = b.albedo + Kd * 1.f;
+    b.scatteringType |= DIFF_REFL;
+    b.weight[0] = 1.f;
+    b.importance[0] = luminance(Kd);
+    b.importanceSum += b.importance[0];
+  }
+  const vec3f T = ld3(m.Tf) * d + v3(1.f - d);
+  if (reduce_max(T) > 0.0f) {
+    b.childAlbedo[1] = T;
+    b.albedo = b.albedo + T * 1.f;
+    b.scatteringType |= SPEC_TRANS;
+    b.weight[1] = 1.f;
+    b.importance[1] = luminance(T);
+    b.importanceSum += b.importance[1];
+  }
+}
+
+// the conductor lobe of a metal material (Metal.ispc:18-46)
+HPT_HD void metalLobe(const DMaterial &m, BSDF &b)
+{
+  b.eta = ld3(m.eta);
+  b.k = ld3(m.k);
+  // RGB conductor Fresnel has no evalAvg (Fresnel.ih:307-319): Favg = 0,
+  // so the albedo and the Kulla-Conty compensation term are zero
+  b.albedo = v3(0.f);
+  if (m.roughness == 0.0f) {
+    b.kind = 2;
+    b.scatteringType = SPEC_REFL;
+  } else {
+    b.kind = 3;
+    b.scatteringType = GLOSSY_REFL;
+    const float aspect = sqrtf(1.f - 0.9f * 0.f);
+    b.alpha = v2(maxf(sqr(m.roughness) / aspect, 0.001f), maxf(sqr(m.roughness) * aspect, 0.001f));
+  }
+}
+
 HPT_HD bool getBSDF(const DScene &S, const DG &dg, const DMedium &medium, BSDF &b)
 {
   if (dg.material < 0)
@@ -280,43 +330,12 @@ HPT_HD bool getBSDF(const DScene &S, const DG &dg, const DMedium &medium, BSDF &
         b.frame = frame(N);
       }
     }
-    const float d = m.d * dg.colorW;
-    const vec3f Kd = ld3(m.Kd) * d * v3(1.f, 1.f, 1.f);
-    if (reduce_max(Kd) > 0.0f) {
-      b.childAlbedo[0] = Kd;
-      b.albedo = b.albedo + Kd * 1.f;
-      b.scatteringType |= DIFF_REFL;
-      b.weight[0] = 1.f;
-      b.importance[0] = luminance(Kd);
-      b.importanceSum += b.importance[0];
-    }
-    const vec3f T = ld3(m.Tf) * d + v3(1.f - d);
-    if (reduce_max(T) > 0.0f) {
-      b.childAlbedo[1] = T;
-      b.albedo = b.albedo + T * 1.f;
-      b.scatteringType |= SPEC_TRANS;
-      b.weight[1] = 1.f;
-      b.importance[1] = luminance(T);
-      b.importanceSum += b.importance[1];
-    }
+    objLobes(m, dg.colorW, b);
     return true;
   }
   if (m.type == MAT_METAL) { // Metal.ispc:18-46
     b.frame = frame(dg.Ns);
-    b.eta = ld3(m.eta);
-    b.k = ld3(m.k);
-    // RGB conductor Fresnel has no evalAvg (Fresnel.ih:307-319): Favg = 0,
-    // so the albedo and the Kulla-Conty compensation term are zero
-    b.albedo = v3(0.f);
-    if (m.roughness == 0.0f) {
-      b.kind = 2;
-      b.scatteringType = SPEC_REFL;
-    } else {
-      b.kind = 3;
-      b.scatteringType = GLOSSY_REFL;
-      const float aspect = sqrtf(1.f - 0.9f * 0.f);
-      b.alpha = v2(maxf(sqr(m.roughness) / aspect, 0.001f), maxf(sqr(m.roughness) * aspect, 0.001f));
-    }
+    metalLobe(m, b);
     return true;
   }
   // Glass.ispc:18-35 -> RobustDielectric (bsdfs/RobustDielectric.ih:162-185)
@@ -374,6 +393,21 @@ HPT_HD EvalRes bsdfEval(const BSDF &b, vec3f wo, vec3f wi)
     return r;
   }
   return z;
+}
+
+// A smooth BSDF (OBJ Lambert, MicrofacetConductor) as far as bsdfEval reads
+// it, rebuilt from the material and the shading normal: bsdfEval uses only
+// frame.vz of the OBJ shading frame, which is Ns whichever tangents built it
+// (Material.ih:25-37), and metal's frame(Ns).  colorW is 1: postIntersect
+// has no vertex colours.
+HPT_HD void getEvalBSDF(const DScene &S, int material, vec3f Ns, BSDF &b)
+{
+  const DMaterial &m = S.materials[material];
+  b.frame = frame(Ns);
+  if (m.type == MAT_OBJ)
+    objLobes(m, 1.f, b);
+  else
+    metalLobe(m, b);
 }
 
 // BSDF_dispatch_sample (bsdfs/BSDF.ispc:61-112)

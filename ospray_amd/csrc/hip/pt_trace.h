@@ -2,8 +2,9 @@
 //
 // The reference hands these to Embree (rtcIntersectV,
 // modules/cpu/common/World.ih:29-52, 247-286).  Here:
-//   * two-level BVH2 (TLAS over instances, one BLAS per group) in 64-byte
-//     nodes holding both child boxes; one lane per ray, nearer child first;
+//   * two-level 4-wide BVH (TLAS over instances, one BLAS per group) in
+//     128-byte nodes holding four child boxes; one lane per ray, children
+//     visited nearest first;
 //   * the traversal stack lives in LDS (LDS_DEPTH entries per lane, laid out
 //     [depth][lane] so lanes hit distinct banks) and spills to a per-thread
 //     global overflow row beyond that;
@@ -23,7 +24,7 @@ namespace hpt {
 
 constexpr int kTraceBlock = 128;
 constexpr int kLdsDepth = 24;
-constexpr int kOverflowDepth = 72;
+constexpr int kOverflowDepth = 104;
 constexpr int32_t kMarker = 0x7fffffff;
 
 struct Hit
@@ -152,13 +153,37 @@ struct TraceStats
   uint32_t nodes, prims;
 };
 
-__device__ __forceinline__ void loadNode(const DNode *nodes, int idx, float4 &a, float4 &b, float4 &c, int4 &d)
+// one 4-wide node: child boxes as per-axis float4s (lane c = child c)
+struct Node4
+{
+  float4 lox, hix, loy, hiy, loz, hiz;
+  int4 child;
+};
+
+__device__ __forceinline__ Node4 loadNode(const DNode4 *nodes, int idx)
 {
   const float4 *np = reinterpret_cast<const float4 *>(nodes + idx);
-  a = np[0];
-  b = np[1];
-  c = np[2];
-  d = reinterpret_cast<const int4 *>(np)[3];
+  Node4 n;
+  n.lox = np[0];
+  n.hix = np[1];
+  n.loy = np[2];
+  n.hiy = np[3];
+  n.loz = np[4];
+  n.hiz = np[5];
+  n.child = reinterpret_cast<const int4 *>(np)[6];
+  return n;
+}
+
+// order (t, child) pairs by t
+__device__ __forceinline__ void cswap(float &ta, int &ca, float &tb, int &cb)
+{
+  const bool sw = tb < ta;
+  const float t = sw ? tb : ta;
+  const int c = sw ? cb : ca;
+  tb = sw ? ta : tb;
+  cb = sw ? ca : cb;
+  ta = t;
+  ca = c;
 }
 
 __device__ __forceinline__ float slabNear(float lox, float loy, float loz, float hix, float hiy, float hiz,
@@ -216,34 +241,37 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
   };
   for (;;) {
     if (cur >= 0 && cur != kMarker) {
-      float4 a, b, c;
-      int4 ch;
-      loadNode(S.nodes, cur, a, b, c, ch);
+      const Node4 nd = loadNode(S.nodes, cur);
       if (STATS)
         st.nodes++;
       const float tb = best.t;
-      const float tn0 = slabNear(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, t0, tb);
-      const float tn1 = slabNear(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, t0, tb);
-      const bool h0 = tn0 != kInf, h1 = tn1 != kInf;
-      if (h0 && h1) {
-        int nearC = ch.x, farC = ch.y;
-        if (tn1 < tn0) {
-          nearC = ch.y;
-          farC = ch.x;
-        }
-        if (sp >= kLdsDepth + kOverflowDepth) // stack full: cannot happen for builder-bounded trees
-          return best.hit;
-        push(farC);
-        cur = nearC;
-      } else if (h0) {
-        cur = ch.x;
-      } else if (h1) {
-        cur = ch.y;
-      } else {
+      float t0c = slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tb);
+      float t1c = slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tb);
+      float t2c = slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tb);
+      float t3c = slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, t0, tb);
+      int c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
+      // nearest first: sorting network on the four entry distances
+      cswap(t0c, c0, t1c, c1);
+      cswap(t2c, c2, t3c, c3);
+      cswap(t0c, c0, t2c, c2);
+      cswap(t1c, c1, t3c, c3);
+      cswap(t1c, c1, t2c, c2);
+      if (t0c == kInf) {
         if (sp == 0)
           break;
         cur = pop();
+        continue;
       }
+      const int npush = (t1c != kInf) + (t2c != kInf) + (t3c != kInf);
+      if (sp + npush > kLdsDepth + kOverflowDepth) // cannot happen: the world commit checks the stack need
+        return best.hit;
+      if (t3c != kInf)
+        push(c3);
+      if (t2c != kInf)
+        push(c2);
+      if (t1c != kInf)
+        push(c1);
+      cur = c0;
       continue;
     }
     if (cur == kMarker) {

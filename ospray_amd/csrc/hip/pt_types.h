@@ -2,10 +2,10 @@
 // host set-up code and kernels).
 //
 // HBM layout of a committed world:
-//   DNode[]     BVH2 nodes, 64 B each, both child boxes in one node so a
-//               traversal step is one 64-byte fetch; TLAS and every BLAS share
-//               the array.  child >= 0: inner node; child < 0: leaf,
-//               ~child = first << 3 | (count - 1).
+//   DNode4[]    4-wide BVH nodes, 128 B each (one cache line): the boxes of
+//               up to four children, so a traversal step is one 128-byte
+//               fetch; TLAS and every BLAS share the array.  child >= 0:
+//               inner node; child < 0: leaf, ~child = first << 3 | (count - 1).
 //   DPrim[]     48 B per primitive (triangle, quad half or sphere); vertices
 //               are stored in object space next to the ids so a leaf is a
 //               contiguous run.
@@ -28,14 +28,17 @@ enum PrimKind : uint32_t
   PRIM_SPHERE = 3
 };
 
-struct DNode
+// 4-wide BVH node, one 128-byte line: the bounds of up to four children as
+// per-axis float4s (child c in lane c), and the child references: >= 0 an
+// inner node, < 0 a leaf ~(first << 3 | count - 1).  An unused slot has a
+// box at +inf on every axis, which no ray enters.
+struct DNode4
 {
-  float b0lo[3], b0hi[3];
-  float b1lo[3], b1hi[3];
-  int32_t child[2];
-  int32_t pad[2];
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  int32_t child[4];
+  int32_t pad[4];
 };
-static_assert(sizeof(DNode) == 64, "node size");
+static_assert(sizeof(DNode4) == 128, "node size");
 
 struct DPrim
 {
@@ -125,7 +128,7 @@ struct DLight
 
 struct DScene
 {
-  const DNode *nodes;
+  const DNode4 *nodes;
   const DPrim *prims;
   const DInstance *instances;
   const DModel *models;
@@ -155,6 +158,13 @@ struct DPaths
   uint32_t *ids; // 4: primID, geomID, instID, medium
   float *aux; // 16: position.xyz, firstNormal.xyz, normal.xyz, albedo.xyz, pad
   uint32_t *neeN; // light samples of the pending NEE vertex
+  // NEE vertex record, written by k_shade at a smooth vertex
+  float *vtxP; // 4: P.xyz, epsilon
+  float *vtxNg; // 4: Ng.xyz, material (bits)
+  float *vtxNs; // 4: Ns.xyz, medium (bits)
+  float *vtxThr; // 4: throughput.xyz, light samples (bits)
+  float *vtxWo; // 4: wo.xyz, (unused)
+  uint32_t *vtxLd; // 4: sampler revIndex, scramble, baseIndexLight, (unused)
   // NEE shadow-ray records, slot = path * maxRec + i
   float *recOrg; // 4: org.xyz, tmax
   float *recDir; // 4: dir.xyz, mis weight
@@ -165,8 +175,9 @@ struct DPaths
   uint32_t *shadowQueue;
   uint32_t *neeQueue;
   uint32_t *transQueue; // shadow records that hit only non-opaque surfaces
+  uint32_t *neeSampleQueue; // record slots (path * maxRec + sample) to evaluate
   // [0..1] extension queues, [2] shadow records, [3] NEE vertices, [4] and
-  // [5] extension/shadow work fetch, [6] transparency queue, [7] spare
+  // [5] extension/shadow work fetch, [6] transparency queue, [7] NEE samples
   int32_t *counters;
   uint32_t *overflow; // traversal stack overflow, per resident thread
   unsigned long long *stats; // node visits, primitive tests, traversals: [0..2] extension, [3..5] shadow
