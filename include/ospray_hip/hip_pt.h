@@ -26,10 +26,8 @@ typedef struct HptPaths HptPaths; /* wavefront path state, queues and counters *
 
 #define HPT_API __attribute__((visibility("default")))
 
-/* Queue counters: [0],[1] extension queues, [2] shadow records, [3] NEE
- * vertices, [4],[5] extension/shadow work fetch, [6] transparency queue,
- * [7] NEE light samples;
- * all are set to 0 except counter `keepIndex` (-1: none). */
+/* Sets every queue counter to 0 except the entry count of extension queue
+ * `keepIndex` (0 or 1; -1: none). */
 HPT_API hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s);
 HPT_API hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s);
 /* dst <- src, bytes a multiple of 16: a plain streaming copy that calibrates
@@ -43,9 +41,10 @@ HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, i
 /* One samplePath bounce (PathSampler.ispc:143-407) for queue `queue`; the
  * NEE light samples of its smooth vertices are queued for hpt_nee. */
 HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
-/* One NEE light sample per thread: light selection and sampling, BSDF
- * evaluation, shadow-ray record (NextEventEstimation.ispc:54-146). */
-HPT_API hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s);
+/* One NEE light sample per thread for the vertices shaded from extension
+ * queue `queue`: light selection and sampling, BSDF evaluation, shadow-ray
+ * record (NextEventEstimation.ispc:54-146). */
+HPT_API hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
 /* transparentShadow of every NEE record (TransparentShadow.ispc:18-85):
  * an occlusion pass, then the closest-hit transparency loop for the records
  * that hit only non-opaque surfaces;
@@ -53,8 +52,9 @@ HPT_API hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths 
  * P->stats[3..5]. */
 HPT_API hipError_t hpt_shadow(
     const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s);
-/* Ordered sum of the NEE samples per vertex (NextEventEstimation.ispc:150-159). */
-HPT_API hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s);
+/* Ordered sum of the NEE samples of every vertex shaded from extension
+ * queue `queue` (NextEventEstimation.ispc:150-159). */
+HPT_API hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
 /* ScreenSample_accumulate over the pass (PathTracer.ispc:170-197). */
 HPT_API hipError_t hpt_finish_pass(const HptFrame *F, const HptPaths *P, hipStream_t s);
 /* LocalFB_accumulateSample (fb/LocalFB.ispc:14-78). */

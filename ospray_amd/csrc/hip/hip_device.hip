@@ -1380,7 +1380,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   const size_t perSlot = 9 * 16 + 4 + 16 + 4 + 6 * 16 + (auxAllocated ? 64 : 0);
   pathBuf.ensure(S * perSlot);
   recBuf.ensure(R * 64);
-  queueBuf.ensure((3 * S + 3 * R) * 4);
+  queueBuf.ensure((2 * S + 3 * R) * 4);
   char *b = (char *)pathBuf.ptr;
   auto take = [&](size_t bytes) {
     char *p = b;
@@ -1416,9 +1416,8 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.queue[0] = q;
   paths.queue[1] = q + S;
   paths.shadowQueue = q + 2 * S;
-  paths.neeQueue = q + 2 * S + R;
-  paths.transQueue = q + 3 * S + R;
-  paths.neeSampleQueue = q + 3 * S + 2 * R;
+  paths.transQueue = q + 2 * S + R;
+  paths.neeSampleQueue = q + 2 * S + 2 * R;
   paths.counters = (int32_t *)counters.ptr;
   paths.overflow = (uint32_t *)overflow.ptr;
   paths.stats = (unsigned long long *)stats.ptr;
@@ -1520,7 +1519,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
 
   if (!finishedAccum) {
     const bool aux = F.accAux != nullptr;
-    const size_t perSlotBytes = 9 * 16 + 40 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 76 + 12;
+    const size_t perSlotBytes = 9 * 16 + 40 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 76 + 8;
     const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
     const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
     ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
@@ -1543,9 +1542,9 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
         launch("shade", [&]() { return hpt_shade(S, Fp, P, q, shadeBlocks, stream); });
         if (F.numFirstLightSamples > 0) {
-          launch("nee", [&]() { return hpt_nee(S, Fp, P, shadeBlocks, stream); });
+          launch("nee", [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
           launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
-          launch("resolve", [&]() { return hpt_resolve(Fp, P, shadeBlocks, stream); });
+          launch("resolve", [&]() { return hpt_resolve(Fp, P, q, shadeBlocks, stream); });
         }
         q ^= 1;
       }

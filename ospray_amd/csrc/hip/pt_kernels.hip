@@ -53,26 +53,29 @@ __device__ __forceinline__ void waveAdd(unsigned long long *counter, uint32_t v)
     atomicAdd(counter, s);
 }
 
-// wave-aggregated append of `count` entries per lane (inclusive scan over
-// the wave, one atomic); every lane of the wave calls it.  Returns the
-// lane's first slot.
-__device__ __forceinline__ int enqueueN(int32_t *counter, uint32_t count)
+// wave-aggregated append of up to one entry per lane to an extension
+// queue and of `nee` entries per lane to the NEE sample queue, with one
+// 64-bit atomic per wave on the counter pair (ext, nee); every lane of the
+// wave calls it
+__device__ __forceinline__ void enqueuePair(int32_t *pair, bool ext, uint32_t nee, int &extSlot, int &neeSlot)
 {
   const int lane = laneId();
-  uint32_t x = count;
+  const unsigned long long mask = __ballot(ext);
+  uint32_t x = nee; // inclusive scan of nee over the wave
   for (int o = 1; o < 64; o <<= 1) {
     const uint32_t y = __shfl_up(x, o);
     if (lane >= o)
       x += y;
   }
-  const uint32_t total = __shfl(x, 63);
-  if (total == 0)
-    return 0;
-  int base = 0;
-  if (lane == 63)
-    base = atomicAdd(counter, (int)total);
-  base = __shfl(base, 63);
-  return base + (int)(x - count);
+  const uint32_t totalNee = __shfl(x, 63);
+  const uint32_t totalExt = (uint32_t)__popcll(mask);
+  unsigned long long base = 0;
+  if (lane == 0 && (totalExt | totalNee))
+    base = atomicAdd(reinterpret_cast<unsigned long long *>(pair),
+        (unsigned long long)totalExt | ((unsigned long long)totalNee << 32));
+  base = __shfl(base, 0);
+  extSlot = (int)(uint32_t)base + __popcll(mask & ((1ull << lane) - 1ull));
+  neeSlot = (int)(uint32_t)(base >> 32) + (int)(x - nee);
 }
 
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
           st4(a + k, 0.f, 0.f, 0.f, 0.f);
       }
     }
-    const int slot = enqueue(&P.counters[0], valid);
+    const int slot = enqueue(&P.counters[0], valid); // extension queue 0
     if (valid)
       P.queue[0][slot] = (uint32_t)i;
   }
@@ -222,11 +225,11 @@ template <bool STATS>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int qin)
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
-  const int n = P.counters[qin];
+  const int n = P.counters[2 * qin];
   uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
-  for (int base = nextBatch(&P.counters[4]); base < n; base = nextBatch(&P.counters[4])) {
+  for (int base = nextBatch(&P.counters[kCntFetchExtend]); base < n; base = nextBatch(&P.counters[kCntFetchExtend])) {
     const int i = base + laneId();
     if (i < n) {
       const uint32_t p = P.queue[qin][i];
@@ -255,7 +258,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
 // appends can scan across the wave.
 __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int qin)
 {
-  const int n = P.counters[qin];
+  const int n = P.counters[2 * qin];
   const int qout = qin ^ 1;
   for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
     const int i = w0 + laneId();
@@ -458,17 +461,14 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         st4(P.lastP + p4, newLastP.x, newLastP.y, newLastP.z, newLastPdf);
         st4(P.lastNs + p4, newLastNs.x, newLastNs.y, newLastNs.z, __uint_as_float(newLastNLS));
       }
-      if (nNee)
-        P.neeN[p] = nNee;
+      P.neeN[p] = nNee;
     }
-    const int so = enqueue(&P.counters[qout], cont);
+    // the continuing path to extension queue qout; the light-sample slots of
+    // this vertex (records p*maxRec + [0, nNee)) to the NEE sample queue
+    int so, first;
+    enqueuePair(&P.counters[2 * qout], cont, nNee, so, first);
     if (cont)
       P.queue[qout][so] = p;
-    const int sn = enqueue(&P.counters[3], nNee > 0);
-    if (nNee > 0)
-      P.neeQueue[sn] = p;
-    // light-sample slots of this vertex: records p*maxRec + [0, nNee)
-    const int first = enqueueN(&P.counters[7], nNee);
     for (uint32_t k = 0; k < nNee; k++)
       P.neeSampleQueue[first + k] = p * (uint32_t)F.maxRec + k;
   }
@@ -477,9 +477,9 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
 // One NEE light sample per thread (NextEventEstimation.ispc:54-146): pick a
 // light, sample it, evaluate the BSDF and write the shadow-ray record;
 // records that can contribute go to the shadow queue.
-__global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P)
+__global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int qin)
 {
-  const int n = P.counters[7];
+  const int n = P.counters[2 * (qin ^ 1) + 1];
   for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
     const int i = w0 + laneId();
     bool emit = false;
@@ -530,7 +530,7 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P)
         st4(P.recContrib + 4 * (size_t)r, unshaded.x, unshaded.y, unshaded.z, vNs.w);
       }
     }
-    const int slot = enqueue(&P.counters[2], emit);
+    const int slot = enqueue(&P.counters[kCntShadow], emit);
     if (emit)
       P.shadowQueue[slot] = r;
   }
@@ -541,14 +541,14 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P)
 // Unoccluded records get their result; records that hit only non-opaque
 // primitives go to the transparency queue (k_shadow_trans).
 template <bool STATS>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPaths P)
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(DScene S, DFrame F, DPaths P)
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
-  const int n = P.counters[2];
+  const int n = P.counters[kCntShadow];
   uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
-  for (int base = nextBatch(&P.counters[5]); base < n; base = nextBatch(&P.counters[5])) {
+  for (int base = nextBatch(&P.counters[kCntFetchShadow]); base < n; base = nextBatch(&P.counters[kCntFetchShadow])) {
     const int i = base + laneId();
     bool toTrans = false;
     uint32_t r = 0;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, DFrame F, DPat
         }
       }
     }
-    const int slot = enqueue(&P.counters[6], toTrans);
+    const int slot = enqueue(&P.counters[kCntTrans], toTrans);
     if (toTrans)
       P.transQueue[slot] = r;
   }
@@ -592,7 +592,7 @@ template <bool STATS>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F, DPaths P)
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
-  const int n = P.counters[6];
+  const int n = P.counters[kCntTrans];
   uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
@@ -644,12 +644,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
 
 // sum of the NEE samples of a vertex in light-sample order, added to the
 // path contribution (NextEventEstimation.ispc:150-159, PathSampler.ispc:338)
-__global__ __launch_bounds__(256) void k_resolve(DFrame F, DPaths P)
+__global__ __launch_bounds__(256) void k_resolve(DFrame F, DPaths P, int qin)
 {
-  const int n = P.counters[3];
+  const int n = P.counters[2 * qin];
   for (int i = globalId(); i < n; i += gridStride()) {
-    const uint32_t p = P.neeQueue[i];
+    const uint32_t p = P.queue[qin][i];
     const uint32_t nls = P.neeN[p];
+    if (nls == 0)
+      continue;
     vec3f L = v3(0.f);
     for (uint32_t k = 0; k < nls; k++) {
       const float4 r = ld4(P.recResult + 4 * ((size_t)p * F.maxRec + k));
@@ -922,7 +924,8 @@ extern "C" {
 hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s)
 {
   const DPaths &p = *reinterpret_cast<const DPaths *>(P);
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(8), 0, s, p.counters, 8, keepIndex);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, kNumCounters,
+      keepIndex < 0 ? -1 : 2 * keepIndex);
   return hipGetLastError();
 }
 
@@ -967,10 +970,10 @@ hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, in
   return hipGetLastError();
 }
 
-hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s)
+hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
 {
   hipLaunchKernelGGL(k_nee, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
-      *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P));
+      *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P), queue);
   return hipGetLastError();
 }
 
@@ -990,10 +993,10 @@ hipError_t hpt_shadow(
   return hipGetLastError();
 }
 
-hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int blocks, hipStream_t s)
+hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
 {
   hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DFrame *>(F),
-      *reinterpret_cast<const DPaths *>(P));
+      *reinterpret_cast<const DPaths *>(P), queue);
   return hipGetLastError();
 }
 

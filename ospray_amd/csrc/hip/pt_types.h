@@ -157,7 +157,7 @@ struct DPaths
   uint32_t *hitPrim; // prim | half << 31
   uint32_t *ids; // 4: primID, geomID, instID, medium
   float *aux; // 16: position.xyz, firstNormal.xyz, normal.xyz, albedo.xyz, pad
-  uint32_t *neeN; // light samples of the pending NEE vertex
+  uint32_t *neeN; // light samples of the vertex shaded this bounce (0: none)
   // NEE vertex record, written by k_shade at a smooth vertex
   float *vtxP; // 4: P.xyz, epsilon
   float *vtxNg; // 4: Ng.xyz, material (bits)
@@ -173,15 +173,23 @@ struct DPaths
   // queues and counters
   uint32_t *queue[2]; // extension queues (ping-pong)
   uint32_t *shadowQueue;
-  uint32_t *neeQueue;
   uint32_t *transQueue; // shadow records that hit only non-opaque surfaces
   uint32_t *neeSampleQueue; // record slots (path * maxRec + sample) to evaluate
-  // [0..1] extension queues, [2] shadow records, [3] NEE vertices, [4] and
-  // [5] extension/shadow work fetch, [6] transparency queue, [7] NEE samples
+  // queue counters, see kCnt*: the pair (2q, 2q+1) is one 64-bit word so
+  // that k_shade appends to extension queue q and to the NEE sample queue
+  // with a single atomic
   int32_t *counters;
   uint32_t *overflow; // traversal stack overflow, per resident thread
   unsigned long long *stats; // node visits, primitive tests, traversals: [0..2] extension, [3..5] shadow
 };
+
+// queue counters: [2q] entries of extension queue q (q = 0, 1), [2q+1] NEE
+// light samples appended while queue q was filled, then:
+constexpr int kCntShadow = 4; // shadow-ray records
+constexpr int kCntFetchExtend = 5; // work fetch of the extension traversal
+constexpr int kCntFetchShadow = 6; // work fetch of the shadow traversal
+constexpr int kCntTrans = 7; // records for the transparency pass
+constexpr int kNumCounters = 8;
 
 // path flags in ld[3]
 constexpr uint32_t FLAG_DEPTH_MASK = 0xffu; // bits 0-7
