@@ -283,6 +283,12 @@ class OracleFrameBuffer:
             return np.ctypeslib.as_array((C.c_uint32 * n).from_address(p)).copy().reshape(self.h, self.w)
         return np.ctypeslib.as_array((C.c_float * (3 * n)).from_address(p)).copy().reshape(self.h, self.w, 3)
 
+    def task_variance(self) -> np.ndarray:
+        """Per-8x8-task error of the last variance pass (row-major tasks)."""
+        n = C.c_int(0)
+        p = lib().orc_fb_task_variance(self.ptr, C.byref(n))
+        return np.ctypeslib.as_array((C.c_float * n.value).from_address(p)).copy()
+
     def accum_color(self) -> np.ndarray:
         p = lib().orc_fb_accum_color(self.ptr)
         n = self.w * self.h

@@ -1546,6 +1546,17 @@ void renderFrame(const Scene &s, FrameBuffer &fb, const CameraData &cam, const R
   c.cameraDir = cam.direction;
   c.projectedDepth = fb.projectedDepth;
 
+  // LocalFrameBuffer::getRenderTaskIDs (fb/LocalFB.cpp:238-261): once
+  // frameID >= minimumAdaptiveFrames (FrameBuffer.h:183-186) and a variance
+  // buffer exists, only the 8x8 tasks whose last error exceeds the threshold
+  // are rendered; the others keep their buffers
+  const int tasksX = (fb.w + 7) / 8;
+  const bool adaptive = r.errorThreshold > 0.f && !fb.variance.empty()
+      && fb.frameID >= std::max(2, 16 / std::max(1, r.spp));
+  auto taskActive = [&](uint32_t x, uint32_t y) {
+    return !adaptive || fb.taskVariance[(size_t)(y / 8) * tasksX + x / 8] > r.errorThreshold;
+  };
+
   std::atomic<int> nextRow{y0};
   auto worker = [&]() {
     for (;;) {
@@ -1553,6 +1564,8 @@ void renderFrame(const Scene &s, FrameBuffer &fb, const CameraData &cam, const R
       if (y >= y1)
         break;
       for (int x = 0; x < fb.w; x++) {
+        if (!taskActive((uint32_t)x, (uint32_t)y))
+          continue;
         const ScreenSample smp = renderPixel(c, fb, cam, (uint32_t)x, (uint32_t)y);
         accumulateSample(fb, (uint32_t)x, (uint32_t)y, smp);
       }

@@ -1515,6 +1515,10 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   F.fbObjID = fb->objID.as<uint32_t>();
   F.fbInstID = fb->instID.as<uint32_t>();
   F.taskVariance = fb->taskVar.as<float>();
+  // LocalFrameBuffer::getRenderTaskIDs (fb/LocalFB.cpp:238-261) with
+  // minimumAdaptiveFrames = max(2, 16 / spp) (fb/FrameBuffer.h:183-186)
+  if (r->errorThreshold > 0.f && fb->variance.ptr && fb->frameID >= std::max(2, 16 / std::max(1, r->spp)))
+    F.adaptiveThreshold = r->errorThreshold;
   F.maxRec = (int)std::max<uint32_t>(1u, F.numFirstLightSamples);
 
   if (!finishedAccum) {

@@ -104,6 +104,13 @@ __device__ __forceinline__ int pixelSlot(const DFrame &F, int ix, int iy)
   return ((iy >> 3) * F.tilesX + (ix >> 3)) * 64 + (iy & 7) * 8 + (ix & 7);
 }
 
+// adaptive accumulation: is the 8x8 task of pixel (ix, iy) rendered this
+// frame (LocalFB.cpp:238-261)?  adaptiveThreshold 0: every task is
+__device__ __forceinline__ bool taskActive(const DFrame &F, int ix, int iy)
+{
+  return F.adaptiveThreshold == 0.f || F.taskVariance[(iy >> 3) * F.tilesX + (ix >> 3)] > F.adaptiveThreshold;
+}
+
 __device__ __forceinline__ uint32_t clz32(uint32_t x) { return x ? (uint32_t)__clz((int)x) : 32u; }
 
 // pixel filter (PixelFilterDispatch.ispc:7-19, LUTPixelFilter.ispc:9-15,
@@ -157,7 +164,7 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
     const int q = i - sLocal * F.numPixPadded;
     int ix, iy;
     slotPixel(F, q, ix, iy);
-    const bool valid = ix < F.width && iy < F.height;
+    const bool valid = ix < F.width && iy < F.height && taskActive(F, ix, iy);
     if (valid) {
       const uint32_t spp = (uint32_t)F.spp;
       const uint32_t s = (uint32_t)(F.passS0 + sLocal);
@@ -672,6 +679,8 @@ __global__ __launch_bounds__(256) void k_finish_pass(DFrame F, DPaths P)
   const int npix = F.width * F.height;
   for (int pix = globalId(); pix < npix; pix += gridStride()) {
     const int ix = pix % F.width, iy = pix / F.width;
+    if (!taskActive(F, ix, iy))
+      continue;
     const int q = pixelSlot(F, ix, iy);
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     float aux[12];
@@ -736,6 +745,8 @@ __global__ __launch_bounds__(256) void k_fb_accumulate(DFrame F)
   const float accScale = 1.0f / (float)(accumID + 1);
   const float accVarScale = 1.0f / (float)(accumID / 2 + 1);
   for (int pix = globalId(); pix < npix; pix += gridStride()) {
+    if (!taskActive(F, pix % F.width, pix / F.width))
+      continue;
     const float4 a = ld4(F.accRGBA + 4 * (size_t)pix);
     const float r = a.x * rspp, g = a.y * rspp, b = a.z * rspp, al = a.w * rspp;
     if (F.fbColor) {

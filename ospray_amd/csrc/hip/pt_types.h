@@ -241,6 +241,9 @@ struct DFrame
   float *fbPosition;
   uint32_t *fbPrimID, *fbObjID, *fbInstID;
   float *taskVariance;
+  // adaptive accumulation: > 0 renders only the tasks whose taskVariance
+  // exceeds it
+  float adaptiveThreshold;
 };
 
 } // namespace hpt

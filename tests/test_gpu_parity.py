@@ -136,6 +136,20 @@ def test_reset_accumulation(hip_device):
     assert np.array_equal(A.map_color(fb, 24, 24, K.OSP_FB_RGBA32F), ofb.channel(K.OSP_FB_COLOR))
 
 
+def test_adaptive_accumulation(hip_device):
+    # varianceThreshold between the task errors: from frame 2 on (spp 8)
+    # only the noisier 8x8 tasks are rendered
+    sc = scenes.c0(48, 40, spp=8)
+    ch = K.OSP_FB_COLOR | K.OSP_FB_ACCUM | K.OSP_FB_VARIANCE | K.OSP_FB_DEPTH | K.OSP_FB_ID_OBJECT
+    ofb, _, _ = orc.render_scene(sc, frames=2, channels=ch)
+    thr = float(np.median(ofb.task_variance()))
+    rp = {"varianceThreshold": thr}
+    s, fb, ofb, _, oren = _render_both(sc, frames=5, channels=ch, renderer_params=rp)
+    assert np.array_equal(A.map_color(fb, 48, 40, K.OSP_FB_RGBA32F), ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ch)
+    assert K.lib().ospGetVariance(fb.h) == pytest.approx(ofb.variance(oren), rel=1e-6)
+
+
 def test_pick(hip_device):
     sc = scenes.c0(64, 64)
     s = scenes.to_ospray(sc)
