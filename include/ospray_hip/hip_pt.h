@@ -45,12 +45,16 @@ HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPath
  * queue `queue`: light selection and sampling, BSDF evaluation, shadow-ray
  * record (NextEventEstimation.ispc:54-146). */
 HPT_API hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
-/* transparentShadow of every NEE record (TransparentShadow.ispc:18-85):
- * an occlusion pass, then the closest-hit transparency loop for the records
- * that hit only non-opaque surfaces;
+/* transparentShadow of every NEE record (TransparentShadow.ispc:18-85),
+ * first pass: occlusion with early exit on an opaque primitive; records
+ * that hit only non-opaque surfaces are queued for hpt_shadow_trans.
  * stats != 0 adds node visits, primitive tests and traversals to
  * P->stats[3..5]. */
 HPT_API hipError_t hpt_shadow(
+    const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s);
+/* transparentShadow, second pass: the closest-hit loop through non-opaque
+ * surfaces (TransparentShadow.ispc:36-84) for the queued records. */
+HPT_API hipError_t hpt_shadow_trans(
     const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s);
 /* Ordered sum of the NEE samples of every vertex shaded from extension
  * queue `queue` (NextEventEstimation.ispc:150-159). */

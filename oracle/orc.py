@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liborc.so")
 
 ORC_GEOM = {"triangles": 0, "quads": 1, "spheres": 2}
-ORC_MAT = {"obj": 0, "metal": 1, "glass": 2}
+ORC_MAT = {"obj": 0, "metal": 1, "glass": 2, "luminous": 3}
 ORC_LIGHT = {"quad": 0, "distant": 1, "ambient": 2}
 
 
@@ -42,7 +42,8 @@ class OrcMaterial(C.Structure):
     _fields_ = [("type", C.c_int32), ("kd", C.c_float * 3), ("d", C.c_float), ("tf", C.c_float * 3),
                 ("eta", C.c_float * 3), ("k", C.c_float * 3), ("roughness", C.c_float), ("etaInside", C.c_float),
                 ("etaOutside", C.c_float), ("attenuationColorInside", C.c_float * 3),
-                ("attenuationColorOutside", C.c_float * 3), ("attenuationDistance", C.c_float)]
+                ("attenuationColorOutside", C.c_float * 3), ("attenuationDistance", C.c_float),
+                ("color", C.c_float * 3), ("intensity", C.c_float), ("transparency", C.c_float)]
 
 
 class OrcLight(C.Structure):
@@ -70,7 +71,7 @@ class OrcRenderer(C.Structure):
                 ("maxScatteringEvents", C.c_int32), ("lightSamples", C.c_int32),
                 ("limitIndirectLightSamples", C.c_int32), ("minContribution", C.c_float),
                 ("maxContribution", C.c_float), ("backgroundColor", C.c_float * 4), ("pixelFilter", C.c_int32),
-                ("varianceThreshold", C.c_float)]
+                ("varianceThreshold", C.c_float), ("geometryLights", C.c_int32)]
 
 
 _lib = None
@@ -151,6 +152,7 @@ def make_renderer(p: dict) -> OrcRenderer:
     pf = p.get("pixelFilter", 2)
     r.pixelFilter = int(getattr(pf, "v", pf))
     r.varianceThreshold = float(p.get("varianceThreshold", 0.0))
+    r.geometryLights = int(bool(p.get("geometryLights", True)))
     return r
 
 
@@ -213,6 +215,9 @@ class OracleScene:
             mats[i].attenuationColorInside = _f3(m.get("attenuationColorInside", ac))
             mats[i].attenuationColorOutside = _f3(m.get("attenuationColorOutside", (1.0, 1.0, 1.0)))
             mats[i].attenuationDistance = m.get("attenuationDistance", 1.0)
+            mats[i].color = _f3(m.get("color"), (1.0, 1.0, 1.0))  # Luminous.cpp:37-48
+            mats[i].intensity = m.get("intensity", 1.0)
+            mats[i].transparency = m.get("transparency", 0.0)
         lights = (OrcLight * max(1, len(scene.lights)))()
         for i, l in enumerate(scene.lights):
             lights[i].type = ORC_LIGHT[l["type"]]

@@ -71,6 +71,110 @@ inline vec3f reflect(vec3f I, vec3f N) { return (2.0f * dot(I, N)) * N - I; } //
 
 // ---- setup --------------------------------------------------------------------
 
+static constexpr int kLightGeometry = 3;
+
+inline vec2f uniformSampleTriangleUV(vec2f s) // math/sampling.ih:165-173
+{
+  const float su = std::sqrt(s.x);
+  return v2(s.y * su, (1.0f - s.y) * su);
+}
+
+// the vertices a, b, c, d of a mesh primitive as Mesh_sampleArea reads them
+// (a triangle repeats its last vertex: index (x, y, z, z))
+static bool meshPrim(const OrcGeometry &g, int64_t p, vec3f &a, vec3f &b, vec3f &c, vec3f &d)
+{
+  bool quad = false;
+  uint32_t ix[4];
+  if (g.type == ORC_GEOM_TRIANGLES) {
+    const uint32_t *q = g.index + 3 * p;
+    ix[0] = q[0], ix[1] = q[1], ix[2] = q[2], ix[3] = q[2];
+  } else {
+    const uint32_t *q = g.index + 4 * p;
+    ix[0] = q[0], ix[1] = q[1], ix[2] = q[2], ix[3] = q[3];
+    quad = ix[2] != ix[3];
+  }
+  a = ld3(g.vertex, ix[0]);
+  b = ld3(g.vertex, ix[1]);
+  d = ld3(g.vertex, ix[3]);
+  c = quad ? ld3(g.vertex, ix[2]) : d;
+  return quad;
+}
+
+// world-space area of a primitive (Mesh_getAreas, geometry/Mesh.ispc:377-415;
+// Spheres_getAreas, geometry/Spheres.ispc:76-103)
+static float primArea(const OrcScene &, const OrcGeometry &g, int64_t p, const affine3f &xfm)
+{
+  if (g.type == ORC_GEOM_SPHERES) {
+    const vec3f sc = v3(length(xfm.l.vx), length(xfm.l.vy), length(xfm.l.vz));
+    const float scale = 4.0f * kPi * sqr((sc.x + sc.y + sc.z) / 3.f);
+    return scale * sqr(g.spheres[4 * p + 3]);
+  }
+  vec3f a, b, c, d;
+  const bool quad = meshPrim(g, p, a, b, c, d);
+  const vec3f e1 = xfmVector(xfm, a - d);
+  const vec3f e2 = xfmVector(xfm, b - d);
+  float area = length(cross(e1, e2));
+  if (quad) {
+    const vec3f e3 = xfmVector(xfm, c - d);
+    area += length(cross(e2, e3));
+  }
+  return area * 0.5f;
+}
+
+struct SampleAreaRes
+{
+  vec3f pos, normal;
+  float epsilon;
+};
+
+// Mesh_sampleArea (geometry/Mesh.ispc:270-351), Spheres_sampleArea
+// (geometry/Spheres.ispc:44-74)
+static SampleAreaRes sampleArea(const OrcGeometry &g, int64_t p, const affine3f &xfm, const affine3f &rcpXfm, vec2f s)
+{
+  SampleAreaRes res;
+  const float xfmScale = maxf(std::fabs(xfm.l.vx.x), maxf(std::fabs(xfm.l.vy.y), std::fabs(xfm.l.vz.z)));
+  if (g.type == ORC_GEOM_SPHERES) {
+    const float *sp = g.spheres + 4 * p;
+    const vec3f localNormal = uniformSampleSphere(sp[3], s);
+    res.pos = xfmPoint(xfm, v3(sp[0], sp[1], sp[2]) + localNormal);
+    res.normal = normalize(mulT(rcpXfm.l, localNormal)); // transposed(rcp_xfm.l) * n
+    res.epsilon = sp[3] * kUlpEpsilon;
+    res.epsilon *= xfmScale;
+    return res;
+  }
+  vec3f a, b, c, d;
+  const bool quad = meshPrim(g, p, a, b, c, d);
+  res.epsilon = reduce_max(vmax(vmax(vabs(a), vabs(b)), vmax(vabs(c), vabs(d)))) * kUlpEpsilon;
+  res.epsilon *= xfmScale;
+  const vec3f e1 = xfmVector(xfm, a - d);
+  const vec3f e2 = xfmVector(xfm, b - d);
+  const vec3f m1 = cross(e1, e2);
+  res.normal = m1;
+  vec3f v0 = a, v1 = b, v2 = d;
+  vec2f sp = s;
+  if (quad) {
+    const vec3f e3 = xfmVector(xfm, c - d);
+    const vec3f m2 = cross(e2, e3);
+    const float a1 = length(m1);
+    const float a2 = length(m2);
+    const float p1 = a1 * rcp(a1 + a2);
+    if (s.x < p1) {
+      sp.x = sp.x * rcp(p1);
+    } else {
+      sp.x = (s.x - p1) * rcp(1.f - p1);
+      v0 = c;
+      v1 = d;
+      v2 = b;
+      res.normal = m2;
+    }
+  }
+  const vec2f uv = uniformSampleTriangleUV(sp);
+  const vec3f localPos = v0 + uv.x * (v1 - v0) + uv.y * (v2 - v0);
+  res.normal = normalize(res.normal);
+  res.pos = xfmPoint(xfm, localPos);
+  return res;
+}
+
 void setupScene(Scene &s, const OrcScene &d)
 {
   s.world.build(d);
@@ -92,6 +196,10 @@ void setupScene(Scene &s, const OrcScene &d)
       md.eta = v3(m.eta[0], m.eta[1], m.eta[2]);
       md.k = v3(m.k[0], m.k[1], m.k[2]);
       md.roughness = m.roughness;
+    } else if (m.type == ORC_MAT_LUMINOUS) { // materials/Luminous.cpp:37-48, Luminous.h:25-28
+      md.radiance = v3(m.color[0], m.color[1], m.color[2]) * m.intensity;
+      md.transparency = m.transparency;
+      md.emissive = reduce_max(md.radiance) > 0.f;
     } else { // materials/Glass.cpp:36-62
       const float dist = maxf(m.attenuationDistance, 1e-5f);
       md.inside.ior = m.etaInside;
@@ -102,7 +210,36 @@ void setupScene(Scene &s, const OrcScene &d)
                                    std::log(m.attenuationColorOutside[2])) / dist;
     }
   }
-  // lights: world lights in array order (PathTracerData.cpp:45-50)
+  // geometry lights (PathTracerData.cpp:121-162): every emissive model of
+  // every instance, in instance and group order; the model's areaPDF is the
+  // one of the last instance that gathers it (GeometricModel.ispc:14-55)
+  s.modelAreaPDF.assign(d.numModels, 0.f);
+  std::vector<LightData> geo;
+  for (int i = 0; i < d.numInstances; i++) {
+    const OrcGroup &grp = d.groups[d.instances[i].group];
+    for (int k = 0; k < grp.numModels; k++) {
+      const int om = d.groupModels[grp.firstModel + k];
+      const OrcModel &model = d.models[om];
+      if (model.material < 0 || !s.materials[model.material].emissive)
+        continue;
+      const OrcGeometry &g = d.geometries[model.geometry];
+      if (g.numPrimitives <= 0)
+        continue;
+      LightData ld{};
+      ld.type = kLightGeometry;
+      ld.visible = true;
+      ld.inst = i;
+      ld.model = om;
+      ld.primCDF.resize((size_t)g.numPrimitives);
+      for (int64_t p = 0; p < g.numPrimitives; p++)
+        ld.primCDF[p] = primArea(d, g, p, s.world.xfm[i]);
+      ld.geoPdf = 1.f / Distribution1D_create((int)g.numPrimitives, ld.primCDF.data());
+      s.modelAreaPDF[om] = ld.geoPdf;
+      geo.push_back(std::move(ld));
+    }
+  }
+  s.numGeoLights = (uint32_t)geo.size();
+  // world lights in array order (PathTracerData.cpp:45-50)
   s.lights.resize(d.numLights);
   for (int i = 0; i < d.numLights; i++) {
     const OrcLight &l = d.lights[i];
@@ -150,9 +287,13 @@ void setupScene(Scene &s, const OrcScene &d)
         ld.radiance = v3(0.f);
     }
   }
-  s.lightsCDF.assign(d.numLights, 1.f);
+  s.worldLightsCDF.assign(d.numLights, 1.f);
   if (d.numLights)
-    Distribution1D_create(d.numLights, s.lightsCDF.data());
+    Distribution1D_create(d.numLights, s.worldLightsCDF.data());
+  s.lights.insert(s.lights.begin(), std::make_move_iterator(geo.begin()), std::make_move_iterator(geo.end()));
+  s.lightsCDF.assign(s.lights.size(), 1.f);
+  if (!s.lights.empty())
+    Distribution1D_create((int)s.lights.size(), s.lightsCDF.data());
 }
 
 void setupCamera(CameraData &c, const OrcCamera &d) // PerspectiveCamera.cpp:30-110
@@ -175,6 +316,7 @@ void setupCamera(CameraData &c, const OrcCamera &d) // PerspectiveCamera.cpp:30-
 
 void setupRenderer(RendererData &r, const OrcRenderer &d) // Renderer.cpp:38-92, PathTracer.cpp:29-51
 {
+  r.geometryLights = d.geometryLights != 0;
   r.spp = d.pixelSamples < 1 ? 1 : d.pixelSamples;
   r.maxDepth = d.maxPathLength < 0 ? 0 : d.maxPathLength;
   r.rouletteDepth = d.roulettePathLength;
@@ -299,10 +441,12 @@ static void postIntersect(const Scene &s, const Ray &ray, const Hit &hit, DG &dg
   const OrcScene &d = w.desc;
   const OrcInstance &inst = d.instances[hit.inst];
   const OrcGroup &grp = d.groups[inst.group];
-  const OrcModel &model = d.models[d.groupModels[grp.firstModel + hit.model]];
+  const int om = d.groupModels[grp.firstModel + hit.model];
+  const OrcModel &model = d.models[om];
   const OrcGeometry &g = d.geometries[model.geometry];
 
   dg.primID = hit.prim;
+  dg.areaPDF = s.modelAreaPDF[om];
   dg.material = -1;
   dg.color = {1.f, 1.f, 1.f, 1.f};
   dg.dPds = v3(1.f, 0.f, 0.f);
@@ -495,6 +639,15 @@ static bool getBSDF(const Scene &s, const DG &dg, const Medium &medium, BSDF &b)
   if (dg.material < 0)
     return false;
   const MaterialData &m = s.materials[dg.material];
+  if (m.type == ORC_MAT_LUMINOUS) { // Luminous.ispc:15-32: Transmission or no BSDF
+    if (!(m.transparency > 0.0f))
+      return false;
+    b.kind = 5;
+    b.albedo = v3(m.transparency);
+    b.scatteringType = SPEC_TRANS;
+    b.frame = frame(dg.Ns);
+    return true;
+  }
   if (m.type == ORC_MAT_OBJ) { // OBJ.ispc:29-98
     b.kind = 1;
     b.albedo = v3(0.f);
@@ -614,6 +767,14 @@ static EvalRes bsdfEval(const BSDF &b, vec3f wo, vec3f wi)
 // BSDF_dispatch_sample (bsdfs/BSDF.ispc:61-112)
 static SampleRes bsdfSample(const BSDF &b, vec3f wo, vec2f s, float ss)
 {
+  if (b.kind == 5) { // Transmission_sample (bsdfs/Transmission.ih:11-20)
+    SampleRes res;
+    res.wi = neg(wo);
+    res.pdf = kInf;
+    res.type = SPEC_TRANS;
+    res.weight = b.albedo;
+    return res;
+  }
   SampleRes zero;
   zero.weight = v3(0.f);
   zero.wi = v3(0.f);
@@ -791,6 +952,8 @@ static vec3f getTransparency(const Scene &s, const DG &dg, const Ray &ray, const
     const float d = m.d * dg.color.w;
     return m.Tf * d + v3(1.f - d);
   }
+  if (m.type == ORC_MAT_LUMINOUS) // Luminous.ispc:34-44
+    return v3(m.transparency);
   if (m.type == ORC_MAT_GLASS) { // Glass.ispc:37-52
     const float eta = eq(medium, m.outside) ? m.outside.ior * rcp(m.inside.ior) : m.inside.ior * rcp(m.outside.ior);
     const float cosThetaO = maxf(-dot(ray.dir, dg.Ns), 0.0f);
@@ -947,9 +1110,26 @@ static vec3f cosineWeightedRNG(vec3f q0, vec3f e0, vec3f e1, vec3f P, vec3f Ng, 
   return v3(u, v, pdf);
 }
 
-static LightSample lightSample(const LightData &l, const DG &dg, vec2f s)
+static LightSample lightSample(const Scene &sc, const LightData &l, const DG &dg, vec2f s)
 {
   LightSample res;
+  if (l.type == kLightGeometry) { // GeometryLight_sample (render/pathtracer/GeometryLight.ispc:60-123)
+    const OrcScene &d = sc.world.desc;
+    const OrcModel &model = d.models[l.model];
+    const OrcGeometry &g = d.geometries[model.geometry];
+    const Sample1D smp = Distribution1D_sample((int)l.primCDF.size(), l.primCDF.data(), 0, s.x);
+    const SampleAreaRes as = sampleArea(g, smp.idx, sc.world.xfm[l.inst], sc.world.rcpXfm[l.inst], v2(smp.frac, s.y));
+    const vec3f dir = as.pos - dg.P;
+    const float dist = length(dir);
+    res.dir = dir / dist;
+    const float rcosd = rcp(std::fabs(dot(as.normal, res.dir)));
+    res.pdf = l.geoPdf * sqr(dist) * rcosd;
+    // calcEpsilon(P, t) (common/DifferentialGeometry.ih:48-51)
+    const float epsP = maxf(reduce_max(vabs(as.pos)), dist) * kUlpEpsilon;
+    res.dist = dist - maxf(as.epsilon, epsP) * rcosd;
+    res.weight = sc.materials[model.material].radiance * rcp(res.pdf);
+    return res;
+  }
   if (l.type == ORC_LIGHT_QUAD) { // QuadLight.ispc:82-136
     if (s.x == 0.0f && s.y == 0.0f) { // SampleArea at the centre
       const vec3f p = l.position + l.edge1 * 0.5f + l.edge2 * 0.5f;
@@ -1082,7 +1262,9 @@ struct PathCtx
 {
   const Scene *scene;
   const RendererData *r;
-  uint32_t numLights, numFirst, numIndirect;
+  const LightData *lights; // [geometry lights (numGeoLights), world lights]
+  const float *lightsCDF;
+  uint32_t numLights, numGeoLights, numFirst, numIndirect;
   vec3f cameraDir;
   bool projectedDepth;
 };
@@ -1131,9 +1313,9 @@ static vec3f nextEventEstimation(const PathCtx &c, PathState &ps, PathVertex &pv
   for (uint32_t i = 0; i < n; i++) {
     float ss;
     const vec2f s = LDSampler_get3LightSamples(ps.ld, i, n > 1, ss);
-    const Sample1D sel = Distribution1D_sample((int)c.numLights, c.scene->lightsCDF.data(), 0, ss);
+    const Sample1D sel = Distribution1D_sample((int)c.numLights, c.lightsCDF, 0, ss);
     const float selProb = sel.prob * (float)n;
-    LightSample ls = lightSample(c.scene->lights[sel.idx], pv.dg, s);
+    LightSample ls = lightSample(*c.scene, c.lights[sel.idx], pv.dg, s);
     ls.weight = ls.weight / selProb;
     ls.pdf *= selProb;
     if (reduce_max(ls.weight) <= 0.0f || ls.pdf <= 0.0f)
@@ -1160,13 +1342,13 @@ static vec3f nextEventEstimation(const PathCtx &c, PathState &ps, PathVertex &pv
 static vec3f evaluateVirtualLights(const PathCtx &c, const PathState &ps, const PathVertex &last, const PathVertex &pv, const Ray &ray)
 {
   vec3f L = v3(0.f);
-  if (c.numLights == 0)
+  if (c.numLights <= c.numGeoLights)
     return L;
   const float lower = distance(last.dg.P, ray.org);
   const float upper = pv.type == V_ENVIRONMENT ? kInf : distance(last.dg.P, ray.org + ray.t * ray.dir);
   const float selectionPDF = (float)last.numLightSamples / (float)c.numLights;
-  for (uint32_t i = 0; i < c.numLights; i++) {
-    const LightData &l = c.scene->lights[i];
+  for (uint32_t i = c.numGeoLights; i < c.numLights; i++) {
+    const LightData &l = c.lights[i];
     if (!ps.straightPath || l.visible) {
       const LightEval le = lightEval(l, last.dg, ray.dir, lower, upper);
       if (reduce_max(le.radiance) > 0.0f) {
@@ -1178,6 +1360,26 @@ static vec3f evaluateVirtualLights(const PathCtx &c, const PathState &ps, const 
     }
   }
   return L;
+}
+
+// evaluateGeometryLights (render/pathtracer/GeometryLight.ispc:22-56): the
+// emission of a hit luminous surface, MIS-weighted against NEE
+static vec3f evaluateGeometryLights(const PathCtx &c, const PathState &ps, const PathVertex &last, const PathVertex &pv, const Ray &ray)
+{
+  if (pv.dg.material < 0)
+    return v3(0.f);
+  const MaterialData &m = c.scene->materials[pv.dg.material];
+  if (m.type != ORC_MAT_LUMINOUS || !(reduce_max(m.radiance) > 0.f))
+    return v3(0.f);
+  float misWeight = 1.0f;
+  if (c.numGeoLights > 0 && last.numLightSamples > 0) {
+    const float lightSelectionPDF = (float)last.numLightSamples / (float)c.numLights;
+    const float cosd = dot(pv.dg.Ng, ray.dir);
+    const vec3f vdis = last.dg.P - pv.dg.P;
+    const float lePdf = pv.dg.areaPDF * dot(vdis, vdis) / std::fabs(cosd);
+    misWeight = misHeuristic(last.pdf_w, lightSelectionPDF * lePdf);
+  }
+  return ps.throughput * m.radiance * misWeight;
 }
 
 // PathSampler.ispc:109-415
@@ -1216,10 +1418,12 @@ static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &s
     ps.contribution = ps.contribution + evaluateVirtualLights(c, ps, last, pv, ray);
     if (pv.type == V_ENVIRONMENT)
       break;
-    // (numGeoLights == 0: no emissive geometry lights)
-    if (ps.depth >= (uint32_t)r.maxDepth)
+    // with geometry lights the emission at maxDepth is still evaluated
+    // (PathSampler.ispc:249-255)
+    if (ps.depth >= (uint32_t)r.maxDepth && c.numGeoLights == 0)
       break;
     postIntersect(*c.scene, ray, hit, pv.dg, true);
+    ps.contribution = ps.contribution + evaluateGeometryLights(c, ps, last, pv, ray);
     if (ps.depth == 0) {
       sample.position = pv.dg.P;
       sample.firstNormal = pv.dg.Ns;
@@ -1539,7 +1743,12 @@ void renderFrame(const Scene &s, FrameBuffer &fb, const CameraData &cam, const R
   PathCtx c;
   c.scene = &s;
   c.r = &r;
-  c.numLights = (uint32_t)s.lights.size();
+  // PathTracerData without geometry lights when the renderer disables them
+  // (PathTracer.cpp:49, PathTracerData.cpp:37-43)
+  c.numGeoLights = r.geometryLights ? s.numGeoLights : 0u;
+  c.lights = s.lights.data() + (s.numGeoLights - c.numGeoLights);
+  c.numLights = (uint32_t)s.lights.size() - (s.numGeoLights - c.numGeoLights);
+  c.lightsCDF = r.geometryLights ? s.lightsCDF.data() : s.worldLightsCDF.data();
   c.numFirst = r.numLightSamples >= 0 && c.numLights > 0 ? (uint32_t)r.numLightSamples
                                                        : std::min(kMaxLightSamples, c.numLights);
   c.numIndirect = r.limitIndirectLightSamples && c.numFirst >= 1 ? 1u : c.numFirst;

@@ -27,6 +27,7 @@ struct DG
   vec3f P, Ng, Ns, dPds, dPdt;
   vec4f color;
   float epsilon;
+  float areaPDF; // of the model as a geometry light (GeometricModel.ih:58)
   int material; // -1 = none
   uint32_t primID, objID, instID;
 };
@@ -49,6 +50,10 @@ struct MaterialData
   float roughness;
   // glass
   Medium inside, outside;
+  // luminous (Luminous.cpp:37-48)
+  vec3f radiance;
+  float transparency;
+  bool emissive;
 };
 
 struct LightData
@@ -62,6 +67,11 @@ struct LightData
   linear3f frame;
   vec3f irradiance;
   float cosAngle, pdf;
+  // geometry light (GeometryLightShared.h): an emissive model of one
+  // instance, primitives chosen by area
+  int inst, model; // instance index, OrcModel index
+  std::vector<float> primCDF;
+  float geoPdf; // 1 / total world-space area
 };
 
 struct CameraData
@@ -81,14 +91,19 @@ struct RendererData
   int lutSize;
   std::vector<float> lutCdfX, lutCdfY;
   float errorThreshold;
+  bool geometryLights;
 };
 
 struct Scene
 {
   World world;
   std::vector<MaterialData> materials;
+  // [geometry lights, world lights] (PathTracerData.cpp:32-82) and the
+  // uniform selection CDFs with and without the geometry lights
   std::vector<LightData> lights;
-  std::vector<float> lightsCDF;
+  uint32_t numGeoLights = 0;
+  std::vector<float> lightsCDF, worldLightsCDF;
+  std::vector<float> modelAreaPDF; // per OrcModel
 };
 
 struct FrameBuffer

@@ -57,7 +57,8 @@ enum OrcMaterialType
 {
   ORC_MAT_OBJ = 0,
   ORC_MAT_METAL = 1,
-  ORC_MAT_GLASS = 2
+  ORC_MAT_GLASS = 2,
+  ORC_MAT_LUMINOUS = 3
 };
 
 typedef struct
@@ -68,6 +69,7 @@ typedef struct
   float etaInside, etaOutside; // glass
   float attenuationColorInside[3], attenuationColorOutside[3];
   float attenuationDistance;
+  float color[3], intensity, transparency; // luminous
 } OrcMaterial;
 
 enum OrcLightType
@@ -124,6 +126,7 @@ typedef struct
   float backgroundColor[4];
   int32_t pixelFilter; // OSPPixelFilterType
   float varianceThreshold;
+  int32_t geometryLights; // emissive geometry as NEE lights (PathTracer.cpp:49)
 } OrcRenderer;
 
 #ifdef __cplusplus

@@ -26,6 +26,7 @@
 #include "ospray/ospray_hip.h"
 #include "ospray_hip/hip_pt.h"
 #include "pt_sampler.h"
+#include "pt_shade.h"
 #include "pt_types.h"
 
 using namespace ospcore;
@@ -283,9 +284,23 @@ struct Material : public ManagedObject
         inside.attenuation[k] = std::log(acolIn[k]) / dist;
         outside.attenuation[k] = std::log(acolOut[k]) / dist;
       }
+    } else if (subtype == "luminous") { // Luminous.cpp:37-48
+      m.type = hpt::MAT_LUMINOUS;
+      const float one[3] = {1.f, 1.f, 1.f};
+      float color[3];
+      getVec3f("color", color, one);
+      const float intensity = getFloat("intensity", 1.f);
+      for (int k = 0; k < 3; k++)
+        m.radiance[k] = color[k] * intensity;
+      m.transparency = getFloat("transparency", 0.f);
     } else {
       throw std::runtime_error("unsupported material type '" + subtype + "'");
     }
+  }
+  // Luminous::isEmissive (Luminous.h:25-28)
+  bool emissive() const
+  {
+    return m.type == hpt::MAT_LUMINOUS && std::max(m.radiance[0], std::max(m.radiance[1], m.radiance[2])) > 0.f;
   }
   bool opaque() const
   {
@@ -293,6 +308,8 @@ struct Material : public ManagedObject
       return true;
     if (m.type == hpt::MAT_GLASS)
       return false;
+    if (m.type == hpt::MAT_LUMINOUS)
+      return !(m.transparency != 0.f);
     const vec3f T = hpt::ld3(m.Tf) * m.d + v3(1.f - m.d);
     return T.x == 0.f && T.y == 0.f && T.z == 0.f;
   }
@@ -470,7 +487,7 @@ struct World : public ManagedObject
   std::vector<std::unique_ptr<Ref<Light>>> lightRefs;
   std::vector<Instance *> instances;
   std::vector<Material *> materials; // index order of the material table
-  DevBuf dNodes, dPrims, dInstances, dModels, dMaterials, dMedia, dLights, dLightsCDF;
+  DevBuf dNodes, dPrims, dInstances, dModels, dMaterials, dMedia, dLights, dLightsCDF, dWorldLightsCDF, dGeoCDF;
   hpt::DScene scene{};
   bool built = false;
   int64_t totalNodes = 0, totalPrims = 0;
@@ -532,7 +549,7 @@ struct Camera : public ManagedObject
 struct Renderer : public ManagedObject
 {
   int spp = 1, maxDepth = 20, rouletteDepth = 5, maxScatteringEvents = 20, numLightSamples = -1;
-  bool limitIndirect = true;
+  bool limitIndirect = true, geometryLights = true;
   float minContribution = 0.001f, maxRadiance = hpt::kInf, errorThreshold = 0.f;
   float bg[4] = {0, 0, 0, 0};
   int filterType = 2;
@@ -569,7 +586,7 @@ struct Renderer : public ManagedObject
     getVec4f("shadowCatcherPlane", scp, scpd);
     if (scp[0] != 0.f || scp[1] != 0.f || scp[2] != 0.f)
       throw std::runtime_error("pathtracer: shadow catcher is not supported by the hip device");
-    (void)getBool("geometryLights", true);
+    geometryLights = getBool("geometryLights", true);
     (void)getFloat("mipMapBias", 0.f);
     const uint32_t pf = getUInt("pixelFilter", OSP_PIXELFILTER_GAUSS);
     if (pf == OSP_PIXELFILTER_POINT) {
@@ -798,7 +815,7 @@ class HipDevice : public Device
         return new Geometry(t);
       break;
     case OSP_MATERIAL:
-      if (t == "obj" || t == "metal" || t == "glass")
+      if (t == "obj" || t == "metal" || t == "glass" || t == "luminous")
         return new Material(t);
       break;
     case OSP_LIGHT:
@@ -1319,13 +1336,56 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   for (size_t i = 0; i < T.nodes.size(); i++)
     nodes[tlasOff + i] = remap(T.nodes[i], tlasOff, 0, &tlasOrder);
 
-  // lights (PathTracerData.cpp:32-82): world lights, uniform selection CDF
+  // lights (PathTracerData.cpp:32-82): [geometry lights, world lights],
+  // uniform selection CDFs with and without the geometry lights.  Geometry
+  // lights (PathTracerData.cpp:121-162): every emissive model of every
+  // instance, primitives chosen by world-space area; the model's areaPDF is
+  // the one of the last instance that gathers it (GeometricModel.ispc:14-55)
   std::vector<hpt::DLight> dl;
+  std::vector<float> geoCDF;
+  std::map<const GeometricModel *, float> modelAreaPDF;
+  for (size_t i = 0; i < instances.size(); i++) {
+    const int gi = groupIndex[instances[i]->group.p];
+    const float *x = instances[i]->xfm;
+    const hpt::linear3f xl = {{x[0], x[1], x[2]}, {x[3], x[4], x[5]}, {x[6], x[7], x[8]}};
+    const std::vector<GeometricModel *> &gm = groups[gi]->models;
+    for (size_t k = 0; k < gm.size(); k++) {
+      const GeometricModel *m = gm[k];
+      const Geometry &g = *m->geom.p;
+      if (!m->material || !m->material->emissive() || g.numPrims == 0)
+        continue;
+      hpt::DLight L;
+      std::memset(&L, 0, sizeof(L));
+      L.type = hpt::LIGHT_GEOMETRY;
+      L.visible = 1;
+      L.geoInst = (int32_t)i;
+      L.geoModel = groupModelOffset[gi] + (int32_t)k;
+      L.geoNumPrims = (int32_t)g.numPrims;
+      L.geoCdfOffset = (int32_t)geoCDF.size();
+      const size_t off = geoCDF.size();
+      geoCDF.resize(off + g.numPrims);
+      for (size_t p = 0; p < g.numPrims; p++)
+        geoCDF[off + p] = hpt::primArea(g.vertex.data(), g.index.data(), g.spheres.data(), g.geomType, (uint32_t)p, xl);
+      L.geoPdf = 1.f / hpt::distribution1DCreate((int)g.numPrims, geoCDF.data() + off);
+      modelAreaPDF[m] = L.geoPdf;
+      dl.push_back(L);
+    }
+  }
+  scene.numGeoLights = (int)dl.size();
+  for (size_t gi = 0; gi < groups.size(); gi++)
+    for (size_t k = 0; k < groups[gi]->models.size(); k++) {
+      auto it = modelAreaPDF.find(groups[gi]->models[k]);
+      if (it != modelAreaPDF.end())
+        models[groupModelOffset[gi] + k].areaPDF = it->second;
+    }
   for (auto *l : lights)
     dl.push_back(l->l);
   std::vector<float> cdf(dl.size(), 1.f);
   if (!cdf.empty())
     hpt::distribution1DCreate((int)cdf.size(), cdf.data());
+  std::vector<float> worldCdf(lights.size(), 1.f);
+  if (!worldCdf.empty())
+    hpt::distribution1DCreate((int)worldCdf.size(), worldCdf.data());
 
   dNodes.upload(nodes);
   dPrims.upload(prims);
@@ -1335,6 +1395,8 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   dMedia.upload(media);
   dLights.upload(dl);
   dLightsCDF.upload(cdf);
+  dWorldLightsCDF.upload(worldCdf);
+  dGeoCDF.upload(geoCDF);
   scene.nodes = dNodes.as<const hpt::DNode4>();
   scene.prims = dPrims.as<const hpt::DPrim>();
   scene.instances = dInstances.as<const hpt::DInstance>();
@@ -1343,6 +1405,7 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   scene.media = dMedia.as<const hpt::DMedium>();
   scene.lights = dLights.as<const hpt::DLight>();
   scene.lightsCDF = dLightsCDF.as<const float>();
+  scene.geoCDF = dGeoCDF.as<const float>();
   scene.tlasRoot = tlasOff;
   scene.numInstances = (int)instances.size();
   scene.numLights = (int)dl.size();
@@ -1477,8 +1540,17 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   F.maxDepth = r->maxDepth;
   F.rouletteDepth = r->rouletteDepth;
   F.maxScatteringEvents = r->maxScatteringEvents;
+  // the world's scene as this renderer sees it: without the geometry lights
+  // when "geometryLights" is off (PathTracer.cpp:49, PathTracerData.cpp:37-43)
+  hpt::DScene sceneF = w->scene;
+  if (!r->geometryLights && sceneF.numGeoLights > 0) {
+    sceneF.lights += sceneF.numGeoLights;
+    sceneF.numLights -= sceneF.numGeoLights;
+    sceneF.numGeoLights = 0;
+    sceneF.lightsCDF = w->dWorldLightsCDF.as<const float>();
+  }
   // light sample counts (PathTracer.ispc:95-110)
-  const uint32_t numLights = (uint32_t)w->scene.numLights;
+  const uint32_t numLights = (uint32_t)sceneF.numLights;
   F.numFirstLightSamples = r->numLightSamples >= 0 && numLights > 0 ? (uint32_t)r->numLightSamples
                                                                    : std::min<uint32_t>(1024u, numLights);
   F.numIndirectLightSamples = r->limitIndirect && F.numFirstLightSamples >= 1 ? 1u : F.numFirstLightSamples;
@@ -1530,7 +1602,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     paths.aux = aux ? auxBase : nullptr;
     paths.overflow = (uint32_t *)overflow.ptr;
     F.residentThreads = residentTraceBlocks * hpt_trace_block_size();
-    const HptScene *S = reinterpret_cast<const HptScene *>(&w->scene);
+    const HptScene *S = reinterpret_cast<const HptScene *>(&sceneF);
     const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);
     if (travStats)
       HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 16, 0u, stream));
@@ -1548,6 +1620,8 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         if (F.numFirstLightSamples > 0) {
           launch("nee", [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
           launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
+          launch("shadow_trans",
+              [&]() { return hpt_shadow_trans(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
           launch("resolve", [&]() { return hpt_resolve(Fp, P, q, shadeBlocks, stream); });
         }
         q ^= 1;

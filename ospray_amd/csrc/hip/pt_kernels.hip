@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         contribution = contribution + bg3;
       }
       // evaluateVirtualLights (VirtualLight.ispc:38-107)
-      if (S.numLights > 0) {
+      if (S.numLights > S.numGeoLights) {
         const vec3f lastP = v3(lp4.x, lp4.y, lp4.z);
         const vec3f lastNs = v3(ln4.x, ln4.y, ln4.z);
         const float lastPdf = lp4.w;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         const float lower = distance(lastP, rorg);
         const float upper = !hit.hit ? kInf : distance(lastP, rorg + rayT * rdir);
         const float selectionPDF = (float)lastNLS / (float)S.numLights;
-        for (int li = 0; li < S.numLights; li++) {
+        for (int li = S.numGeoLights; li < S.numLights; li++) {
           const DLight &l = S.lights[li];
           if (!straight || l.visible) {
             const LightEval le = lightEval(l, lastP, lastNs, rdir, lower, upper);
@@ -356,10 +356,14 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
       do {
         if (!hit.hit)
           break;
-        if (depth >= (uint32_t)F.maxDepth)
+        // with geometry lights the emission at maxDepth is still evaluated
+        // (PathSampler.ispc:249-255)
+        if (depth >= (uint32_t)F.maxDepth && S.numGeoLights == 0)
           break;
         DG dg;
         postIntersect(S, rorg, rdir, hit, dg, true);
+        contribution = contribution
+            + evaluateGeometryLights(S, dg, rdir, thr, v3(lp4.x, lp4.y, lp4.z), lp4.w, __float_as_uint(ln4.w));
         if (depth == 0) {
           P.ids[p4 + 0] = dg.primID;
           P.ids[p4 + 1] = dg.objID;
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
       const vec2f s = ldGet3LightSamples(lds, li, nls > 1, ss);
       const Sample1D sel = distribution1DSample(S.numLights, S.lightsCDF, 0, ss);
       const float selProb = sel.prob * (float)nls;
-      LightSample ls = lightSample(S.lights[sel.idx], Pv, Ng, Ns, s);
+      LightSample ls = lightSample(S, S.lights[sel.idx], Pv, Ng, Ns, s);
       ls.weight = ls.weight / selProb;
       ls.pdf *= selProb;
       st4(P.recResult + 4 * (size_t)r, 0.f, 0.f, 0.f, 0.f);
@@ -603,7 +607,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
   uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
-  for (int i = globalId(); i < n; i += gridStride()) {
+  for (int base = nextBatch(&P.counters[kCntFetchTrans]); base < n; base = nextBatch(&P.counters[kCntFetchTrans])) {
+    const int i = base + laneId();
+    if (i >= n)
+      continue;
     const uint32_t r = P.transQueue[i];
     const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
     const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
@@ -996,11 +1003,22 @@ hipError_t hpt_shadow(
   const DPaths &p = *reinterpret_cast<const DPaths *>(P);
   if (stats) {
     hipLaunchKernelGGL(k_shadow<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
-    hipLaunchKernelGGL(k_shadow_trans<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
   } else {
     hipLaunchKernelGGL(k_shadow<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
-    hipLaunchKernelGGL(k_shadow_trans<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
   }
+  return hipGetLastError();
+}
+
+hipError_t hpt_shadow_trans(
+    const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s)
+{
+  const DScene &sc = *reinterpret_cast<const DScene *>(S);
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  if (stats)
+    hipLaunchKernelGGL(k_shadow_trans<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
+  else
+    hipLaunchKernelGGL(k_shadow_trans<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, f, p);
   return hipGetLastError();
 }
 

@@ -31,6 +31,7 @@ struct DG
   vec3f P, Ng, Ns, dPds, dPdt;
   float colorW;
   float epsilon;
+  float areaPDF; // of the model as a geometry light (GeometricModel.ih:58)
   int material;
   uint32_t primID, objID, instID;
 };
@@ -78,6 +79,7 @@ HPT_HD void postIntersect(const DScene &S, vec3f rorg, vec3f rdir, const Hit &hi
   const DInstance &I = S.instances[hit.inst];
   const DModel &M = S.models[I.modelOffset + hit.model];
   dg.primID = hit.prim;
+  dg.areaPDF = M.areaPDF;
   dg.material = -1;
   dg.colorW = 1.f;
   dg.dPds = v3(1.f, 0.f, 0.f);
@@ -311,6 +313,15 @@ HPT_HD bool getBSDF(const DScene &S, const DG &dg, const DMedium &medium, BSDF &
   if (dg.material < 0)
     return false;
   const DMaterial &m = S.materials[dg.material];
+  if (m.type == MAT_LUMINOUS) { // Luminous.ispc:15-32: Transmission, or no BSDF
+    if (!(m.transparency > 0.0f))
+      return false;
+    b.kind = 5;
+    b.albedo = v3(m.transparency);
+    b.scatteringType = SPEC_TRANS;
+    b.frame = frame(dg.Ns);
+    return true;
+  }
   if (m.type == MAT_OBJ) { // OBJ.ispc:29-98
     b.kind = 1;
     b.albedo = v3(0.f);
@@ -413,6 +424,14 @@ HPT_HD void getEvalBSDF(const DScene &S, int material, vec3f Ns, BSDF &b)
 // BSDF_dispatch_sample (bsdfs/BSDF.ispc:61-112)
 HPT_HD SampleRes bsdfSample(const BSDF &b, vec3f wo, vec2f s, float ss)
 {
+  if (b.kind == 5) { // Transmission_sample (bsdfs/Transmission.ih:11-20)
+    SampleRes res;
+    res.wi = neg(wo);
+    res.pdf = kInf;
+    res.type = SPEC_TRANS;
+    res.weight = b.albedo;
+    return res;
+  }
   SampleRes zero;
   zero.weight = v3(0.f);
   zero.wi = v3(0.f);
@@ -586,6 +605,8 @@ HPT_HD vec3f getTransparency(const DScene &S, const DG &dg, vec3f rdir, const DM
     const float d = m.d * dg.colorW;
     return ld3(m.Tf) * d + v3(1.f - d);
   }
+  if (m.type == MAT_LUMINOUS) // Luminous.ispc:34-44
+    return v3(m.transparency);
   if (m.type == MAT_GLASS) { // Glass.ispc:37-52
     const DMedium &inside = S.media[m.inside];
     const DMedium &outside = S.media[m.outside];
@@ -745,9 +766,131 @@ HPT_HD vec3f cosineWeightedRNG(vec3f q0, vec3f e0, vec3f e1, vec3f P, vec3f Ng, 
 
 // Light_dispatch_sample: QuadLight.ispc:82-136, DirectionalLight.ispc:14-27,
 // AmbientLight.ispc:17-38
-HPT_HD LightSample lightSample(const DLight &l, vec3f P, vec3f Ng, vec3f Ns, vec2f s)
+HPT_HD vec2f uniformSampleTriangleUV(vec2f s) // math/sampling.ih:165-173
+{
+  const float su = sqrtf(s.x);
+  return v2(s.y * su, (1.0f - s.y) * su);
+}
+
+// the vertices a, b, c, d of a mesh primitive as Mesh_sampleArea reads them
+// (a triangle repeats its last vertex: index (x, y, z, z))
+HPT_HD bool meshPrim(const float *vertex, const uint32_t *index, int geomType, uint32_t p, vec3f &a, vec3f &b,
+    vec3f &c, vec3f &d)
+{
+  uint32_t i0, i1, i2, i3;
+  bool quad = false;
+  if (geomType == GEOM_TRIANGLES) {
+    const uint32_t *q = index + 3 * (size_t)p;
+    i0 = q[0], i1 = q[1], i2 = q[2], i3 = q[2];
+  } else {
+    const uint32_t *q = index + 4 * (size_t)p;
+    i0 = q[0], i1 = q[1], i2 = q[2], i3 = q[3];
+    quad = i2 != i3;
+  }
+  a = ld3(vertex + 3 * (size_t)i0);
+  b = ld3(vertex + 3 * (size_t)i1);
+  d = ld3(vertex + 3 * (size_t)i3);
+  c = quad ? ld3(vertex + 3 * (size_t)i2) : d;
+  return quad;
+}
+
+// world-space area of a primitive (Mesh_getAreas, geometry/Mesh.ispc:377-415;
+// Spheres_getAreas, geometry/Spheres.ispc:76-103); xfm: columns vx, vy, vz
+HPT_HD float primArea(
+    const float *vertex, const uint32_t *index, const float *spheres, int geomType, uint32_t p, const linear3f &xl)
+{
+  if (geomType == GEOM_SPHERES) {
+    const vec3f sc = v3(length(xl.vx), length(xl.vy), length(xl.vz));
+    const float scale = 4.0f * kPi * sqr((sc.x + sc.y + sc.z) / 3.f);
+    return scale * sqr(spheres[4 * (size_t)p + 3]);
+  }
+  vec3f a, b, c, d;
+  const bool quad = meshPrim(vertex, index, geomType, p, a, b, c, d);
+  const vec3f e1 = mul(xl, a - d);
+  const vec3f e2 = mul(xl, b - d);
+  float area = length(cross(e1, e2));
+  if (quad) {
+    const vec3f e3 = mul(xl, c - d);
+    area += length(cross(e2, e3));
+  }
+  return area * 0.5f;
+}
+
+struct SampleAreaRes
+{
+  vec3f pos, normal;
+  float epsilon;
+};
+
+// Mesh_sampleArea (geometry/Mesh.ispc:270-351), Spheres_sampleArea
+// (geometry/Spheres.ispc:44-74) under instance I
+HPT_HD SampleAreaRes sampleArea(const DModel &M, const DInstance &I, uint32_t p, vec2f s)
+{
+  SampleAreaRes res;
+  const linear3f xl = {{I.xfm[0], I.xfm[1], I.xfm[2]}, {I.xfm[3], I.xfm[4], I.xfm[5]}, {I.xfm[6], I.xfm[7], I.xfm[8]}};
+  const vec3f xp = v3(I.xfm[9], I.xfm[10], I.xfm[11]);
+  const float xfmScale = maxf(fabsf(xl.vx.x), maxf(fabsf(xl.vy.y), fabsf(xl.vz.z)));
+  if (M.geomType == GEOM_SPHERES) {
+    const float *sp = M.spheres + 4 * (size_t)p;
+    const vec3f localNormal = uniformSampleSphere(sp[3], s);
+    const linear3f rl = {{I.rcp[0], I.rcp[1], I.rcp[2]}, {I.rcp[3], I.rcp[4], I.rcp[5]}, {I.rcp[6], I.rcp[7], I.rcp[8]}};
+    res.pos = mul(xl, v3(sp[0], sp[1], sp[2]) + localNormal) + xp;
+    res.normal = normalize(mulT(rl, localNormal)); // transposed(rcp_xfm.l) * n
+    res.epsilon = sp[3] * kUlpEpsilon;
+    res.epsilon *= xfmScale;
+    return res;
+  }
+  vec3f a, b, c, d;
+  const bool quad = meshPrim(M.vertex, M.index, M.geomType, p, a, b, c, d);
+  res.epsilon = reduce_max(vmax(vmax(vabs(a), vabs(b)), vmax(vabs(c), vabs(d)))) * kUlpEpsilon;
+  res.epsilon *= xfmScale;
+  const vec3f e1 = mul(xl, a - d);
+  const vec3f e2 = mul(xl, b - d);
+  const vec3f m1 = cross(e1, e2);
+  res.normal = m1;
+  vec3f v0 = a, v1 = b, v2 = d;
+  vec2f sp = s;
+  if (quad) {
+    const vec3f e3 = mul(xl, c - d);
+    const vec3f m2 = cross(e2, e3);
+    const float a1 = length(m1);
+    const float a2 = length(m2);
+    const float p1 = a1 * rcp(a1 + a2);
+    if (s.x < p1) {
+      sp.x = sp.x * rcp(p1);
+    } else {
+      sp.x = (s.x - p1) * rcp(1.f - p1);
+      v0 = c;
+      v1 = d;
+      v2 = b;
+      res.normal = m2;
+    }
+  }
+  const vec2f uv = uniformSampleTriangleUV(sp);
+  const vec3f localPos = v0 + uv.x * (v1 - v0) + uv.y * (v2 - v0);
+  res.normal = normalize(res.normal);
+  res.pos = mul(xl, localPos) + xp;
+  return res;
+}
+
+HPT_HD LightSample lightSample(const DScene &S, const DLight &l, vec3f P, vec3f Ng, vec3f Ns, vec2f s)
 {
   LightSample res;
+  if (l.type == LIGHT_GEOMETRY) { // GeometryLight_sample (render/pathtracer/GeometryLight.ispc:60-123)
+    const DModel &M = S.models[l.geoModel];
+    const Sample1D smp = distribution1DSample(l.geoNumPrims, S.geoCDF + l.geoCdfOffset, 0, s.x);
+    const SampleAreaRes as = sampleArea(M, S.instances[l.geoInst], (uint32_t)smp.idx, v2(smp.frac, s.y));
+    const vec3f dir = as.pos - P;
+    const float dist = length(dir);
+    res.dir = dir / dist;
+    const float rcosd = rcp(fabsf(dot(as.normal, res.dir)));
+    res.pdf = l.geoPdf * sqr(dist) * rcosd;
+    // calcEpsilon(P, t) (common/DifferentialGeometry.ih:48-51)
+    const float epsP = maxf(reduce_max(vabs(as.pos)), dist) * kUlpEpsilon;
+    res.dist = dist - maxf(as.epsilon, epsP) * rcosd;
+    res.weight = ld3(S.materials[M.material].radiance) * rcp(res.pdf);
+    return res;
+  }
   if (l.type == LIGHT_QUAD) {
     const vec3f position = ld3(l.position), edge1 = ld3(l.edge1), edge2 = ld3(l.edge2), nn = ld3(l.nnormal);
     if (s.x == 0.0f && s.y == 0.0f) {
@@ -845,6 +988,29 @@ HPT_HD float misHeuristic(float pdf1, float pdf2) // PathTracerUtil.ih:30-46
 {
   const float p = sqr(pdf1) * rcp(sqr(pdf1) + sqr(pdf2));
   return pdf1 > 1e17f ? 1.0f : p;
+}
+
+// evaluateGeometryLights (render/pathtracer/GeometryLight.ispc:22-56): the
+// emission of a hit luminous surface, MIS-weighted against NEE; lastP,
+// lastPdf, lastNLS describe the previous vertex
+HPT_HD vec3f evaluateGeometryLights(const DScene &S, const DG &dg, vec3f rdir, vec3f thr, vec3f lastP, float lastPdf,
+    uint32_t lastNLS)
+{
+  if (dg.material < 0)
+    return v3(0.f);
+  const DMaterial &m = S.materials[dg.material];
+  const vec3f radiance = ld3(m.radiance);
+  if (m.type != MAT_LUMINOUS || !(reduce_max(radiance) > 0.f))
+    return v3(0.f);
+  float misWeight = 1.0f;
+  if (S.numGeoLights > 0 && lastNLS > 0) {
+    const float lightSelectionPDF = (float)lastNLS / (float)S.numLights;
+    const float cosd = dot(dg.Ng, rdir);
+    const vec3f vdis = lastP - dg.P;
+    const float lePdf = dg.areaPDF * dot(vdis, vdis) / fabsf(cosd);
+    misWeight = misHeuristic(lastPdf, lightSelectionPDF * lePdf);
+  }
+  return thr * radiance * misWeight;
 }
 
 } // namespace hpt

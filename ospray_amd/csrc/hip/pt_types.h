@@ -83,14 +83,16 @@ struct DModel
   uint32_t userID;
   int32_t invertNormals;
   int32_t opaque; // material transparency is identically zero
-  int32_t pad[3];
+  float areaPDF; // as a geometry light (GeometricModel.ispc:51), else 0
+  int32_t pad[2];
 };
 
 enum MaterialType : int32_t
 {
   MAT_OBJ = 0,
   MAT_METAL = 1,
-  MAT_GLASS = 2
+  MAT_GLASS = 2,
+  MAT_LUMINOUS = 3
 };
 
 struct DMaterial
@@ -99,6 +101,7 @@ struct DMaterial
   float Kd[3], Tf[3], d; // obj
   float eta[3], k[3], roughness; // metal
   int32_t inside, outside; // glass: indices into the media table
+  float radiance[3], transparency; // luminous
 };
 
 struct DMedium
@@ -111,7 +114,8 @@ enum LightType : int32_t
 {
   LIGHT_QUAD = 0,
   LIGHT_DISTANT = 1,
-  LIGHT_AMBIENT = 2
+  LIGHT_AMBIENT = 2,
+  LIGHT_GEOMETRY = 3 // an emissive model of one instance
 };
 
 struct DLight
@@ -124,6 +128,10 @@ struct DLight
   float frame[9]; // distant: vx, vy, vz
   float irradiance[3];
   float cosAngle, pdf;
+  // geometry light: instance, model (DModel index), primitive CDF by area
+  // (DScene::geoCDF + geoCdfOffset), 1 / total area
+  int32_t geoInst, geoModel, geoNumPrims, geoCdfOffset;
+  float geoPdf;
 };
 
 struct DScene
@@ -134,11 +142,14 @@ struct DScene
   const DModel *models;
   const DMaterial *materials;
   const DMedium *media;
+  // [geometry lights, world lights] and their uniform selection CDF
   const DLight *lights;
   const float *lightsCDF;
+  const float *geoCDF; // primitive CDFs of the geometry lights
   int32_t tlasRoot;
   int32_t numInstances;
   int32_t numLights;
+  int32_t numGeoLights;
   int32_t numMaterials;
 };
 
@@ -189,7 +200,8 @@ constexpr int kCntShadow = 4; // shadow-ray records
 constexpr int kCntFetchExtend = 5; // work fetch of the extension traversal
 constexpr int kCntFetchShadow = 6; // work fetch of the shadow traversal
 constexpr int kCntTrans = 7; // records for the transparency pass
-constexpr int kNumCounters = 8;
+constexpr int kCntFetchTrans = 8; // work fetch of the transparency pass
+constexpr int kNumCounters = 9;
 
 // path flags in ld[3]
 constexpr uint32_t FLAG_DEPTH_MASK = 0xffu; // bits 0-7

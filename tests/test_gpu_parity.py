@@ -150,6 +150,35 @@ def test_adaptive_accumulation(hip_device):
     assert K.lib().ospGetVariance(fb.h) == pytest.approx(ofb.variance(oren), rel=1e-6)
 
 
+def _c0_with_emitters(w=48, h=48, spp=2, max_path_length=3):
+    """C0 with three luminous icospheres (one semi-transparent), a luminous
+    sphere primitive, and a second, scaled instance of the whole group: six
+    geometry lights before the quad and ambient lights; a model instanced
+    twice takes the area pdf of its last instance."""
+    sc = scenes.c0(w, h, spp=spp, max_path_length=max_path_length)
+    for k, t in ((3, 0.0), (17, 0.3), (40, 0.0)):
+        sc.materials[sc.models[k].material] = {"type": "luminous", "intensity": 3.0 + k * 0.1,
+                                               "color": (1.0, 0.8, 0.6), "transparency": t}
+    sc.geometries.append(scenes.Geometry("spheres", spheres=np.array([[2.0, 5.0, -3.0, 0.7]], np.float32)))
+    sc.materials.append({"type": "luminous", "intensity": 2.0})
+    sc.models.append(scenes.Model(geometry=len(sc.geometries) - 1, material=len(sc.materials) - 1))
+    sc.groups.append([len(sc.models) - 1])
+    sc.instances.append(scenes.Instance(1))
+    sc.instances.append(scenes.Instance(0, xfm=(0.5, 0, 0, 0, 0.5, 0, 0, 0, 0.5, 6.0, 1.0, 4.0)))
+    return sc
+
+
+@pytest.mark.parametrize("geometry_lights", [True, False])
+def test_geometry_lights_bit_exact(hip_device, geometry_lights):
+    sc = _c0_with_emitters()
+    rp = {"geometryLights": geometry_lights}
+    s, fb, ofb, _, oren = _render_both(sc, frames=3, renderer_params=rp)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    assert col[..., :3].max() > 1.0  # emitters seen directly
+
+
 def test_pick(hip_device):
     sc = scenes.c0(64, 64)
     s = scenes.to_ospray(sc)
