@@ -193,7 +193,7 @@ def main() -> int:
     log(f"traversal counters: {tr}")
 
     kern = {}
-    for k, key in (("extend", 0), ("shadow", 3)):
+    for k, key in (("extend", 0), ("shadow", 3), ("shadow_trans", 6)):
         nodes, prims, rays = tr[key:key + 3]
         launches, ms = kstats.get(k, (0, 0.0))
         if not launches:

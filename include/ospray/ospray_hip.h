@@ -28,9 +28,10 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void);
 
 /* Traversal counters of the last frame rendered while the device parameter
  * "traversalStats" (bool) was set: node visits, primitive tests and
- * traversals of the extension rays (out6[0..2]) and of the shadow rays
- * (out6[3..5]). */
-OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out6[6]);
+ * traversals of the extension rays (out9[0..2]), of the shadow-ray
+ * occlusion pass (out9[3..5]) and of the shadow-ray transparency pass
+ * (out9[6..8]). */
+OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9]);
 
 /* Closest hits of n rays (org.xyz, dir.xyz, t0, t1) against a committed
  * world, traced by the device's BVH kernels.  outTUV: t,u,v; outIDs:

@@ -289,9 +289,10 @@ def reset_kernel_stats() -> None:
 
 
 def traversal_stats() -> tuple:
-    """(node visits, primitive tests, traversals) of the extension rays, then
-    of the shadow rays, of the last frame rendered with "traversalStats"."""
-    out = (C.c_uint64 * 6)()
+    """(node visits, primitive tests, traversals) of the extension rays, of
+    the shadow-ray occlusion pass and of the shadow-ray transparency pass, of
+    the last frame rendered with "traversalStats"."""
+    out = (C.c_uint64 * 9)()
     K.ext().ospHipGetTraversalStats(out)
     return tuple(int(x) for x in out)
 

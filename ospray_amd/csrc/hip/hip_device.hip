@@ -737,8 +737,8 @@ class HipDevice : public Device
       gpuName = prop.gcnArchName;
       HIP_CHECK(hipMalloc(&counters.ptr, 64));
       counters.bytes = 64;
-      stats.ensure(64);
-      HIP_CHECK(hipMemset(stats.ptr, 0, 64));
+      stats.ensure(128);
+      HIP_CHECK(hipMemset(stats.ptr, 0, 128));
       sizeGrids();
     }
   }
@@ -1605,7 +1605,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     const HptScene *S = reinterpret_cast<const HptScene *>(&sceneF);
     const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);
     if (travStats)
-      HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 16, 0u, stream));
+      HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 32, 0u, stream));
     for (int s0 = 0; s0 < F.spp; s0 += sppPerPass) {
       F.passS0 = s0;
       F.passCount = std::min(sppPerPass, F.spp - s0);
@@ -1663,7 +1663,7 @@ int HipDevice::traceRays(World *w, const float *rays8, int n, float *outTUV, uin
   dt.ensure((size_t)std::max(n, 1) * 12);
   di.ensure((size_t)std::max(n, 1) * 12);
   HIP_CHECK(hipMemcpy(dr.ptr, rays8, (size_t)n * 32, hipMemcpyHostToDevice));
-  HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 16, 0u, stream));
+  HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 32, 0u, stream));
   HIP_CHECK(hpt_trace_rays(reinterpret_cast<const HptScene *>(&w->scene), dr.as<float>(), n, dt.as<float>(),
       di.as<uint32_t>(), (uint32_t *)overflow.ptr, residentTraceBlocks, (unsigned long long *)stats.ptr, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
@@ -1762,7 +1762,7 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void)
   return 0;
 }
 
-OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out6[6])
+OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9])
 {
   HipDevice *d = current();
   if (!d)
@@ -1770,11 +1770,11 @@ OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out6[6])
   if (!d->stream)
     return -2;
   try {
-    unsigned long long s[6];
+    unsigned long long s[9];
     HIP_CHECK(hipStreamSynchronize(d->stream));
     HIP_CHECK(hipMemcpy(s, d->stats.ptr, sizeof(s), hipMemcpyDeviceToHost));
-    for (int k = 0; k < 6; k++)
-      out6[k] = s[k];
+    for (int k = 0; k < 9; k++)
+      out9[k] = s[k];
   } catch (const std::exception &) {
     return -2;
   }

@@ -650,9 +650,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
     }
   }
   if (STATS) {
-    waveAdd(&P.stats[3], st.nodes);
-    waveAdd(&P.stats[4], st.prims);
-    waveAdd(&P.stats[5], rays);
+    waveAdd(&P.stats[6], st.nodes);
+    waveAdd(&P.stats[7], st.prims);
+    waveAdd(&P.stats[8], rays);
   }
 }
 

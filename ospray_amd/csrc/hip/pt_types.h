@@ -191,7 +191,9 @@ struct DPaths
   // with a single atomic
   int32_t *counters;
   uint32_t *overflow; // traversal stack overflow, per resident thread
-  unsigned long long *stats; // node visits, primitive tests, traversals: [0..2] extension, [3..5] shadow
+  // node visits, primitive tests, traversals: [0..2] extension rays, [3..5]
+  // shadow occlusion pass, [6..8] shadow transparency pass
+  unsigned long long *stats;
 };
 
 // queue counters: [2q] entries of extension queue q (q = 0, 1), [2q+1] NEE
