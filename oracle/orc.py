@@ -31,7 +31,15 @@ class OrcModel(C.Structure):
 
 
 class OrcGroup(C.Structure):
-    _fields_ = [("firstModel", C.c_int32), ("numModels", C.c_int32)]
+    _fields_ = [("firstModel", C.c_int32), ("numModels", C.c_int32), ("firstVolume", C.c_int32),
+                ("numVolumes", C.c_int32)]
+
+
+class OrcVolume(C.Structure):
+    _fields_ = [("voxels", C.c_void_p), ("dims", C.c_int64 * 3), ("gridOrigin", C.c_float * 3),
+                ("gridSpacing", C.c_float * 3), ("tfColor", C.c_void_p), ("numColors", C.c_int32),
+                ("tfOpacity", C.c_void_p), ("numOpacities", C.c_int32), ("tfValueRange", C.c_float * 2),
+                ("densityScale", C.c_float), ("anisotropy", C.c_float), ("userID", C.c_uint32)]
 
 
 class OrcInstance(C.Structure):
@@ -58,7 +66,9 @@ class OrcScene(C.Structure):
                 ("groupModels", C.POINTER(C.c_int32)), ("groups", C.POINTER(OrcGroup)), ("numGroups", C.c_int32),
                 ("instances", C.POINTER(OrcInstance)), ("numInstances", C.c_int32),
                 ("materials", C.POINTER(OrcMaterial)), ("numMaterials", C.c_int32),
-                ("lights", C.POINTER(OrcLight)), ("numLights", C.c_int32)]
+                ("lights", C.POINTER(OrcLight)), ("numLights", C.c_int32),
+                ("volumes", C.POINTER(OrcVolume)), ("numVolumes", C.c_int32),
+                ("groupVolumes", C.POINTER(C.c_int32))]
 
 
 class OrcCamera(C.Structure):
@@ -111,6 +121,16 @@ def lib():
         L.orc_camera_rays.argtypes = [C.POINTER(OrcCamera), C.c_void_p, C.c_void_p, C.c_int64]
         L.orc_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
                                     C.POINTER(C.c_uint64)]
+        L.orc_volume_sample.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_tf_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_tf_max_opacity.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_volume_intervals.restype = C.c_int
+        L.orc_volume_intervals.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_volume_estimates.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_uint32, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
+        L.orc_hg_sample.argtypes = [C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int64]
+        L.orc_hg_eval.argtypes = [C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         _lib = L
     return _lib
 
@@ -187,12 +207,37 @@ class OracleScene:
             models[i].geometry = m.geometry
             models[i].material = m.material
             models[i].userID = 0xFFFFFFFF if m.user_id is None else m.user_id
-        gm, groups = [], (OrcGroup * len(scene.groups))()
+        gm, gv, groups = [], [], (OrcGroup * len(scene.groups))()
+        group_volumes = getattr(scene, "group_volumes", None) or []
         for i, grp in enumerate(scene.groups):
             groups[i].firstModel = len(gm)
             groups[i].numModels = len(grp)
             gm += list(grp)
+            vols = group_volumes[i] if i < len(group_volumes) else []
+            groups[i].firstVolume = len(gv)
+            groups[i].numVolumes = len(vols)
+            gv += list(vols)
         gmodels = (C.c_int32 * max(1, len(gm)))(*gm)
+        gvols = (C.c_int32 * max(1, len(gv)))(*gv)
+        volumes = getattr(scene, "volumes", None) or []
+        vols = (OrcVolume * max(1, len(volumes)))()
+        for i, v in enumerate(volumes):
+            data = np.ascontiguousarray(v.data, np.float32)  # (nz, ny, nx)
+            col = np.ascontiguousarray(v.tf_color, np.float32).reshape(-1, 3)
+            op = np.ascontiguousarray(v.tf_opacity, np.float32).reshape(-1)
+            self._keep += [data, col, op]
+            vols[i].voxels = data.ctypes.data
+            vols[i].dims = (C.c_int64 * 3)(data.shape[2], data.shape[1], data.shape[0])
+            vols[i].gridOrigin = _f3(v.origin)
+            vols[i].gridSpacing = _f3(v.spacing, (1, 1, 1))
+            vols[i].tfColor = col.ctypes.data
+            vols[i].numColors = col.shape[0]
+            vols[i].tfOpacity = op.ctypes.data
+            vols[i].numOpacities = op.shape[0]
+            vols[i].tfValueRange = (C.c_float * 2)(*[float(np.float32(x)) for x in v.tf_range])
+            vols[i].densityScale = v.density_scale
+            vols[i].anisotropy = v.anisotropy
+            vols[i].userID = 0xFFFFFFFF if v.user_id is None else v.user_id
         insts = (OrcInstance * len(scene.instances))()
         for i, inst in enumerate(scene.instances):
             insts[i].group = inst.group
@@ -231,9 +276,10 @@ class OracleScene:
             lights[i].edge2 = _f3(l.get("edge2"), (0, 1, 0))
             lights[i].direction = _f3(l.get("direction"), (0, 0, 1))
             lights[i].angularDiameter = l.get("angularDiameter", 0.0)
-        self._keep += [geos, models, groups, gmodels, insts, mats, lights]
+        self._keep += [geos, models, groups, gmodels, insts, mats, lights, gvols, vols]
         desc = OrcScene(geos, len(scene.geometries), models, len(scene.models), gmodels, groups, len(scene.groups),
-                        insts, len(scene.instances), mats, len(scene.materials), lights, len(scene.lights))
+                        insts, len(scene.instances), mats, len(scene.materials), lights, len(scene.lights),
+                        vols, len(volumes), gvols)
         self._desc = desc
         self.h = L.orc_scene_create(C.byref(desc))
 
@@ -251,6 +297,64 @@ class OracleScene:
         st = (C.c_uint64 * 2)()
         L.orc_intersect(self.h, r.ctypes.data, n, int(fp64), tuv.ctypes.data, ids.ctypes.data, st)
         return tuv, ids, (int(st[0]), int(st[1]))
+
+
+    # ---- volume hooks ----
+    def volume_sample(self, vol: int, pts: np.ndarray) -> np.ndarray:
+        p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.empty(p.shape[0], np.float32)
+        lib().orc_volume_sample(self.h, vol, p.ctypes.data, out.ctypes.data, p.shape[0])
+        return out
+
+    def tf_get(self, vol: int, values) -> np.ndarray:
+        v = np.ascontiguousarray(values, np.float32).reshape(-1)
+        out = np.empty((v.size, 4), np.float32)
+        lib().orc_tf_get(self.h, vol, v.ctypes.data, out.ctypes.data, v.size)
+        return out
+
+    def tf_max_opacity(self, vol: int, ranges) -> np.ndarray:
+        r = np.ascontiguousarray(ranges, np.float32).reshape(-1, 2)
+        out = np.empty(r.shape[0], np.float32)
+        lib().orc_tf_max_opacity(self.h, vol, r.ctypes.data, out.ctypes.data, r.shape[0])
+        return out
+
+    def volume_intervals(self, vol: int, ray8) -> np.ndarray:
+        r = np.ascontiguousarray(ray8, np.float32).reshape(8)
+        out = np.empty((4096, 4), np.float32)
+        n = lib().orc_volume_intervals(self.h, vol, r.ctypes.data, out.ctypes.data, 4096)
+        assert n <= 4096
+        return out[:n].copy()
+
+    def volume_estimates(self, rays8, trials: int, seed: int = 7):
+        """Per world ray: (mean transmittance, fraction of free-path
+        collisions, mean collision distance) over `trials` PCG streams."""
+        r = np.ascontiguousarray(rays8, np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        T, hf, md = (np.empty(n, np.float32) for _ in range(3))
+        lib().orc_volume_estimates(self.h, r.ctypes.data, n, trials, seed, T.ctypes.data, hf.ctypes.data,
+                                   md.ctypes.data)
+        return T, hf, md
+
+
+def hg_sample(g: float, wo, s2):
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    s2 = np.ascontiguousarray(s2, np.float32).reshape(-1, 2)
+    n = s2.shape[0]
+    wo = np.broadcast_to(wo, (n, 3)).copy()
+    wi = np.empty((n, 3), np.float32)
+    pdf = np.empty(n, np.float32)
+    typ = np.empty(n, np.uint32)
+    lib().orc_hg_sample(g, wo.ctypes.data, s2.ctypes.data, wi.ctypes.data, pdf.ctypes.data, typ.ctypes.data, n)
+    return wi, pdf, typ
+
+
+def hg_eval(g: float, wo, wi):
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    n = wi.shape[0]
+    wo = np.broadcast_to(np.ascontiguousarray(wo, np.float32).reshape(-1, 3), (n, 3)).copy()
+    out = np.empty(n, np.float32)
+    lib().orc_hg_eval(g, wo.ctypes.data, wi.ctypes.data, out.ctypes.data, n)
+    return out
 
 
 class OracleFrameBuffer:

@@ -228,4 +228,106 @@ void orc_intersect(void *scene, const float *rays8, int64_t n, int fp64, float *
   }
 }
 
+
+// ---- volume hooks ----------------------------------------------------------------
+
+// trilinear samples of volume `vol` at volume-space points (NaN outside)
+void orc_volume_sample(void *scene, int vol, const float *p3, float *out, int64_t n)
+{
+  const VolumeData &v = ((OrcSceneHandle *)scene)->scene.volumes[vol];
+  for (int64_t i = 0; i < n; i++)
+    out[i] = sampleVolume(v, v3(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]));
+}
+
+// transfer function: rgba per value; max opacity per value range
+void orc_tf_get(void *scene, int vol, const float *values, float *out4, int64_t n)
+{
+  const VolumeData &v = ((OrcSceneHandle *)scene)->scene.volumes[vol];
+  for (int64_t i = 0; i < n; i++) {
+    const vec4f c = tfGet(v, values[i]);
+    out4[4 * i] = c.x;
+    out4[4 * i + 1] = c.y;
+    out4[4 * i + 2] = c.z;
+    out4[4 * i + 3] = c.w;
+  }
+}
+
+void orc_tf_max_opacity(void *scene, int vol, const float *ranges2, float *out, int64_t n)
+{
+  const VolumeData &v = ((OrcSceneHandle *)scene)->scene.volumes[vol];
+  for (int64_t i = 0; i < n; i++)
+    out[i] = tfMaxOpacity(v, ranges2[2 * i], ranges2[2 * i + 1]);
+}
+
+// macrocell intervals of a volume-space ray (org, dir, t0, t1) clipped to the
+// box: up to maxn rows (ta, tb, valueLo, valueHi); returns the count
+int orc_volume_intervals(void *scene, int vol, const float *ray8, float *out4, int maxn)
+{
+  const VolumeData &v = ((OrcSceneHandle *)scene)->scene.volumes[vol];
+  const vec3f o = v3(ray8[0], ray8[1], ray8[2]), d = v3(ray8[3], ray8[4], ray8[5]);
+  float lo, hi;
+  if (!boxInterval(v, o, d, ray8[6], ray8[7], lo, hi))
+    return 0;
+  int n = 0;
+  forEachInterval(v, o, d, lo, hi, [&](float ta, float tb, float vlo, float vhi) {
+    if (n < maxn) {
+      out4[4 * n] = ta;
+      out4[4 * n + 1] = tb;
+      out4[4 * n + 2] = vlo;
+      out4[4 * n + 3] = vhi;
+    }
+    n++;
+    return true;
+  });
+  return n;
+}
+
+// per world ray (org, dir, t0, t1), `trials` independent estimates with the
+// PCG stream (seed, i * trials + k): mean transmittance and mean free path
+// (inf counted as t1)
+void orc_volume_estimates(
+    void *scene, const float *rays8, int64_t n, int trials, uint32_t seed, float *outT, float *outHitFrac, float *outMeanDist)
+{
+  const Scene &s = ((OrcSceneHandle *)scene)->scene;
+  for (int64_t i = 0; i < n; i++) {
+    const float *r = rays8 + 8 * i;
+    Ray ray{v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7]};
+    double sumT = 0.0, hits = 0.0, sumD = 0.0;
+    for (int k = 0; k < trials; k++) {
+      RandomSampler rs;
+      RandomSampler_init(rs, seed, (uint32_t)(i * trials + k));
+      sumT += volumeTransmittanceHook(s, ray, ray.t0, ray.t, rs);
+      RandomSampler_init(rs, seed ^ 0x9e3779b9u, (uint32_t)(i * trials + k));
+      Ray fr = ray;
+      int vi;
+      vec3f alb;
+      const float d = volumeFreePathHook(s, fr, ray.t0, ray.t, rs, vi, alb);
+      if (d < kInf) {
+        hits += 1.0;
+        sumD += d;
+      }
+    }
+    outT[i] = (float)(sumT / trials);
+    outHitFrac[i] = (float)(hits / trials);
+    outMeanDist[i] = hits > 0 ? (float)(sumD / hits) : 0.f;
+  }
+}
+
+// Henyey-Greenstein: sample (wi, pdf, type) for (g, wo, s) rows; eval pdf
+void orc_hg_sample(float g, const float *wo3, const float *s2, float *outWi3, float *outPdf, uint32_t *outType, int64_t n)
+{
+  for (int64_t i = 0; i < n; i++) {
+    vec3f wi;
+    hgSampleHook(g, v3(wo3[3 * i], wo3[3 * i + 1], wo3[3 * i + 2]), v2(s2[2 * i], s2[2 * i + 1]), wi, outPdf[i], outType[i]);
+    outWi3[3 * i] = wi.x;
+    outWi3[3 * i + 1] = wi.y;
+    outWi3[3 * i + 2] = wi.z;
+  }
+}
+
+void orc_hg_eval(float g, const float *wo3, const float *wi3, float *outPdf, int64_t n)
+{
+  for (int64_t i = 0; i < n; i++)
+    outPdf[i] = hgEvalHook(g, v3(wo3[3 * i], wo3[3 * i + 1], wo3[3 * i + 2]), v3(wi3[3 * i], wi3[3 * i + 1], wi3[3 * i + 2]));
+}
 } // extern "C"

@@ -294,6 +294,37 @@ void setupScene(Scene &s, const OrcScene &d)
   s.lightsCDF.assign(s.lights.size(), 1.f);
   if (!s.lights.empty())
     Distribution1D_create((int)s.lights.size(), s.lightsCDF.data());
+  // volumetric models (VolumetricModel.cpp:31-82, TransferFunction.cpp:19,
+  // LinearTransferFunction.cpp:18-31)
+  s.volumes.resize(d.numVolumes);
+  for (int i = 0; i < d.numVolumes; i++) {
+    const OrcVolume &ov = d.volumes[i];
+    VolumeData &v = s.volumes[i];
+    for (int a = 0; a < 3; a++)
+      v.dims[a] = ov.dims[a];
+    v.voxels = ov.voxels;
+    v.origin = v3(ov.gridOrigin[0], ov.gridOrigin[1], ov.gridOrigin[2]);
+    v.spacing = v3(ov.gridSpacing[0], ov.gridSpacing[1], ov.gridSpacing[2]);
+    v.upper = v.origin + v3((float)(ov.dims[0] - 1), (float)(ov.dims[1] - 1), (float)(ov.dims[2] - 1)) * v.spacing;
+    v.color.resize(ov.numColors);
+    for (int k = 0; k < ov.numColors; k++)
+      v.color[k] = v3(ov.tfColor[3 * k], ov.tfColor[3 * k + 1], ov.tfColor[3 * k + 2]);
+    v.opacity.assign(ov.tfOpacity, ov.tfOpacity + ov.numOpacities);
+    v.tfLower = ov.tfValueRange[0];
+    v.tfUpper = ov.tfValueRange[1];
+    v.densityScale = ov.densityScale;
+    v.anisotropy = ov.anisotropy;
+    v.userID = ov.userID;
+    precomputeMaxOpacity(v);
+    buildMacrocells(v);
+  }
+  s.volInsts.clear();
+  for (int i = 0; i < d.numInstances; i++) {
+    const OrcGroup &grp = d.groups[d.instances[i].group];
+    for (int k = 0; k < grp.numVolumes; k++)
+      s.volInsts.push_back({i, d.groupVolumes[grp.firstVolume + k], k});
+  }
+  s.hasVolumes = !s.volInsts.empty();
 }
 
 void setupCamera(CameraData &c, const OrcCamera &d) // PerspectiveCamera.cpp:30-110
@@ -1239,6 +1270,7 @@ struct PathVertex
   float pdf_w;
   int type;
   uint32_t numLightSamples;
+  int volInst; // V_VOLUME: index into Scene::volInsts
 };
 
 struct PathState
@@ -1248,6 +1280,7 @@ struct PathState
   bool firstBounceLight, straightPath, specularTransmissionPath;
   uint32_t scatteringEvents, depth;
   LDSampler ld;
+  RandomSampler rs; // volume sampling (PathStructs.ih:83)
 };
 
 struct ScreenSample
@@ -1278,7 +1311,9 @@ static inline float misHeuristic(float pdf1, float pdf2) // PathTracerUtil.ih:30
 static Hit trace(const Scene &s, const Ray &ray) { return s.world.intersect(ray); }
 
 // TransparentShadow.ispc:18-85
-static vec3f transparentShadow(const PathCtx &c, vec3f lightContrib, Ray shadowRay, Medium medium)
+// On return shadowRay spans [t0 of the last segment traced, original t]:
+// the segment that volumeTransmittance then covers.
+static vec3f transparentShadow(const PathCtx &c, vec3f lightContrib, Ray &shadowRay, Medium medium)
 {
   uint32_t maxDepth = (uint32_t)c.r->maxDepth;
   const float tOriginal = shadowRay.t;
@@ -1304,6 +1339,126 @@ static vec3f transparentShadow(const PathCtx &c, vec3f lightContrib, Ray shadowR
   }
 }
 
+// ---- volumes (render/pathtracer/volumes/VolumeSampler.ispc) -------------------------
+
+// volumeSampleFreePath (VolumeSampler.ispc:96-199): delta tracking through
+// every volume interval of the ray up to its surface hit (ray.t), the ray
+// range of the clipping pass being [r0, r1].  The nearest collision wins;
+// once one is found, later intervals are clipped to it.  Returns the
+// collision distance (inf: none) and sets ray.t, volInst and the albedo.
+static float volumeSampleFreePath(const Scene &s, Ray &ray, bool surfaceHit, float r0, float r1, RandomSampler &rs,
+    int &volInst, vec3f &albedo)
+{
+  float minDist = kInf;
+  volInst = -1;
+  const float tTrace = ray.t;
+  bool clipped = surfaceHit;
+  for (size_t j = 0; j < s.volInsts.size(); j++) {
+    const Scene::VolInst &vi = s.volInsts[j];
+    const VolumeData &v = s.volumes[vi.vol];
+    const affine3f &rx = s.world.rcpXfm[vi.inst];
+    const vec3f o = xfmPoint(rx, ray.org), d = xfmVector(rx, ray.dir);
+    float lo, hi;
+    if (!boxInterval(v, o, d, ray.t0, tTrace, lo, hi))
+      continue;
+    lo = maxf(0.f, lo);
+    if (clipped)
+      hi = minf(hi, ray.t);
+    if (hi < lo)
+      continue;
+    if (hi < r0 || r1 < lo)
+      continue;
+    float mu;
+    vec3f alb;
+    const float dist = deltaTracking(v, o, d, maxf(r0, lo), minf(r1, hi), rs, mu, alb);
+    if (dist < kInf && dist < minDist) {
+      minDist = dist;
+      volInst = (int)j;
+      albedo = alb;
+      ray.t = dist;
+      clipped = true;
+    }
+  }
+  return minDist;
+}
+
+// volumeTransmittance (VolumeSampler.ispc:201-277): 1 if delta tracking
+// passes every volume interval of [ray.t0, ray.t] (within [r0, r1]), else 0
+static float volumeTransmittance(const Scene &s, const Ray &ray, float r0, float r1, RandomSampler &rs)
+{
+  float T = 1.f;
+  for (size_t j = 0; j < s.volInsts.size() && T > 0.f; j++) {
+    const Scene::VolInst &vi = s.volInsts[j];
+    const VolumeData &v = s.volumes[vi.vol];
+    const affine3f &rx = s.world.rcpXfm[vi.inst];
+    const vec3f o = xfmPoint(rx, ray.org), d = xfmVector(rx, ray.dir);
+    float lo, hi;
+    if (!boxInterval(v, o, d, ray.t0, ray.t, lo, hi))
+      continue;
+    if (hi < r0 || r1 < lo)
+      continue;
+    const float b = minf(r1, hi);
+    float mu;
+    vec3f alb;
+    const float dist = deltaTracking(v, o, d, maxf(r0, lo), b, rs, mu, alb);
+    T = T * (dist < b ? 0.f : 1.f);
+  }
+  return T;
+}
+
+// HenyeyGreenstein_eval / _sample (volumes/HenyeyGreenstein.ih:28-77); the
+// eval's pdf is the phase function whatever the anisotropy
+static EvalRes hgEval(float g, vec3f wo, vec3f wi)
+{
+  const float p = hgPhase(dot(neg(wo), wi), g);
+  return EvalRes{v3(p), p};
+}
+
+static SampleRes hgSample(float g, vec3f wo, vec2f s)
+{
+  SampleRes res;
+  res.weight = v3(1.f);
+  if (!hgIsSmooth(g)) { // a delta function
+    res.wi = g > 0 ? neg(wo) : wo;
+    res.pdf = kInf;
+    res.type = g > 0 ? SPEC_TRANS : SPEC_REFL;
+    return res;
+  }
+  float cosTheta;
+  if (std::fabs(g) < 1e-3f)
+    cosTheta = 1.f - 2.f * s.x;
+  else {
+    const float sqrTerm = (1.f - g * g) * rcp(1.f - g + 2.f * g * s.x);
+    cosTheta = (1.f + g * g - sqrTerm * sqrTerm) * rcp(2.f * g);
+  }
+  res.type = S_SMOOTH;
+  const float sinTheta = std::sqrt(maxf(0.f, 1.f - cosTheta * cosTheta));
+  const float phi = kTwoPi * s.y;
+  res.wi = mul(frame(neg(wo)), cartesian(phi, sinTheta, cosTheta));
+  res.pdf = hgPhase(cosTheta, g);
+  return res;
+}
+
+float volumeTransmittanceHook(const Scene &s, const Ray &ray, float r0, float r1, RandomSampler &rs)
+{
+  return volumeTransmittance(s, ray, r0, r1, rs);
+}
+
+float volumeFreePathHook(const Scene &s, Ray &ray, float r0, float r1, RandomSampler &rs, int &volInst, vec3f &albedo)
+{
+  return volumeSampleFreePath(s, ray, false, r0, r1, rs, volInst, albedo);
+}
+
+void hgSampleHook(float g, vec3f wo, vec2f s, vec3f &wi, float &pdf, uint32_t &type)
+{
+  const SampleRes r = hgSample(g, wo, s);
+  wi = r.wi;
+  pdf = r.pdf;
+  type = r.type;
+}
+
+float hgEvalHook(float g, vec3f wo, vec3f wi) { return hgEval(g, wo, wi).pdf; }
+
 // NextEventEstimation.ispc:25-160
 static vec3f nextEventEstimation(const PathCtx &c, PathState &ps, PathVertex &pv)
 {
@@ -1323,23 +1478,35 @@ static vec3f nextEventEstimation(const PathCtx &c, PathState &ps, PathVertex &pv
     EvalRes fe{v3(0.f), 0.f};
     if (pv.type == V_SURFACE && pv.hasBsdf)
       fe = bsdfEval(pv.bsdf, pv.wo, ls.dir);
+    else if (pv.type == V_VOLUME)
+      fe = hgEval(c.scene->volumes[c.scene->volInsts[pv.volInst].vol].anisotropy, pv.wo, ls.dir);
     if (reduce_max(fe.value) <= 0.0f)
       continue;
     vec3f org = pv.dg.P;
-    if (dot(pv.dg.Ng, ls.dir) < 0.f)
+    if (pv.type != V_VOLUME && dot(pv.dg.Ng, ls.dir) < 0.f)
       org = org - (2.0f * pv.dg.epsilon) * pv.dg.Ng;
     Ray sr{org, ls.dir, 0.f, ls.dist};
     const vec3f throughput = ps.throughput * fe.value;
     const vec3f unshaded = throughput * ls.weight;
     const vec3f lc = transparentShadow(c, unshaded, sr, ps.currentMedium);
-    if (reduce_max(lc) > 0)
-      L = L + lc * misHeuristic(ls.pdf, fe.pdf);
+    if (reduce_max(lc) > 0) {
+      if (c.scene->hasVolumes) {
+        // two independent transmittance estimates: one weights the sample,
+        // the other enters the MIS weight (NextEventEstimation.ispc:126-149)
+        const float T = volumeTransmittance(*c.scene, sr, 0.f, ls.dist, ps.rs);
+        if (T > 0.f) {
+          const float Tmis = volumeTransmittance(*c.scene, sr, 0.f, ls.dist, ps.rs);
+          L = L + (T * lc) * misHeuristic(ls.pdf, fe.pdf * Tmis);
+        }
+      } else
+        L = L + lc * misHeuristic(ls.pdf, fe.pdf);
+    }
   }
   return L;
 }
 
 // VirtualLight.ispc:38-107
-static vec3f evaluateVirtualLights(const PathCtx &c, const PathState &ps, const PathVertex &last, const PathVertex &pv, const Ray &ray)
+static vec3f evaluateVirtualLights(const PathCtx &c, PathState &ps, const PathVertex &last, const PathVertex &pv, const Ray &ray)
 {
   vec3f L = v3(0.f);
   if (c.numLights <= c.numGeoLights)
@@ -1353,8 +1520,16 @@ static vec3f evaluateVirtualLights(const PathCtx &c, const PathState &ps, const 
       const LightEval le = lightEval(l, last.dg, ray.dir, lower, upper);
       if (reduce_max(le.radiance) > 0.0f) {
         float mis = 1.0f;
-        if (selectionPDF != 0.f)
-          mis = misHeuristic(last.pdf_w, selectionPDF * le.pdf);
+        if (selectionPDF != 0.f) {
+          // transmittance from the last vertex over the light's distance
+          // range (VirtualLight.ispc:76-99)
+          float T = 1.f;
+          if (c.scene->hasVolumes) {
+            const Ray sr{last.dg.P, ray.dir, lower, upper};
+            T = volumeTransmittance(*c.scene, sr, lower, upper, ps.rs);
+          }
+          mis = misHeuristic(last.pdf_w * T, selectionPDF * le.pdf);
+        }
         L = L + ps.throughput * le.radiance * mis;
       }
     }
@@ -1382,6 +1557,25 @@ static vec3f evaluateGeometryLights(const PathCtx &c, const PathState &ps, const
   return ps.throughput * m.radiance * misWeight;
 }
 
+// postIntersect of a volume vertex (PathSampler.ispc:51-68,
+// VolumetricModel.ih:16-33)
+static void volumePostIntersect(const Scene &s, const Ray &ray, PathVertex &pv)
+{
+  const Scene::VolInst &vi = s.volInsts[pv.volInst];
+  DG &dg = pv.dg;
+  dg.P = ray.org + ray.t * ray.dir;
+  const uint32_t instUser = s.world.desc.instances[vi.inst].userID;
+  dg.instID = instUser != kInvalidID ? instUser : (uint32_t)vi.inst;
+  dg.Ng = dg.Ns = dg.dPds = dg.dPdt = v3(0.f);
+  dg.color = vec4f{0.f, 0.f, 0.f, 0.f};
+  dg.primID = 0;
+  const uint32_t volUser = s.volumes[vi.vol].userID;
+  dg.objID = volUser != kInvalidID ? volUser : (uint32_t)vi.local;
+  dg.areaPDF = 0.f;
+  dg.epsilon = 0.f;
+  dg.material = -1;
+}
+
 // PathSampler.ispc:109-415
 static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &sample)
 {
@@ -1401,12 +1595,21 @@ static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &s
     pv.pdf_w = kInf;
     pv.type = V_ENVIRONMENT;
     pv.numLightSamples = 0;
+    pv.volInst = -1;
+    const float r0 = ray.t0, r1 = ray.t; // the clipping pass's ray interval
     const Hit hit = trace(*c.scene, ray);
     if (hit.hit) {
       pv.type = V_SURFACE;
       ray.t = hit.t;
     }
     pv.wo = neg(ray.dir);
+    if (c.scene->hasVolumes) {
+      const float freePath = volumeSampleFreePath(*c.scene, ray, hit.hit, r0, r1, ps.rs, pv.volInst, pv.albedo);
+      if (freePath < kInf) {
+        pv.type = V_VOLUME;
+        ps.throughput = ps.throughput * pv.albedo;
+      }
+    }
     if (ps.depth == 0)
       sample.z = c.projectedDepth ? ray.t * dot(ray.dir, c.cameraDir) : ray.t;
     if (pv.type == V_ENVIRONMENT && ps.straightPath) {
@@ -1415,15 +1618,20 @@ static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &s
       const vec3f bg3 = ps.throughput * v3(bg.x, bg.y, bg.z);
       ps.contribution = ps.contribution + bg3;
     }
-    ps.contribution = ps.contribution + evaluateVirtualLights(c, ps, last, pv, ray);
+    if (pv.type != V_VOLUME)
+      ps.contribution = ps.contribution + evaluateVirtualLights(c, ps, last, pv, ray);
     if (pv.type == V_ENVIRONMENT)
       break;
     // with geometry lights the emission at maxDepth is still evaluated
     // (PathSampler.ispc:249-255)
     if (ps.depth >= (uint32_t)r.maxDepth && c.numGeoLights == 0)
       break;
-    postIntersect(*c.scene, ray, hit, pv.dg, true);
-    ps.contribution = ps.contribution + evaluateGeometryLights(c, ps, last, pv, ray);
+    if (pv.type == V_VOLUME)
+      volumePostIntersect(*c.scene, ray, pv);
+    else {
+      postIntersect(*c.scene, ray, hit, pv.dg, true);
+      ps.contribution = ps.contribution + evaluateGeometryLights(c, ps, last, pv, ray);
+    }
     if (ps.depth == 0) {
       sample.position = pv.dg.P;
       sample.firstNormal = pv.dg.Ns;
@@ -1433,15 +1641,21 @@ static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &s
     }
     if (ps.depth >= (uint32_t)r.maxDepth || ps.scatteringEvents >= (uint32_t)r.maxScatteringEvents)
       break;
-    pv.hasBsdf = getBSDF(*c.scene, pv.dg, ps.currentMedium, pv.bsdf);
-    if (!pv.hasBsdf)
-      break;
-    pv.albedo = pv.bsdf.albedo;
+    float g = 0.f; // anisotropy of a volume vertex
+    if (pv.type == V_SURFACE) {
+      pv.hasBsdf = getBSDF(*c.scene, pv.dg, ps.currentMedium, pv.bsdf);
+      if (!pv.hasBsdf)
+        break;
+      pv.albedo = pv.bsdf.albedo;
+    } else
+      g = c.scene->volumes[c.scene->volInsts[pv.volInst].vol].anisotropy;
     LDSampler_nextGroup(ps.ld);
-    if (pv.bsdf.scatteringType & S_SMOOTH) {
+    // isSmooth (PathTracerUtil.ih:15-28)
+    const bool smooth = pv.type == V_SURFACE ? (pv.bsdf.scatteringType & S_SMOOTH) != 0 : hgIsSmooth(g);
+    if (smooth) {
       if (ps.scatteringEvents == 0) { // updateAuxilliaryData (PathTracerUtil.ih:48-60)
         sample.albedo = ps.throughput * pv.albedo;
-        sample.normal = ps.throughput * pv.bsdf.frame.vz;
+        sample.normal = ps.throughput * (pv.type == V_SURFACE ? pv.bsdf.frame.vz : pv.dg.Ns);
       }
       const vec3f shaded = nextEventEstimation(c, ps, pv);
       ps.contribution = ps.contribution + shaded;
@@ -1450,14 +1664,17 @@ static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &s
     float ss;
     uint32_t uu3;
     const vec2f s = LDSampler_getNext4Samples(ps.ld, ss, uu3);
-    SampleRes fs = bsdfSample(pv.bsdf, pv.wo, s, ss);
-    // sampleDirection: same-side check (PathSampler.ispc:72-107)
-    {
+    // sampleDirection (PathSampler.ispc:72-107): the surface's same-side
+    // check keeps light from leaking
+    SampleRes fs;
+    if (pv.type == V_SURFACE) {
+      fs = bsdfSample(pv.bsdf, pv.wo, s, ss);
       const bool sameSide = dot(fs.wi, pv.dg.Ng) >= 0.f;
       const bool reflection = (fs.type & S_REFLECTION) != 0;
       if (sameSide != reflection)
         fs.weight = v3(0.0f);
-    }
+    } else
+      fs = hgSample(g, pv.wo, s);
     pv.pdf_w = fs.pdf;
     if (reduce_max(fs.weight) <= 0.0f || fs.pdf <= 0.0f)
       break;
@@ -1474,11 +1691,11 @@ static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &s
       ps.throughput = ps.throughput * v3(fexp(a.x), fexp(a.y), fexp(a.z));
     }
     vec3f rayOrg = pv.dg.P;
-    if (fs.type & S_TRANSMISSION) {
+    if (pv.type == V_SURFACE && (fs.type & S_TRANSMISSION)) {
       rayOrg = rayOrg - (2.0f * pv.dg.epsilon) * pv.dg.Ng;
       selectNextMedium(*c.scene, pv.dg, ps.currentMedium);
     }
-    if (fs.type & ~SPEC_TRANS)
+    if ((fs.type & ~SPEC_TRANS) || pv.type == V_VOLUME)
       last = pv;
     if (!eq(ray.dir, fs.wi))
       ps.straightPath = false;
@@ -1532,6 +1749,7 @@ static ScreenSample renderPixel(const PathCtx &c, const FrameBuffer &fb, const C
     const uint32_t sampleID = accumID * spp + s;
     PathState ps;
     LDSampler_init(ps.ld, seed, sampleOffs + sampleID, sampleOffsLight + sampleID * c.numFirst, sampleBits, sampleBitsLight);
+    RandomSampler_init(ps.rs, seed, sampleOffs + sampleID); // PathTracer.ispc:157
     const vec2f center = v2(((float)ix + 0.5f) * rcpW, ((float)iy + 0.5f) * rcpH);
     vec3ui ss234;
     const vec2f pixelSample = LDSampler_getNext5Samples(ps.ld, ss234);

@@ -17,6 +17,7 @@
 
 #include "orc_geom.h"
 #include "orc_sampler.h"
+#include "orc_volume.h"
 
 namespace orc {
 
@@ -104,6 +105,15 @@ struct Scene
   uint32_t numGeoLights = 0;
   std::vector<float> lightsCDF, worldLightsCDF;
   std::vector<float> modelAreaPDF; // per OrcModel
+  std::vector<VolumeData> volumes; // per OrcVolume
+  // every (instance, volume) pair in instance order, then group order: the
+  // order in which a ray meets the volume intervals
+  struct VolInst
+  {
+    int inst, vol, local; // instance, OrcVolume, index in the group
+  };
+  std::vector<VolInst> volInsts;
+  bool hasVolumes = false; // some instance holds a volume (FFO_VOLUME_IN_SCENE)
 };
 
 struct FrameBuffer
@@ -143,5 +153,12 @@ void cameraRay(const CameraData &c, vec2f screen, Ray &ray);
 
 // portable transcendental float functions
 float hostExp(float x);
+
+// volume sampling test hooks: transmittance (0/1) and free path of a world
+// ray, Henyey-Greenstein sampling and evaluation
+float volumeTransmittanceHook(const Scene &s, const Ray &ray, float r0, float r1, RandomSampler &rs);
+float volumeFreePathHook(const Scene &s, Ray &ray, float r0, float r1, RandomSampler &rs, int &volInst, vec3f &albedo);
+void hgSampleHook(float g, vec3f wo, vec2f s, vec3f &wi, float &pdf, uint32_t &type);
+float hgEvalHook(float g, vec3f wo, vec3f wi);
 
 } // namespace orc

@@ -44,7 +44,26 @@ typedef struct
 {
   int32_t firstModel; // range into groupModels
   int32_t numModels;
+  int32_t firstVolume; // range into groupVolumes
+  int32_t numVolumes;
 } OrcGroup;
+
+// a volumetric model: structured-regular volume (vertex-centred voxels,
+// x fastest), piecewise-linear transfer function, densityScale, anisotropy
+// (VolumetricModel.cpp:74-82, LinearTransferFunction.cpp:18-28)
+typedef struct
+{
+  const float *voxels;
+  int64_t dims[3];
+  float gridOrigin[3], gridSpacing[3];
+  const float *tfColor; // vec3f
+  int32_t numColors;
+  const float *tfOpacity;
+  int32_t numOpacities;
+  float tfValueRange[2];
+  float densityScale, anisotropy;
+  uint32_t userID; // 0xffffffff = not set
+} OrcVolume;
 
 typedef struct
 {
@@ -105,6 +124,9 @@ typedef struct
   int32_t numMaterials;
   const OrcLight *lights;
   int32_t numLights;
+  const OrcVolume *volumes;
+  int32_t numVolumes;
+  const int32_t *groupVolumes; // volume indices, grouped by OrcGroup ranges
 } OrcScene;
 
 typedef struct

@@ -77,6 +77,21 @@ class Instance:
 
 
 @dataclasses.dataclass
+class Volume:
+    """A structured-regular volume (vertex-centred voxels) with its
+    piecewise-linear transfer function and volumetric-model parameters."""
+    data: np.ndarray  # (nz, ny, nx) float32, x fastest
+    tf_color: np.ndarray  # (N, 3) float32
+    tf_opacity: np.ndarray  # (M,) float32
+    origin: tuple = (0.0, 0.0, 0.0)
+    spacing: tuple = (1.0, 1.0, 1.0)
+    tf_range: tuple = (0.0, 1.0)
+    density_scale: float = 1.0
+    anisotropy: float = 0.0
+    user_id: Optional[int] = None
+
+
+@dataclasses.dataclass
 class Scene:
     name: str
     width: int
@@ -89,6 +104,8 @@ class Scene:
     lights: list  # dicts with "type" + OSPRay parameters
     camera: dict
     renderer: dict
+    volumes: list = dataclasses.field(default_factory=list)  # Volume
+    group_volumes: list = dataclasses.field(default_factory=list)  # per group: volume indices
 
     @property
     def num_triangles(self) -> int:
