@@ -38,6 +38,11 @@ HPT_API hipError_t hpt_copy(void *dst, const void *src, size_t bytes, hipStream_
 HPT_API hipError_t hpt_generate(const HptFrame *F, const HptPaths *P, hipStream_t s);
 /* Closest hit of every ray in extension queue `queue` (World.ih:247-286). */
 HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int residentBlocks, int stats, hipStream_t s);
+/* volumeSampleFreePath (volumes/VolumeSampler.ispc:96-199) of every ray of
+ * extension queue `queue` after hpt_extend: delta tracking up to the
+ * surface hit; a collision turns the hit record into a volume vertex.
+ * Launched only for worlds with volumes. */
+HPT_API hipError_t hpt_volume(const HptScene *S, const HptPaths *P, int queue, int blocks, hipStream_t s);
 /* One samplePath bounce (PathSampler.ispc:143-407) for queue `queue`; the
  * NEE light samples of its smooth vertices are queued for hpt_nee. */
 HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
@@ -57,8 +62,10 @@ HPT_API hipError_t hpt_shadow(
 HPT_API hipError_t hpt_shadow_trans(
     const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s);
 /* Ordered sum of the NEE samples of every vertex shaded from extension
- * queue `queue` (NextEventEstimation.ispc:150-159). */
-HPT_API hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
+ * queue `queue` (NextEventEstimation.ispc:150-159); with volumes, each
+ * visible sample's transmittance estimates (NextEventEstimation.ispc:126-149). */
+HPT_API hipError_t hpt_resolve(
+    const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
 /* ScreenSample_accumulate over the pass (PathTracer.ispc:170-197). */
 HPT_API hipError_t hpt_finish_pass(const HptFrame *F, const HptPaths *P, hipStream_t s);
 /* LocalFB_accumulateSample (fb/LocalFB.ispc:14-78). */

@@ -130,7 +130,7 @@ inline vec4f tfGet(const VolumeData &v, float value)
 {
   value = (value - v.tfLower) * rcp(v.tfUpper - v.tfLower);
   const float nextBefore1 = 0.999999940395355224609375f; // 0x1.fffffep-1
-  value = value < 0.f ? 0.f : (value > nextBefore1 ? nextBefore1 : value);
+  value = !(value > 0.f) ? 0.f : (value > nextBefore1 ? nextBefore1 : value); // NaN -> 0
   const int maxIdxC = (int)v.color.size() - 1;
   const float idxCf = value * (float)maxIdxC;
   const float fracC = idxCf - std::floor(idxCf);
@@ -150,7 +150,7 @@ inline float tfMaxOpacity(const VolumeData &v, float lower, float upper)
 {
   const int maxDim = kTfSubranges - 1;
   const float denom = v.tfUpper - v.tfLower;
-  auto cl = [](float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); };
+  auto cl = [](float x) { return !(x > 0.f) ? 0.f : (x > 1.f ? 1.f : x); }; // NaN -> 0
   const int i = (int)std::floor(cl((lower - v.tfLower) / denom) * (float)maxDim);
   const int j = (int)std::ceil(cl((upper - v.tfLower) / denom) * (float)maxDim);
   return v.maxOpacityInRange[std::min(i, maxDim)][std::min(j, maxDim)];
@@ -195,7 +195,7 @@ inline void forEachInterval(const VolumeData &v, vec3f o, vec3f d, float t0, flo
   for (int a = 0; a < 3; a++) {
     const float p = oo[a] + t0 * dd[a];
     const float cf = std::floor((p - org[a]) / msz[a]);
-    const int c = cf < 0.f ? 0 : (cf >= (float)v.mdims[a] ? v.mdims[a] - 1 : (int)cf);
+    const int c = !(cf > 0.f) ? 0 : (cf >= (float)v.mdims[a] ? v.mdims[a] - 1 : (int)cf);
     m[a] = c;
     step[a] = dd[a] > 0.f ? 1 : (dd[a] < 0.f ? -1 : 0);
     tnext[a] = step[a] == 0 ? INFINITY : (org[a] + (float)(c + (step[a] > 0 ? 1 : 0)) * msz[a] - oo[a]) * ii[a];

@@ -93,6 +93,9 @@ def set_param(obj: int, name: str, value) -> None:
     elif isinstance(value, Affine3f):
         arr = (C.c_float * 12)(*value.v)
         L.ospSetParam(obj, n, K.OSP_AFFINE3F, arr)
+    elif isinstance(value, Box1f):
+        arr = (C.c_float * 2)(*value.v)
+        L.ospSetParam(obj, n, K.OSP_BOX1F, arr)
     elif isinstance(value, Handle):
         h = C.c_void_p(value.h)
         L.ospSetParam(obj, n, value.type, C.byref(h))
@@ -118,6 +121,13 @@ class UInt:
 
     def __init__(self, v: int):
         self.v = int(v)
+
+
+class Box1f:
+    """box1f parameter (e.g. a transfer function's 'value' range)."""
+
+    def __init__(self, lo: float, hi: float):
+        self.v = [float(lo), float(hi)]
 
 
 class Affine3f:
@@ -168,6 +178,17 @@ def shared_data(arr: np.ndarray) -> Handle:
     return Handle(h, K.OSP_DATA, keep=a)
 
 
+def shared_data3d(arr: np.ndarray) -> Handle:
+    """ospNewSharedData over a C-contiguous (nz, ny, nx) float32 array: a 3D
+    array with x fastest, as structured volumes take it."""
+    a = np.ascontiguousarray(arr, np.float32)
+    if a.ndim != 3:
+        raise ValueError("expected a 3D array")
+    nz, ny, nx = a.shape
+    h = K.lib().ospNewSharedData(a.ctypes.data, K.OSP_FLOAT, nx, 0, ny, 0, nz, 0, None, None)
+    return Handle(h, K.OSP_DATA, keep=a)
+
+
 def object_data(handles: Iterable[Handle], type_: int) -> Handle:
     """A data array of object handles (copied with ospCopyData, so the
     array holds its own references)."""
@@ -197,6 +218,10 @@ def new(kind: str, type_: str = "", params: dict | None = None, commit: bool = T
         h = Handle(fn(_b(type_)), t)
     elif kind == "model":
         h = Handle(L.ospNewGeometricModel(None), K.OSP_GEOMETRIC_MODEL)
+    elif kind == "transfer_function":
+        h = Handle(L.ospNewTransferFunction(_b(type_)), K.OSP_TRANSFER_FUNCTION)
+    elif kind == "volumetric_model":
+        h = Handle(L.ospNewVolumetricModel(None), K.OSP_VOLUMETRIC_MODEL)
     elif kind == "group":
         h = Handle(L.ospNewGroup(), K.OSP_GROUP)
     elif kind == "instance":

@@ -348,10 +348,195 @@ struct GeometricModel : public ManagedObject
   }
 };
 
+// ---- volumes ---------------------------------------------------------------------------
+
+// "structuredRegular" volume (OpenVKL's structured-regular grid behind
+// volume/Volume.cpp:27-160): vertex-centred voxels of a 3D "data" array
+// (x fastest) at gridOrigin + i * gridSpacing, trilinear filter.  The
+// voxels go to HBM as float; the macrocell (min, max) grid of kMacro^3
+// cells is built at commit.
+struct Volume : public ManagedObject
+{
+  int32_t dims[3] = {0, 0, 0}, mdims[3] = {0, 0, 0};
+  float origin[3] = {0, 0, 0}, spacing[3] = {1, 1, 1};
+  DevBuf dVoxels, dMacro;
+  uint64_t version = 0;
+  Volume(const std::string &t)
+  {
+    managedType = OSP_VOLUME;
+    subtype = t;
+  }
+  std::string toString() const override { return "ospray::Volume(" + subtype + ")"; }
+  void commit() override
+  {
+    Data *d = getData(this, "data");
+    if (!d)
+      throw std::runtime_error("structuredRegular volume requires a 3D 'data' array");
+    for (int a = 0; a < 3; a++) {
+      if (d->numItems[a] < 2 || d->numItems[a] > (1u << 20))
+        throw std::runtime_error("structuredRegular volume: every dimension of 'data' must be in [2, 2^20]");
+      dims[a] = (int32_t)d->numItems[a];
+    }
+    const float zero[3] = {0.f, 0.f, 0.f}, one[3] = {1.f, 1.f, 1.f};
+    getVec3f("gridOrigin", origin, zero);
+    getVec3f("gridSpacing", spacing, one);
+    if (getBool("cellCentered", false))
+      throw std::runtime_error("cell-centred structuredRegular volumes are not supported by the hip device");
+    if (getInt("filter", OSP_VOLUME_FILTER_LINEAR) != OSP_VOLUME_FILTER_LINEAR)
+      throw std::runtime_error("the hip device samples volumes with the trilinear filter only");
+    (void)getFloat("background", 0.f);
+    const size_t n = d->size();
+    std::vector<float> vox(n);
+    for (size_t i = 0; i < n; i++) {
+      switch (d->elemType) {
+      case OSP_FLOAT: vox[i] = d->get<float>(i); break;
+      case OSP_DOUBLE: vox[i] = (float)d->get<double>(i); break;
+      case OSP_UCHAR: vox[i] = (float)d->get<uint8_t>(i); break;
+      case OSP_USHORT: vox[i] = (float)d->get<uint16_t>(i); break;
+      case OSP_SHORT: vox[i] = (float)d->get<int16_t>(i); break;
+      default: throw std::runtime_error("structuredRegular 'data' must be float, double, uchar, short or ushort");
+      }
+    }
+    // macrocell m covers voxels [m * kMacro, (m + 1) * kMacro] per axis
+    for (int a = 0; a < 3; a++)
+      mdims[a] = (dims[a] - 1 + hpt::kMacro - 1) / hpt::kMacro;
+    std::vector<float> macro(2 * (size_t)mdims[0] * mdims[1] * mdims[2]);
+    for (int mz = 0; mz < mdims[2]; mz++)
+      for (int my = 0; my < mdims[1]; my++)
+        for (int mx = 0; mx < mdims[0]; mx++) {
+          float lo = hpt::kInf, hi = -hpt::kInf;
+          for (int z = mz * hpt::kMacro; z <= std::min((mz + 1) * hpt::kMacro, dims[2] - 1); z++)
+            for (int y = my * hpt::kMacro; y <= std::min((my + 1) * hpt::kMacro, dims[1] - 1); y++) {
+              const float *row = vox.data() + ((size_t)z * dims[1] + y) * dims[0];
+              for (int x = mx * hpt::kMacro; x <= std::min((mx + 1) * hpt::kMacro, dims[0] - 1); x++) {
+                lo = std::min(lo, row[x]);
+                hi = std::max(hi, row[x]);
+              }
+            }
+          const size_t m = ((size_t)mz * mdims[1] + my) * mdims[0] + mx;
+          macro[2 * m] = lo;
+          macro[2 * m + 1] = hi;
+        }
+    dVoxels.upload(vox);
+    dMacro.upload(macro);
+    version++;
+  }
+};
+
+// "piecewiseLinear" transfer function (LinearTransferFunction.cpp:18-160,
+// TransferFunction.cpp:19): colour and opacity control points spread
+// evenly over the value range, and the precomputed maximum opacity of
+// every pair of kTfSubranges value sub-ranges
+struct TransferFunction : public ManagedObject
+{
+  std::vector<float> color, opacity;
+  float valueRange[2] = {0.f, 1.f};
+  float maxOpacity[hpt::kTfSubranges * hpt::kTfSubranges];
+  DevBuf dColor, dOpacity, dMaxOpacity;
+  TransferFunction(const std::string &t)
+  {
+    managedType = OSP_TRANSFER_FUNCTION;
+    subtype = t;
+  }
+  std::string toString() const override { return "ospray::TransferFunction(" + subtype + ")"; }
+  void commit() override
+  {
+    Data *c = getData(this, "color");
+    Data *o = getData(this, "opacity");
+    if (!c || !o || c->size() == 0 || o->size() == 0)
+      throw std::runtime_error("piecewiseLinear transfer function requires 'color' and 'opacity' arrays");
+    color = dataFloats(c, 3, "transfer function 'color'");
+    opacity = dataFloats(o, 1, "transfer function 'opacity'");
+    valueRange[0] = 0.f;
+    valueRange[1] = 1.f;
+    if (const Param *pr = findParam("value", OSP_BOX1F))
+      std::memcpy(valueRange, pr->bytes, 8);
+    // precomputeMaxOpacityRanges (LinearTransferFunction.cpp:108-160)
+    const int maxOpacityDim = (int)opacity.size() - 1;
+    const int maxPre = hpt::kTfSubranges - 1;
+    auto at = [&](int i, int j) -> float & { return maxOpacity[i * hpt::kTfSubranges + j]; };
+    for (int i = 0; i < hpt::kTfSubranges; i++) {
+      const int lo = (int)std::floor(maxOpacityDim * (float)i / maxPre);
+      const int hi = (int)std::ceil(maxOpacityDim * (float)i / maxPre);
+      float m = opacity[lo];
+      for (int k = lo; k <= hi; k++)
+        m = std::max(m, opacity[k]);
+      at(i, i) = m;
+    }
+    for (int i = 0; i < hpt::kTfSubranges; i++)
+      for (int j = i + 1; j < hpt::kTfSubranges; j++) {
+        const int lo = (int)std::floor(maxOpacityDim * (float)i / maxPre);
+        const int hi = (int)std::ceil(maxOpacityDim * (float)j / maxPre);
+        float m = at(i, i);
+        for (int k = lo; k <= hi; k++)
+          m = std::max(m, opacity[k]);
+        at(i, j) = m;
+      }
+    for (int i = 0; i < hpt::kTfSubranges; i++) // inverted ranges: zero-filled
+      for (int j = 0; j < i; j++)
+        at(i, j) = 0.f;
+    dColor.upload(color);
+    dOpacity.upload(opacity);
+    dMaxOpacity.upload(std::vector<float>(maxOpacity, maxOpacity + hpt::kTfSubranges * hpt::kTfSubranges));
+  }
+};
+
+// VolumetricModel (volume/VolumetricModel.cpp:31-82)
+struct VolumetricModel : public ManagedObject
+{
+  Ref<Volume> volume;
+  Ref<TransferFunction> tf;
+  float densityScale = 1.f, anisotropy = 0.f;
+  uint32_t userID = hpt::kInvalidID;
+  VolumetricModel() { managedType = OSP_VOLUMETRIC_MODEL; }
+  std::string toString() const override { return "ospray::VolumetricModel"; }
+  void commit() override
+  {
+    volume.reset((Volume *)getObject("volume", OSP_VOLUME));
+    if (!volume)
+      throw std::runtime_error("volumetric model requires a 'volume'");
+    tf.reset((TransferFunction *)getObject("transferFunction", OSP_TRANSFER_FUNCTION));
+    if (!tf)
+      throw std::runtime_error("volumetric model requires a 'transferFunction'");
+    densityScale = getFloat("densityScale", 1.f);
+    anisotropy = getFloat("anisotropy", 0.f);
+    (void)getFloat("gradientShadingScale", 0.f);
+    userID = getUInt("id", hpt::kInvalidID);
+  }
+  hpt::DVolume device() const
+  {
+    hpt::DVolume v;
+    std::memset(&v, 0, sizeof(v));
+    const Volume &V = *volume.p;
+    v.voxels = V.dVoxels.as<const float>();
+    v.macro = V.dMacro.as<const float>();
+    v.tfColor = tf->dColor.as<const float>();
+    v.tfOpacity = tf->dOpacity.as<const float>();
+    v.maxOpacity = tf->dMaxOpacity.as<const float>();
+    for (int a = 0; a < 3; a++) {
+      v.dims[a] = V.dims[a];
+      v.mdims[a] = V.mdims[a];
+      v.origin[a] = V.origin[a];
+      v.spacing[a] = V.spacing[a];
+      v.upper[a] = V.origin[a] + (float)(V.dims[a] - 1) * V.spacing[a];
+    }
+    v.numColors = (int32_t)(tf->color.size() / 3);
+    v.numOpacities = (int32_t)tf->opacity.size();
+    v.tfLower = tf->valueRange[0];
+    v.tfUpper = tf->valueRange[1];
+    v.densityScale = densityScale;
+    v.anisotropy = anisotropy;
+    v.userID = userID;
+    return v;
+  }
+};
+
 struct Group : public ManagedObject
 {
   std::vector<GeometricModel *> models;
   std::vector<std::unique_ptr<Ref<GeometricModel>>> refs;
+  std::vector<VolumetricModel *> volumes;
+  std::vector<std::unique_ptr<Ref<VolumetricModel>>> volumeRefs;
   Group() { managedType = OSP_GROUP; }
   std::string toString() const override { return "ospray::Group"; }
   void commit() override // Group.cpp:69-209
@@ -364,8 +549,14 @@ struct Group : public ManagedObject
       refs.emplace_back(new Ref<GeometricModel>());
       refs.back()->reset(m);
     }
-    if (getData(this, "volume"))
-      throw std::runtime_error("volumes are not supported by the hip device");
+    volumes = dataObjects<VolumetricModel>(getData(this, "volume"), OSP_VOLUMETRIC_MODEL);
+    volumeRefs.clear();
+    for (auto *v : volumes) {
+      if (!v || !v->volume)
+        throw std::runtime_error("group 'volume' holds a null or uncommitted volumetric model");
+      volumeRefs.emplace_back(new Ref<VolumetricModel>());
+      volumeRefs.back()->reset(v);
+    }
     if (getData(this, "light"))
       throw std::runtime_error("group (instance) lights are not supported by the hip device");
     (void)getData(this, "clippingGeometry");
@@ -488,6 +679,7 @@ struct World : public ManagedObject
   std::vector<Instance *> instances;
   std::vector<Material *> materials; // index order of the material table
   DevBuf dNodes, dPrims, dInstances, dModels, dMaterials, dMedia, dLights, dLightsCDF, dWorldLightsCDF, dGeoCDF;
+  DevBuf dVolumes, dVolInsts;
   hpt::DScene scene{};
   bool built = false;
   int64_t totalNodes = 0, totalPrims = 0;
@@ -826,6 +1018,16 @@ class HipDevice : public Device
       if (t == "pathtracer")
         return new Renderer(t);
       break;
+    case OSP_VOLUME:
+      if (t == "structuredRegular")
+        return new Volume(t);
+      break;
+    case OSP_TRANSFER_FUNCTION:
+      if (t == "piecewiseLinear")
+        return new TransferFunction(t);
+      break;
+    case OSP_VOLUMETRIC_MODEL:
+      return new VolumetricModel();
     case OSP_GEOMETRIC_MODEL:
       return new GeometricModel();
     case OSP_GROUP:
@@ -1387,6 +1589,25 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   if (!worldCdf.empty())
     hpt::distribution1DCreate((int)worldCdf.size(), worldCdf.data());
 
+  // volumes: one DVolume per volumetric model, and every (instance, model)
+  // pair in instance order, then group order
+  std::vector<hpt::DVolume> dvols;
+  std::vector<hpt::DVolInst> volInsts;
+  std::map<const VolumetricModel *, int> volIndex;
+  for (size_t i = 0; i < instances.size(); i++) {
+    const std::vector<VolumetricModel *> &gv = groups[groupIndex[instances[i]->group.p]]->volumes;
+    for (size_t k = 0; k < gv.size(); k++) {
+      auto it = volIndex.find(gv[k]);
+      if (it == volIndex.end()) {
+        it = volIndex.emplace(gv[k], (int)dvols.size()).first;
+        dvols.push_back(gv[k]->device());
+      }
+      volInsts.push_back({(int32_t)i, it->second, (int32_t)k, 0});
+    }
+  }
+  if (instances.size() >= hpt::kVolumeHit || volInsts.size() >= hpt::kVolumeHit)
+    throw std::runtime_error("too many instances for the hip device");
+
   dNodes.upload(nodes);
   dPrims.upload(prims);
   dInstances.upload(dinst);
@@ -1410,6 +1631,11 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   scene.numInstances = (int)instances.size();
   scene.numLights = (int)dl.size();
   scene.numMaterials = (int)dmats.size();
+  dVolumes.upload(dvols);
+  dVolInsts.upload(volInsts);
+  scene.volumes = dVolumes.as<const hpt::DVolume>();
+  scene.volInsts = dVolInsts.as<const hpt::DVolInst>();
+  scene.numVolInsts = (int32_t)volInsts.size();
   totalNodes = (int64_t)nodes.size();
   totalPrims = (int64_t)prims.size();
   for (int k = 0; k < 3; k++) {
@@ -1438,11 +1664,11 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   recSlots = std::max((size_t)maxRec * pathSlots, recSlots);
   auxAllocated = auxAllocated || aux;
   const size_t S = pathSlots, R = recSlots;
-  // per-slot arrays: 9 x float4 + hitPrim + ids(uint4) + neeN + NEE vertex
-  // record (6 x float4) (+ aux 16 floats)
-  const size_t perSlot = 9 * 16 + 4 + 16 + 4 + 6 * 16 + (auxAllocated ? 64 : 0);
+  // per-slot arrays: 9 x float4 + hitPrim + ids(uint4) + neeN + volume
+  // sampler (uint4) + NEE vertex record (6 x float4) (+ aux 16 floats)
+  const size_t perSlot = 9 * 16 + 4 + 16 + 4 + 16 + 6 * 16 + (auxAllocated ? 64 : 0);
   pathBuf.ensure(S * perSlot);
-  recBuf.ensure(R * 64);
+  recBuf.ensure(R * 80);
   queueBuf.ensure((2 * S + 3 * R) * 4);
   char *b = (char *)pathBuf.ptr;
   auto take = [&](size_t bytes) {
@@ -1462,6 +1688,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.ids = (uint32_t *)take(S * 16);
   paths.hitPrim = (uint32_t *)take(S * 4);
   paths.neeN = (uint32_t *)take(S * 4);
+  paths.rng = (uint32_t *)take(S * 16);
   paths.vtxP = (float *)take(S * 16);
   paths.vtxNg = (float *)take(S * 16);
   paths.vtxNs = (float *)take(S * 16);
@@ -1475,6 +1702,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.recDir = (float *)(r + R * 16);
   paths.recContrib = (float *)(r + R * 32);
   paths.recResult = (float *)(r + R * 48);
+  paths.recVol = (float *)(r + R * 64);
   uint32_t *q = (uint32_t *)queueBuf.ptr;
   paths.queue[0] = q;
   paths.queue[1] = q + S;
@@ -1592,10 +1820,11 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   if (r->errorThreshold > 0.f && fb->variance.ptr && fb->frameID >= std::max(2, 16 / std::max(1, r->spp)))
     F.adaptiveThreshold = r->errorThreshold;
   F.maxRec = (int)std::max<uint32_t>(1u, F.numFirstLightSamples);
+  F.hasVolumes = sceneF.numVolInsts > 0 ? 1 : 0;
 
   if (!finishedAccum) {
     const bool aux = F.accAux != nullptr;
-    const size_t perSlotBytes = 9 * 16 + 40 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 76 + 8;
+    const size_t perSlotBytes = 9 * 16 + 56 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 92 + 8;
     const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
     const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
     ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
@@ -1616,13 +1845,15 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       for (int b = 0; b <= F.maxDepth; b++) {
         launch("reset", [&]() { return hpt_reset_counters(P, q, stream); });
         launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
+        if (F.hasVolumes)
+          launch("volume", [&]() { return hpt_volume(S, P, q, shadeBlocks, stream); });
         launch("shade", [&]() { return hpt_shade(S, Fp, P, q, shadeBlocks, stream); });
         if (F.numFirstLightSamples > 0) {
           launch("nee", [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
           launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
           launch("shadow_trans",
               [&]() { return hpt_shadow_trans(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
-          launch("resolve", [&]() { return hpt_resolve(Fp, P, q, shadeBlocks, stream); });
+          launch("resolve", [&]() { return hpt_resolve(S, Fp, P, q, shadeBlocks, stream); });
         }
         q ^= 1;
       }

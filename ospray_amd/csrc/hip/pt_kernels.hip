@@ -21,6 +21,7 @@
 
 #include "ospray_hip/hip_pt.h"
 #include "pt_shade.h"
+#include "pt_volume.h"
 
 using namespace hpt;
 
@@ -113,6 +114,21 @@ __device__ __forceinline__ bool taskActive(const DFrame &F, int ix, int iy)
 
 __device__ __forceinline__ uint32_t clz32(uint32_t x) { return x ? (uint32_t)__clz((int)x) : 32u; }
 
+// the path's PCG32 volume sampler in P.rng (state lo, hi, stream)
+__device__ __forceinline__ RandomSampler loadRng(const DPaths &P, uint32_t p)
+{
+  const uint4 r = *reinterpret_cast<const uint4 *>(P.rng + 4 * (size_t)p);
+  RandomSampler rs;
+  rs.state = (uint64_t)r.x | ((uint64_t)r.y << 32);
+  rs.stream = r.z;
+  return rs;
+}
+__device__ __forceinline__ void storeRng(const DPaths &P, uint32_t p, const RandomSampler &rs)
+{
+  *reinterpret_cast<uint4 *>(P.rng + 4 * (size_t)p) =
+      make_uint4((uint32_t)rs.state, (uint32_t)(rs.state >> 32), rs.stream, 0u);
+}
+
 // pixel filter (PixelFilterDispatch.ispc:7-19, LUTPixelFilter.ispc:9-15,
 // Distribution2D.ispc:9-25)
 __device__ vec2f pixelFilterSample(const DFrame &F, vec2f s)
@@ -203,6 +219,11 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
       st4(P.lastP + 4 * (size_t)i, org.x, org.y, org.z, kInf);
       st4(P.lastNs + 4 * (size_t)i, dir.x, dir.y, dir.z, __uint_as_float(0u));
       st4u(P.ids + 4 * (size_t)i, kInvalidID, kInvalidID, kInvalidID, 0u);
+      if (F.hasVolumes) { // RandomSampler_init (PathTracer.ispc:157)
+        RandomSampler rs;
+        randomSamplerInit(rs, seed, sampleOffs + sampleID);
+        storeRng(P, (uint32_t)i, rs);
+      }
       if (P.aux) {
         float *a = P.aux + 16 * (size_t)i;
         for (int k = 0; k < 16; k += 4)
@@ -258,11 +279,40 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
   }
 }
 
+// volumeSampleFreePath (VolumeSampler.ispc:96-199) of every queued ray
+// after its closest surface hit: delta tracking through the volume
+// intervals up to the hit.  A collision replaces the hit record by a
+// volume vertex (t, pair index, albedo).  The ray range of the clipping
+// pass is [t0, inf) (no clipping geometry).
+__global__ __launch_bounds__(256) void k_volume(DScene S, DPaths P, int qin)
+{
+  const int n = P.counters[2 * qin];
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t p = P.queue[qin][i];
+    const size_t p4 = 4 * (size_t)p;
+    const float4 o4 = ld4(P.org + p4);
+    const float4 d4 = ld4(P.dir + p4);
+    const float4 hA = ld4(P.hitA + p4);
+    const bool surface = __float_as_uint(hA.w) != kInvalidID;
+    RandomSampler rs = loadRng(P, p);
+    int pair;
+    vec3f albedo;
+    const float dist = volumeFreePath(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), o4.w, surface ? hA.x : kInf,
+        surface, o4.w, kInf, rs, pair, albedo);
+    storeRng(P, p, rs);
+    if (dist < kInf) {
+      st4(P.hitA + p4, dist, 0.f, 0.f, __uint_as_float(kVolumeHit | (uint32_t)pair));
+      st4(P.hitB + p4, albedo.x, albedo.y, albedo.z, 0.f);
+    }
+  }
+}
+
 // one iteration of samplePath (PathSampler.ispc:143-407) for every queued
 // path.  At a smooth vertex it stores the NEE vertex record and appends the
 // vertex's light-sample slots to the NEE sample queue; k_nee evaluates them
 // (NextEventEstimation.ispc:25-160).  The loop runs over whole waves so the
 // appends can scan across the wave.
+template <bool VOL>
 __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int qin)
 {
   const int n = P.counters[2 * qin];
@@ -313,6 +363,14 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
       hit.prim = hp & 0x7fffffffu;
       hit.half = hp >> 31;
       const float rayT = hit.hit ? hit.t : kInf;
+      // a volume collision of k_volume (PathSampler.ispc:178-193)
+      const bool volHit = VOL && hit.hit && (hit.inst & kVolumeHit) != 0;
+      constexpr bool volumes = VOL;
+      RandomSampler rs;
+      if (volumes)
+        rs = loadRng(P, p);
+      if (volHit)
+        thr = thr * hit.Ng;
 
       const bool straight = (flags & FLAG_STRAIGHT) != 0;
       if (depth == 0)
@@ -324,8 +382,8 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         const vec3f bg3 = thr * v3(F.bgColor[0], F.bgColor[1], F.bgColor[2]);
         contribution = contribution + bg3;
       }
-      // evaluateVirtualLights (VirtualLight.ispc:38-107)
-      if (S.numLights > S.numGeoLights) {
+      // evaluateVirtualLights (VirtualLight.ispc:38-107), not at volume vertices
+      if (!volHit && S.numLights > S.numGeoLights) {
         const vec3f lastP = v3(lp4.x, lp4.y, lp4.z);
         const vec3f lastNs = v3(ln4.x, ln4.y, ln4.z);
         const float lastPdf = lp4.w;
@@ -340,8 +398,14 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
             const LightEval le = lightEval(l, lastP, lastNs, rdir, lower, upper);
             if (reduce_max(le.radiance) > 0.0f) {
               float mis = 1.0f;
-              if (selectionPDF != 0.f)
-                mis = misHeuristic(lastPdf, selectionPDF * le.pdf);
+              if (selectionPDF != 0.f) {
+                // transmittance over the light's distance range
+                // (VirtualLight.ispc:76-99)
+                float T = 1.f;
+                if (volumes)
+                  T = volumeTransmittance(S, lastP, rdir, lower, upper, lower, upper, rs);
+                mis = misHeuristic(lastPdf * T, selectionPDF * le.pdf);
+              }
               L = L + thr * le.radiance * mis;
             }
           }
@@ -361,9 +425,27 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         if (depth >= (uint32_t)F.maxDepth && S.numGeoLights == 0)
           break;
         DG dg;
-        postIntersect(S, rorg, rdir, hit, dg, true);
-        contribution = contribution
-            + evaluateGeometryLights(S, dg, rdir, thr, v3(lp4.x, lp4.y, lp4.z), lp4.w, __float_as_uint(ln4.w));
+        float g = 0.f; // anisotropy of a volume vertex
+        const int pair = (int)(hit.inst & ~kVolumeHit);
+        if (volHit) { // PathSampler.ispc:51-68, VolumetricModel.ih:16-33
+          const DVolInst vi = S.volInsts[pair];
+          const DVolume &vol = S.volumes[vi.vol];
+          g = vol.anisotropy;
+          dg.P = rorg + rayT * rdir;
+          const uint32_t instUser = S.instances[vi.inst].userID;
+          dg.instID = instUser != kInvalidID ? instUser : (uint32_t)vi.inst;
+          dg.Ng = dg.Ns = dg.dPds = dg.dPdt = v3(0.f);
+          dg.colorW = 0.f;
+          dg.primID = 0;
+          dg.objID = vol.userID != kInvalidID ? vol.userID : (uint32_t)vi.local;
+          dg.areaPDF = 0.f;
+          dg.epsilon = 0.f;
+          dg.material = -2 - pair;
+        } else {
+          postIntersect(S, rorg, rdir, hit, dg, true);
+          contribution = contribution
+              + evaluateGeometryLights(S, dg, rdir, thr, v3(lp4.x, lp4.y, lp4.z), lp4.w, __float_as_uint(ln4.w));
+        }
         if (depth == 0) {
           P.ids[p4 + 0] = dg.primID;
           P.ids[p4 + 1] = dg.objID;
@@ -381,16 +463,17 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         if (depth >= (uint32_t)F.maxDepth || scat >= (uint32_t)F.maxScatteringEvents)
           break;
         BSDF b;
-        if (!getBSDF(S, dg, S.media[medium], b))
+        if (!volHit && !getBSDF(S, dg, S.media[medium], b))
           break;
         ldNextGroup(lds);
         const vec3f wo = neg(rdir);
         uint32_t pvNLS = 0;
-        if (b.scatteringType & S_SMOOTH) {
-          if (scat == 0 && P.aux) {
+        // isSmooth (PathTracerUtil.ih:15-28)
+        if (volHit ? hgIsSmooth(g) : (b.scatteringType & S_SMOOTH) != 0) {
+          if (scat == 0 && P.aux) { // updateAuxilliaryData (PathTracerUtil.ih:48-60)
             float *a = P.aux + 16 * (size_t)p;
-            const vec3f nn = thr * b.frame.vz;
-            const vec3f al = thr * b.albedo;
+            const vec3f nn = thr * (volHit ? dg.Ns : b.frame.vz);
+            const vec3f al = thr * (volHit ? hit.Ng : b.albedo);
             a[6] = nn.x;
             a[7] = nn.y;
             a[8] = nn.z;
@@ -416,8 +499,11 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         float ss;
         uint32_t uu3;
         const vec2f s = ldGetNext4Samples(lds, ss, uu3);
-        SampleRes fs = bsdfSample(b, wo, s, ss);
-        {
+        SampleRes fs;
+        if (volHit)
+          fs = hgSample(g, wo, s);
+        else {
+          fs = bsdfSample(b, wo, s, ss);
           const bool sameSide = dot(fs.wi, dg.Ng) >= 0.f;
           const bool reflection = (fs.type & S_REFLECTION) != 0;
           if (sameSide != reflection)
@@ -439,11 +525,11 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
           thr = thr * v3(fexp(a.x), fexp(a.y), fexp(a.z));
         }
         vec3f rayOrg = dg.P;
-        if (fs.type & S_TRANSMISSION) {
+        if (!volHit && (fs.type & S_TRANSMISSION)) {
           rayOrg = rayOrg - (2.0f * dg.epsilon) * dg.Ng;
           medium = selectNextMedium(S, dg, medium);
         }
-        if (fs.type & ~SPEC_TRANS) {
+        if ((fs.type & ~SPEC_TRANS) || volHit) {
           newLastP = dg.P;
           newLastNs = dg.Ns;
           newLastPdf = fs.pdf;
@@ -465,6 +551,8 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
 
       st4(P.thr + p4, thr.x, thr.y, thr.z, alpha);
       st4(P.contrib + p4, contribution.x, contribution.y, contribution.z, con4.w);
+      if (volumes)
+        storeRng(P, p, rs);
       if (cont) {
         st4(P.org + p4, newOrg.x, newOrg.y, newOrg.z, 0.f);
         st4(P.dir + p4, newDir.x, newDir.y, newDir.z, 0.f);
@@ -523,22 +611,29 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
       st4(P.recResult + 4 * (size_t)r, 0.f, 0.f, 0.f, 0.f);
       emit = !(reduce_max(ls.weight) <= 0.0f || ls.pdf <= 0.0f);
       EvalRes fe = {v3(0.f), 0.f};
+      const int mat = __float_as_int(vNg.w); // <= -2: volume vertex of pair -2 - mat
       if (emit) {
         const float4 vWo = ld4(P.vtxWo + p4);
-        BSDF b;
-        getEvalBSDF(S, __float_as_int(vNg.w), Ns, b);
-        fe = bsdfEval(b, v3(vWo.x, vWo.y, vWo.z), ls.dir);
+        if (mat <= -2) {
+          fe = hgEval(S.volumes[S.volInsts[-2 - mat].vol].anisotropy, v3(vWo.x, vWo.y, vWo.z), ls.dir);
+        } else {
+          BSDF b;
+          getEvalBSDF(S, mat, Ns, b);
+          fe = bsdfEval(b, v3(vWo.x, vWo.y, vWo.z), ls.dir);
+        }
         emit = !(reduce_max(fe.value) <= 0.0f);
       }
       if (emit) {
         vec3f org = Pv;
-        if (dot(Ng, ls.dir) < 0.f)
+        if (mat > -2 && dot(Ng, ls.dir) < 0.f)
           org = org - (2.0f * vP.w) * Ng;
         const vec3f throughput = v3(vThr.x, vThr.y, vThr.z) * fe.value;
         const vec3f unshaded = throughput * ls.weight;
         st4(P.recOrg + 4 * (size_t)r, org.x, org.y, org.z, ls.dist);
         st4(P.recDir + 4 * (size_t)r, ls.dir.x, ls.dir.y, ls.dir.z, misHeuristic(ls.pdf, fe.pdf));
         st4(P.recContrib + 4 * (size_t)r, unshaded.x, unshaded.y, unshaded.z, vNs.w);
+        if (S.numVolInsts > 0)
+          st4(P.recVol + 4 * (size_t)r, ls.pdf, fe.pdf, 0.f, 0.f);
       }
     }
     const int slot = enqueue(&P.counters[kCntShadow], emit);
@@ -579,7 +674,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
           const float4 c4 = ld4(P.recContrib + 4 * (size_t)r);
           const vec3f lc = v3(c4.x, c4.y, c4.z);
           if (reduce_max(lc) > 0) {
-            const vec3f res = lc * d4.w;
+            // with volumes, transmittance and MIS are applied in k_resolve
+            const vec3f res = S.numVolInsts > 0 ? lc : lc * d4.w;
             st4(P.recResult + 4 * (size_t)r, res.x, res.y, res.z, 1.f);
           }
         }
@@ -645,8 +741,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
       t0 = h.t + dg.epsilon;
     }
     if (reduce_max(lc) > 0) {
-      const vec3f res = lc * d4.w;
+      const vec3f res = S.numVolInsts > 0 ? lc : lc * d4.w;
       st4(P.recResult + 4 * (size_t)r, res.x, res.y, res.z, 1.f);
+      if (S.numVolInsts > 0)
+        P.recVol[4 * (size_t)r + 2] = t0; // the last segment [t0, tOriginal]
     }
   }
   if (STATS) {
@@ -657,8 +755,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
 }
 
 // sum of the NEE samples of a vertex in light-sample order, added to the
-// path contribution (NextEventEstimation.ispc:150-159, PathSampler.ispc:338)
-__global__ __launch_bounds__(256) void k_resolve(DFrame F, DPaths P, int qin)
+// path contribution (NextEventEstimation.ispc:150-159, PathSampler.ispc:338).
+// With volumes each visible sample is weighted by two independent
+// transmittance estimates along its last shadow segment, one for the
+// sample and one inside the MIS weight (NextEventEstimation.ispc:126-149),
+// drawn from the path's sampler in sample order.
+template <bool VOL>
+__global__ __launch_bounds__(256) void k_resolve(DScene S, DFrame F, DPaths P, int qin)
 {
   const int n = P.counters[2 * qin];
   for (int i = globalId(); i < n; i += gridStride()) {
@@ -667,10 +770,30 @@ __global__ __launch_bounds__(256) void k_resolve(DFrame F, DPaths P, int qin)
     if (nls == 0)
       continue;
     vec3f L = v3(0.f);
-    for (uint32_t k = 0; k < nls; k++) {
-      const float4 r = ld4(P.recResult + 4 * ((size_t)p * F.maxRec + k));
-      if (r.w != 0.f)
-        L = L + v3(r.x, r.y, r.z);
+    if (VOL) {
+      RandomSampler rs = loadRng(P, p);
+      for (uint32_t k = 0; k < nls; k++) {
+        const size_t r4 = 4 * ((size_t)p * F.maxRec + k);
+        const float4 r = ld4(P.recResult + r4);
+        if (r.w == 0.f)
+          continue;
+        const float4 o4 = ld4(P.recOrg + r4);
+        const float4 d4 = ld4(P.recDir + r4);
+        const float4 v4 = ld4(P.recVol + r4);
+        const vec3f org = v3(o4.x, o4.y, o4.z), dir = v3(d4.x, d4.y, d4.z);
+        const float T = volumeTransmittance(S, org, dir, v4.z, o4.w, 0.f, o4.w, rs);
+        if (T > 0.f) {
+          const float Tmis = volumeTransmittance(S, org, dir, v4.z, o4.w, 0.f, o4.w, rs);
+          L = L + (T * v3(r.x, r.y, r.z)) * misHeuristic(v4.x, v4.y * Tmis);
+        }
+      }
+      storeRng(P, p, rs);
+    } else {
+      for (uint32_t k = 0; k < nls; k++) {
+        const float4 r = ld4(P.recResult + 4 * ((size_t)p * F.maxRec + k));
+        if (r.w != 0.f)
+          L = L + v3(r.x, r.y, r.z);
+      }
     }
     float *c = P.contrib + 4 * (size_t)p;
     c[0] = c[0] + L.x;
@@ -983,8 +1106,15 @@ hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int resid
 
 hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
-      *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P), queue);
+  const DScene &sc = *reinterpret_cast<const DScene *>(S);
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  // the volume variant carries the sampler and transmittance code (more
+  // registers); surface-only worlds run the lean one
+  if (sc.numVolInsts > 0)
+    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
   return hipGetLastError();
 }
 
@@ -1022,10 +1152,22 @@ hipError_t hpt_shadow_trans(
   return hipGetLastError();
 }
 
-hipError_t hpt_resolve(const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
+hipError_t hpt_volume(const HptScene *S, const HptPaths *P, int queue, int blocks, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DFrame *>(F),
+  hipLaunchKernelGGL(k_volume, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
       *reinterpret_cast<const DPaths *>(P), queue);
+  return hipGetLastError();
+}
+
+hipError_t hpt_resolve(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
+{
+  const DScene &sc = *reinterpret_cast<const DScene *>(S);
+  const DFrame &f = *reinterpret_cast<const DFrame *>(F);
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  if (sc.numVolInsts > 0)
+    hipLaunchKernelGGL(k_resolve<true>, dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
+  else
+    hipLaunchKernelGGL(k_resolve<false>, dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
   return hipGetLastError();
 }
 

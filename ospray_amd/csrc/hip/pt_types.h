@@ -134,6 +134,41 @@ struct DLight
   float geoPdf;
 };
 
+// Volumes.  A structured-regular grid of vertex-centred voxels (x fastest)
+// at origin + i * spacing, its piecewise-linear transfer function and the
+// volumetric model's densityScale / anisotropy / id.  `macro` holds the
+// (min, max) voxel value of every macrocell of kMacro^3 cells, the
+// majorant grid of delta tracking; `maxOpacity` the transfer function's
+// kTfSubranges x kTfSubranges table of maximum opacity per value sub-range.
+constexpr int kMacro = 8;
+constexpr int kTfSubranges = 32;
+
+struct DVolume
+{
+  const float *voxels;
+  const float *macro; // 2 per macrocell: min, max
+  const float *tfColor; // vec3f per control point
+  const float *tfOpacity;
+  const float *maxOpacity; // [i * kTfSubranges + j]
+  int32_t dims[3], mdims[3];
+  float origin[3], spacing[3], upper[3];
+  int32_t numColors, numOpacities;
+  float tfLower, tfUpper;
+  float densityScale, anisotropy;
+  uint32_t userID;
+  int32_t pad;
+};
+
+// one (instance, volume) pair: rays meet the volume intervals in the order
+// of this list (instance order, then the group's volume order)
+struct DVolInst
+{
+  int32_t inst, vol, local, pad; // instance, DVolume, index within the group
+};
+
+// a volume collision in the hit record: inst field = kVolumeHit | pair index
+constexpr uint32_t kVolumeHit = 0x40000000u;
+
 struct DScene
 {
   const DNode4 *nodes;
@@ -146,11 +181,14 @@ struct DScene
   const DLight *lights;
   const float *lightsCDF;
   const float *geoCDF; // primitive CDFs of the geometry lights
+  const DVolume *volumes;
+  const DVolInst *volInsts;
   int32_t tlasRoot;
   int32_t numInstances;
   int32_t numLights;
   int32_t numGeoLights;
   int32_t numMaterials;
+  int32_t numVolInsts; // > 0: volumes in the scene (FFO_VOLUME_IN_SCENE)
 };
 
 // Path state of one pass, structure of arrays indexed by path slot.
@@ -163,15 +201,16 @@ struct DPaths
   uint32_t *ld; // 4: revIndex, scramble, baseIndexLight, flags
   float *lastP; // 4: last vertex P.xyz, pdf_w
   float *lastNs; // 4: last vertex Ns.xyz, numLightSamples (bits)
-  float *hitA; // 4: t, u, v, inst (bits)
-  float *hitB; // 4: Ng.xyz (object space), model (bits)
+  float *hitA; // 4: t, u, v, inst (bits); volume collision: t, 0, 0, kVolumeHit | pair
+  float *hitB; // 4: Ng.xyz (object space), model (bits); volume collision: albedo.xyz, 0
   uint32_t *hitPrim; // prim | half << 31
   uint32_t *ids; // 4: primID, geomID, instID, medium
   float *aux; // 16: position.xyz, firstNormal.xyz, normal.xyz, albedo.xyz, pad
   uint32_t *neeN; // light samples of the vertex shaded this bounce (0: none)
+  uint32_t *rng; // 4: PCG32 state (lo, hi), stream, (unused): volume sampling
   // NEE vertex record, written by k_shade at a smooth vertex
   float *vtxP; // 4: P.xyz, epsilon
-  float *vtxNg; // 4: Ng.xyz, material (bits)
+  float *vtxNg; // 4: Ng.xyz, material (bits; volume vertex: -2 - pair)
   float *vtxNs; // 4: Ns.xyz, medium (bits)
   float *vtxThr; // 4: throughput.xyz, light samples (bits)
   float *vtxWo; // 4: wo.xyz, (unused)
@@ -180,7 +219,10 @@ struct DPaths
   float *recOrg; // 4: org.xyz, tmax
   float *recDir; // 4: dir.xyz, mis weight
   float *recContrib; // 4: unshaded contribution.xyz, medium (bits)
-  float *recResult; // 4: L.xyz, valid flag
+  float *recResult; // 4: L.xyz, valid flag (with volumes: light contribution
+                    // before transmittance and MIS, valid flag)
+  float *recVol; // 4: light pdf, BSDF pdf, t0 of the last shadow segment,
+                 // (unused): the volume transmittance of k_resolve
   // queues and counters
   uint32_t *queue[2]; // extension queues (ping-pong)
   uint32_t *shadowQueue;
@@ -258,6 +300,7 @@ struct DFrame
   // adaptive accumulation: > 0 renders only the tasks whose taskVariance
   // exceeds it
   float adaptiveThreshold;
+  int32_t hasVolumes; // the paths carry a volume sampler (P.rng)
 };
 
 } // namespace hpt

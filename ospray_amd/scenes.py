@@ -335,11 +335,31 @@ def to_ospray(scene: Scene, renderer_params: Optional[dict] = None) -> OspScene:
         if m.user_id is not None:
             p["id"] = A.UInt(m.user_id)
         model_h.append(A.new("model", params=p))
+    vmodel_h = []
+    for v in scene.volumes:
+        vol = A.new("volume", "structuredRegular",
+                    {"data": A.shared_data3d(v.data), "gridOrigin": tuple(float(x) for x in v.origin),
+                     "gridSpacing": tuple(float(x) for x in v.spacing)})
+        tf = A.new("transfer_function", "piecewiseLinear",
+                   {"color": A.shared_data(np.ascontiguousarray(v.tf_color, np.float32).reshape(-1, 3)),
+                    "opacity": A.shared_data(np.ascontiguousarray(v.tf_opacity, np.float32).reshape(-1)),
+                    "value": A.Box1f(*v.tf_range)})
+        p = {"volume": vol, "transferFunction": tf, "densityScale": float(v.density_scale),
+             "anisotropy": float(v.anisotropy)}
+        if v.user_id is not None:
+            p["id"] = A.UInt(v.user_id)
+        vmodel_h.append(A.new("volumetric_model", params=p))
+        keep += [vol, tf]
     group_h = []
-    for grp in scene.groups:
-        arr = A.object_data([model_h[i] for i in grp], K.OSP_GEOMETRIC_MODEL)
-        group_h.append(A.new("group", params={"geometry": arr}))
-        keep.append(arr)
+    for gi, grp in enumerate(scene.groups):
+        params = {}
+        if grp:
+            params["geometry"] = A.object_data([model_h[i] for i in grp], K.OSP_GEOMETRIC_MODEL)
+        vols = scene.group_volumes[gi] if gi < len(scene.group_volumes) else []
+        if vols:
+            params["volume"] = A.object_data([vmodel_h[i] for i in vols], K.OSP_VOLUMETRIC_MODEL)
+        group_h.append(A.new("group", params=params))
+        keep += list(params.values())
     inst_h = []
     for inst in scene.instances:
         p = {"group": group_h[inst.group], "transform": A.Affine3f(inst.xfm)}
@@ -360,5 +380,5 @@ def to_ospray(scene: Scene, renderer_params: Optional[dict] = None) -> OspScene:
     if renderer_params:
         rp.update(renderer_params)
     ren = A.new("renderer", "pathtracer", rp)
-    handles = keep + geo_h + mat_h + model_h + group_h + inst_h + light_h
+    handles = keep + geo_h + mat_h + model_h + vmodel_h + group_h + inst_h + light_h
     return OspScene(world, cam, ren, handles)

@@ -179,6 +179,73 @@ def test_geometry_lights_bit_exact(hip_device, geometry_lights):
     assert col[..., :3].max() > 1.0  # emitters seen directly
 
 
+def _volume(anisotropy=0.3, user_id=None, seed=3):
+    """A 24x22x20 random grid (box smoothed) with an 8-point transfer
+    function: non-cubic, so a swapped axis shows."""
+    rng = np.random.default_rng(seed)
+    d = rng.random((20, 22, 24)).astype(np.float32)
+    d = (d + np.roll(d, 1, 0) + np.roll(d, 1, 1) + np.roll(d, 1, 2)) * 0.25
+    col = rng.uniform(0.2, 1.0, (8, 3)).astype(np.float32)
+    op = (rng.random(8) * 0.5).astype(np.float32)
+    return scenes.Volume(d, col, op, origin=(-1.0, -1.0, -1.0), spacing=(0.1, 0.09, 0.11), tf_range=(0.15, 0.85),
+                         density_scale=1.5, anisotropy=anisotropy, user_id=user_id)
+
+
+def _add_volume_instance(sc, vol, xfm, inst_id=None):
+    if vol not in sc.volumes:
+        sc.volumes.append(vol)
+        sc.groups.append([])
+        sc.group_volumes = sc.group_volumes + [[] for _ in range(len(sc.groups) - 1 - len(sc.group_volumes))]
+        sc.group_volumes.append([len(sc.volumes) - 1])
+    gi = len(sc.groups) - 1
+    sc.instances.append(scenes.Instance(gi, xfm=xfm, user_id=inst_id))
+
+
+@pytest.mark.parametrize("anisotropy", [0.3, -0.6, 0.0])
+def test_volume_in_c0_bit_exact(hip_device, anisotropy):
+    """C0 with a scaled, translated heterogeneous volume around some of the
+    spheres: free paths, HG scattering, NEE and virtual-light transmittance."""
+    sc = scenes.c0(48, 48, spp=2, max_path_length=5)
+    _add_volume_instance(sc, _volume(anisotropy), (4, 0, 0, 0, 4, 0, 0, 0, 4, -1.0, -2.0, 0.0))
+    s, fb, ofb, _, _ = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    # some camera rays collide in the volume (model id = group-local 0 of the
+    # volume's instance)
+    inst = A.map_channel(fb, K.OSP_FB_ID_INSTANCE, sc.width, sc.height)
+    assert (inst == len(sc.instances) - 1).sum() > 20
+
+
+def test_two_volume_instances_with_ids_bit_exact(hip_device):
+    """The same volumetric model instanced twice (overlapping boxes, one
+    rotated), with model and instance ids."""
+    sc = scenes.c0(40, 40, spp=2, max_path_length=4)
+    vol = _volume(0.5, user_id=7, seed=4)
+    _add_volume_instance(sc, vol, (4, 0, 0, 0, 4, 0, 0, 0, 4, -1.0, -2.0, 0.0), inst_id=21)
+    c, sn = 0.8, 0.6
+    _add_volume_instance(sc, vol, (3 * c, 3 * sn, 0, -3 * sn, 3 * c, 0, 0, 0, 3, 1.0, 0.0, -2.0))
+    s, fb, ofb, _, _ = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    obj = A.map_channel(fb, K.OSP_FB_ID_OBJECT, sc.width, sc.height)
+    assert (obj == 7).sum() > 10
+
+
+def test_volume_only_world_bit_exact(hip_device):
+    """A world with no geometry: one volume lit by a quad and an ambient light."""
+    sc = scenes.c0(32, 32, spp=4, max_path_length=6)
+    sc.geometries, sc.models, sc.materials = [], [], []
+    sc.groups, sc.instances, sc.group_volumes = [], [], []
+    _add_volume_instance(sc, _volume(0.0, seed=5), (5, 0, 0, 0, 5, 0, 0, 0, 5, -2.0, -3.0, -2.0))
+    s, fb, ofb, _, _ = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    assert col[..., :3].max() > 0.0
+
+
 def test_pick(hip_device):
     sc = scenes.c0(64, 64)
     s = scenes.to_ospray(sc)
