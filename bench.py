@@ -19,6 +19,12 @@ API, the scene already committed and resident in HBM.  Prints one JSON line.
   cpu_baseline the CPU oracle (a C++ restatement of the reference path
               tracer) on a bounded sample of the same frame's rows, all host
               threads up to 16
+
+--workload c4 measures the mixed surface + volume scene instead (the C0
+spheres inside a 512^3 fBm density volume, 1280x720, 2 spp, maxPathLength
+10; scenes.c4()); its roofline line also covers the free-path kernel
+k_volume: 32 B per voxel sample (the trilinear footprint) + 8 B per
+macrocell interval + 80 B per queued ray (ray, hit record, sampler state).
 """
 from __future__ import annotations
 
@@ -35,6 +41,7 @@ sys.path.insert(0, ROOT)
 METRIC = "path samples/s at 1080p; plain vs neural-cache time-to-equal-RMSE"
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, PRIM_BYTES, RAY_BYTES = 128, 48, 56
+SAMPLE_BYTES, CELL_BYTES, VOL_RAY_BYTES = 32, 8, 80
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_pathtracer_1080p.md")
 
 
@@ -88,7 +95,8 @@ def cpu_baseline(sc, target_s: float) -> dict:
     log(f"oracle: {total_n} samples in {total_t:.1f} s on {nt} threads")
     return {"value": round(total_n / total_t, 1), "unit": "samples/s", "cores": nt, "kind": "port",
             "sample": f"{len(picks)} 8-row bands ({frames} whole frames + {len(picks) - frames * nbands} bands spread "
-                      f"over the frame) of the 1920x1080 4-spp frame: {total_n} path samples in {total_t:.1f} s; "
+                      f"over the frame) of the {sc.width}x{sc.height} {spp}-spp frame: {total_n} path samples in "
+                      f"{total_t:.1f} s; "
                       f"oracle C++ restatement of the reference path tracer, {nt} threads"}
 
 
@@ -113,9 +121,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample size (0: skip)")
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--spp", type=int, default=4)
+    ap.add_argument("--workload", choices=("c1", "c4"), default="c1",
+                    help="c1: the 1080p surface scene (default); c4: surfaces inside a 512^3 volume")
+    ap.add_argument("--width", type=int, default=None)
+    ap.add_argument("--height", type=int, default=None)
+    ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time the plain copy kernel on BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -139,8 +149,13 @@ def main() -> int:
         return 0
 
     t = time.perf_counter()
-    sc = scenes.c1(width=args.width, height=args.height, spp=args.spp)
-    ntris = sum(g.num_prims for g in sc.geometries[:1]) + len(sc.instances[1:]) * sc.geometries[1].num_prims
+    if args.workload == "c4":
+        args.width, args.height, args.spp = args.width or 1280, args.height or 720, args.spp or 2
+        sc = scenes.c4(width=args.width, height=args.height, spp=args.spp)
+    else:
+        args.width, args.height, args.spp = args.width or 1920, args.height or 1080, args.spp or 4
+        sc = scenes.c1(width=args.width, height=args.height, spp=args.spp)
+    ntris = sc.num_triangles
     log(f"scene generated in {time.perf_counter() - t:.1f} s ({ntris} instanced triangles)")
 
     A.init("hip", profileKernels=True)
@@ -183,6 +198,7 @@ def main() -> int:
     L.ospDeviceCommit(dev)
     frame()
     tr = A.traversal_stats()
+    vst = A.volume_stats()
     A.set_device_param(dev, "traversalStats", False)
     L.ospDeviceCommit(dev)
     col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
@@ -204,6 +220,16 @@ def main() -> int:
                    "bytes_per_frame": per_frame_bytes, "rays_per_frame": rays,
                    "nodes_per_ray": nodes / max(rays, 1), "prims_per_ray": prims / max(rays, 1),
                    "gbs": per_frame_bytes / (ms_frame * 1e-3) / 1e9}
+    if kstats.get("volume", (0, 0.0))[0]:
+        launches, ms = kstats["volume"]
+        samples_v, cells_v, tracks_v = vst[:3]
+        per_frame_bytes = SAMPLE_BYTES * samples_v + CELL_BYTES * cells_v + VOL_RAY_BYTES * tr[2]
+        ms_frame = ms / args.steps
+        kern["volume"] = {"ms_per_frame": ms_frame, "launches_per_frame": launches / args.steps,
+                          "bytes_per_frame": per_frame_bytes, "rays_per_frame": 0,
+                          "samples_per_frame": samples_v, "cells_per_frame": cells_v,
+                          "tracks_per_frame": tracks_v, "gbs": per_frame_bytes / (ms_frame * 1e-3) / 1e9}
+        log(f"volume counters: free path {vst[:3]}, transmittance {vst[3:]}")
     dom = max(kern, key=lambda k: kern[k]["ms_per_frame"])
     d = kern[dom]
     lpf = d["launches_per_frame"]
@@ -221,20 +247,31 @@ def main() -> int:
         cpu = cpu_baseline(sc, args.cpu_seconds)
 
     ms_step = elapsed / args.steps * 1e3
-    out = {"metric": METRIC, "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+    mpl = sc.renderer["maxPathLength"]
+    if args.workload == "c4":
+        workload = (f"surface+volume pathtracer {sc.width}x{sc.height} {args.spp}spp maxPathLength {mpl}, "
+                    "C0 spheres + ground in a 512^3 fBm density volume (8-point transfer function, opacity "
+                    "scale 0.5), delta tracking, quad + ambient light")
+    else:
+        workload = (f"plain pathtracer {sc.width}x{sc.height} {args.spp}spp maxPathLength {mpl}, "
+                    "fBm heightfield 1448^2 (4,187,618 tris) + 256 icosphere instances, sun + 4 quad lights")
+    out = {"metric": METRIC if args.workload == "c1" else "path samples/s, surface+volume scene",
+           "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"plain pathtracer {sc.width}x{sc.height} {args.spp}spp maxPathLength 8, "
-                                  "fBm heightfield 1448^2 (4,187,618 tris) + 256 icosphere instances, "
-                                  "sun + 4 quad lights",
-                      "width": sc.width, "height": sc.height, "spp": args.spp, "max_path_length": 8,
-                      "instanced_triangles": int(ntris), "device": name},
+           "config": {"workload": workload, "width": sc.width, "height": sc.height, "spp": args.spp,
+                      "max_path_length": mpl, "instanced_triangles": int(ntris), "device": name},
            "rays_per_s": round(rays_total * args.steps / elapsed, 1),
            "kernels_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats.items())},
            "traversal": {k: {"nodes_per_ray": round(v["nodes_per_ray"], 2), "prims_per_ray": round(v["prims_per_ray"], 2),
                              "rays_per_frame": v["rays_per_frame"], "gbs": round(v["gbs"], 1)}
-                         for k, v in kern.items()},
+                         for k, v in kern.items() if k != "volume"},
            "roofline": roofline, "cpu_baseline": cpu}
+    if "volume" in kern:
+        v = kern["volume"]
+        out["volume"] = {"free_path": {"samples": vst[0], "cells": vst[1], "tracks": vst[2],
+                                       "ms_per_frame": round(v["ms_per_frame"], 3), "gbs": round(v["gbs"], 1)},
+                         "transmittance": {"samples": vst[3], "cells": vst[4], "tracks": vst[5]}}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -33,6 +33,12 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void);
  * (out9[6..8]). */
 OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9]);
 
+/* Delta-tracking counters of the last frame rendered while "traversalStats"
+ * was set: voxel samples (tentative collisions), macrocell intervals and
+ * tracked segments of the free-path sampling (out6[0..2]) and of the
+ * transmittance estimates (out6[3..5]). */
+OSPRAY_INTERFACE int ospHipGetVolumeStats(uint64_t out6[6]);
+
 /* Closest hits of n rays (org.xyz, dir.xyz, t0, t1) against a committed
  * world, traced by the device's BVH kernels.  outTUV: t,u,v; outIDs:
  * instance index, model index within the group, primitive index

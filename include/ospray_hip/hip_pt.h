@@ -41,8 +41,9 @@ HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, i
 /* volumeSampleFreePath (volumes/VolumeSampler.ispc:96-199) of every ray of
  * extension queue `queue` after hpt_extend: delta tracking up to the
  * surface hit; a collision turns the hit record into a volume vertex.
- * Launched only for worlds with volumes. */
-HPT_API hipError_t hpt_volume(const HptScene *S, const HptPaths *P, int queue, int blocks, hipStream_t s);
+ * Launched only for worlds with volumes.  stats != 0 adds voxel samples,
+ * macrocell intervals and tracked segments to P->stats[9..11]. */
+HPT_API hipError_t hpt_volume(const HptScene *S, const HptPaths *P, int queue, int blocks, int stats, hipStream_t s);
 /* One samplePath bounce (PathSampler.ispc:143-407) for queue `queue`; the
  * NEE light samples of its smooth vertices are queued for hpt_nee. */
 HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);

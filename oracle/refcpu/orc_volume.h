@@ -28,7 +28,7 @@
 
 namespace orc {
 
-static constexpr int kMacro = 8; // cells per macrocell edge
+static constexpr int kMacro = 16; // cells per macrocell edge
 static constexpr int kTfSubranges = 32; // PRECOMPUTED_OPACITY_SUBRANGE_COUNT
 
 inline float lerpf(float t, float a, float b) { return (1.f - t) * a + t * b; }

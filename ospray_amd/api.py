@@ -322,6 +322,15 @@ def traversal_stats() -> tuple:
     return tuple(int(x) for x in out)
 
 
+def volume_stats() -> tuple:
+    """(voxel samples, macrocell intervals, tracked segments) of the
+    free-path sampling, then of the transmittance estimates, of the last
+    frame rendered with "traversalStats"."""
+    out = (C.c_uint64 * 6)()
+    K.ext().ospHipGetVolumeStats(out)
+    return tuple(int(x) for x in out)
+
+
 def trace_rays(world: Handle, rays8: np.ndarray):
     """Closest hits of rays (N x 8: org, dir, t0, t1) on the GPU.
     Returns (tuv N x 3, ids N x 3 [instance, model, primitive], (nodes, prims))."""

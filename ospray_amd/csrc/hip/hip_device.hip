@@ -27,6 +27,7 @@
 #include "ospray_hip/hip_pt.h"
 #include "pt_sampler.h"
 #include "pt_shade.h"
+#include "pt_volume.h"
 #include "pt_types.h"
 
 using namespace ospcore;
@@ -359,7 +360,8 @@ struct Volume : public ManagedObject
 {
   int32_t dims[3] = {0, 0, 0}, mdims[3] = {0, 0, 0};
   float origin[3] = {0, 0, 0}, spacing[3] = {1, 1, 1};
-  DevBuf dVoxels, dMacro;
+  DevBuf dVoxels;
+  std::vector<float> macro; // (min, max) per macrocell
   uint64_t version = 0;
   Volume(const std::string &t)
   {
@@ -400,7 +402,7 @@ struct Volume : public ManagedObject
     // macrocell m covers voxels [m * kMacro, (m + 1) * kMacro] per axis
     for (int a = 0; a < 3; a++)
       mdims[a] = (dims[a] - 1 + hpt::kMacro - 1) / hpt::kMacro;
-    std::vector<float> macro(2 * (size_t)mdims[0] * mdims[1] * mdims[2]);
+    macro.assign(2 * (size_t)mdims[0] * mdims[1] * mdims[2], 0.f);
     for (int mz = 0; mz < mdims[2]; mz++)
       for (int my = 0; my < mdims[1]; my++)
         for (int mx = 0; mx < mdims[0]; mx++) {
@@ -418,7 +420,6 @@ struct Volume : public ManagedObject
           macro[2 * m + 1] = hi;
         }
     dVoxels.upload(vox);
-    dMacro.upload(macro);
     version++;
   }
 };
@@ -432,7 +433,7 @@ struct TransferFunction : public ManagedObject
   std::vector<float> color, opacity;
   float valueRange[2] = {0.f, 1.f};
   float maxOpacity[hpt::kTfSubranges * hpt::kTfSubranges];
-  DevBuf dColor, dOpacity, dMaxOpacity;
+  DevBuf dColor, dOpacity;
   TransferFunction(const std::string &t)
   {
     managedType = OSP_TRANSFER_FUNCTION;
@@ -477,7 +478,6 @@ struct TransferFunction : public ManagedObject
         at(i, j) = 0.f;
     dColor.upload(color);
     dOpacity.upload(opacity);
-    dMaxOpacity.upload(std::vector<float>(maxOpacity, maxOpacity + hpt::kTfSubranges * hpt::kTfSubranges));
   }
 };
 
@@ -488,6 +488,7 @@ struct VolumetricModel : public ManagedObject
   Ref<TransferFunction> tf;
   float densityScale = 1.f, anisotropy = 0.f;
   uint32_t userID = hpt::kInvalidID;
+  DevBuf dMajorant; // densityScale * max opacity per macrocell
   VolumetricModel() { managedType = OSP_VOLUMETRIC_MODEL; }
   std::string toString() const override { return "ospray::VolumetricModel"; }
   void commit() override
@@ -503,16 +504,28 @@ struct VolumetricModel : public ManagedObject
     (void)getFloat("gradientShadingScale", 0.f);
     userID = getUInt("id", hpt::kInvalidID);
   }
-  hpt::DVolume device() const
+  // the majorant grid of the current volume, transfer function and
+  // densityScale (built when the world is committed)
+  void buildMajorant()
   {
+    const Volume &V = *volume.p;
+    const TransferFunction &T = *tf.p;
+    const size_t n = V.macro.size() / 2;
+    std::vector<float> mj(n);
+    for (size_t m = 0; m < n; m++)
+      mj[m] = densityScale * hpt::tfMaxOpacity(T.maxOpacity, T.valueRange[0], T.valueRange[1], V.macro[2 * m], V.macro[2 * m + 1]);
+    dMajorant.upload(mj);
+  }
+  hpt::DVolume device()
+  {
+    buildMajorant();
     hpt::DVolume v;
     std::memset(&v, 0, sizeof(v));
     const Volume &V = *volume.p;
     v.voxels = V.dVoxels.as<const float>();
-    v.macro = V.dMacro.as<const float>();
+    v.majorant = dMajorant.as<const float>();
     v.tfColor = tf->dColor.as<const float>();
     v.tfOpacity = tf->dOpacity.as<const float>();
-    v.maxOpacity = tf->dMaxOpacity.as<const float>();
     for (int a = 0; a < 3; a++) {
       v.dims[a] = V.dims[a];
       v.mdims[a] = V.mdims[a];
@@ -1821,6 +1834,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     F.adaptiveThreshold = r->errorThreshold;
   F.maxRec = (int)std::max<uint32_t>(1u, F.numFirstLightSamples);
   F.hasVolumes = sceneF.numVolInsts > 0 ? 1 : 0;
+  F.stats = travStats ? 1 : 0;
 
   if (!finishedAccum) {
     const bool aux = F.accAux != nullptr;
@@ -1846,7 +1860,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         launch("reset", [&]() { return hpt_reset_counters(P, q, stream); });
         launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
         if (F.hasVolumes)
-          launch("volume", [&]() { return hpt_volume(S, P, q, shadeBlocks, stream); });
+          launch("volume", [&]() { return hpt_volume(S, P, q, shadeBlocks, travStats ? 1 : 0, stream); });
         launch("shade", [&]() { return hpt_shade(S, Fp, P, q, shadeBlocks, stream); });
         if (F.numFirstLightSamples > 0) {
           launch("nee", [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
@@ -2006,6 +2020,25 @@ OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9])
     HIP_CHECK(hipMemcpy(s, d->stats.ptr, sizeof(s), hipMemcpyDeviceToHost));
     for (int k = 0; k < 9; k++)
       out9[k] = s[k];
+  } catch (const std::exception &) {
+    return -2;
+  }
+  return 0;
+}
+
+OSPRAY_INTERFACE int ospHipGetVolumeStats(uint64_t out6[6])
+{
+  HipDevice *d = current();
+  if (!d)
+    return -1;
+  if (!d->stream)
+    return -2;
+  try {
+    unsigned long long s[15];
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    HIP_CHECK(hipMemcpy(s, d->stats.ptr, sizeof(s), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; k++)
+      out6[k] = s[9 + k];
   } catch (const std::exception &) {
     return -2;
   }

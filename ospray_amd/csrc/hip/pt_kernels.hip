@@ -284,10 +284,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
 // intervals up to the hit.  A collision replaces the hit record by a
 // volume vertex (t, pair index, albedo).  The ray range of the clipping
 // pass is [t0, inf) (no clipping geometry).
-__global__ __launch_bounds__(256) void k_volume(DScene S, DPaths P, int qin)
+__global__ __launch_bounds__(256) void k_volume(DScene S, DPaths P, int qin, int stats)
 {
   const int n = P.counters[2 * qin];
-  for (int i = globalId(); i < n; i += gridStride()) {
+  VolCount cnt;
+  for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
+    const int i = w0 + laneId();
+    if (i >= n)
+      continue;
     const uint32_t p = P.queue[qin][i];
     const size_t p4 = 4 * (size_t)p;
     const float4 o4 = ld4(P.org + p4);
@@ -298,12 +302,17 @@ __global__ __launch_bounds__(256) void k_volume(DScene S, DPaths P, int qin)
     int pair;
     vec3f albedo;
     const float dist = volumeFreePath(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), o4.w, surface ? hA.x : kInf,
-        surface, o4.w, kInf, rs, pair, albedo);
+        surface, o4.w, kInf, rs, pair, albedo, cnt);
     storeRng(P, p, rs);
     if (dist < kInf) {
       st4(P.hitA + p4, dist, 0.f, 0.f, __uint_as_float(kVolumeHit | (uint32_t)pair));
       st4(P.hitB + p4, albedo.x, albedo.y, albedo.z, 0.f);
     }
+  }
+  if (stats) {
+    waveAdd(&P.stats[9], cnt.samples);
+    waveAdd(&P.stats[10], cnt.cells);
+    waveAdd(&P.stats[11], cnt.tracks);
   }
 }
 
@@ -317,6 +326,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
 {
   const int n = P.counters[2 * qin];
   const int qout = qin ^ 1;
+  VolCount vcnt; // transmittance work of the virtual-light MIS weights
   for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
     const int i = w0 + laneId();
     bool cont = false;
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
                 // (VirtualLight.ispc:76-99)
                 float T = 1.f;
                 if (volumes)
-                  T = volumeTransmittance(S, lastP, rdir, lower, upper, lower, upper, rs);
+                  T = volumeTransmittance(S, lastP, rdir, lower, upper, lower, upper, rs, vcnt);
                 mis = misHeuristic(lastPdf * T, selectionPDF * le.pdf);
               }
               L = L + thr * le.radiance * mis;
@@ -570,6 +580,11 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
       P.queue[qout][so] = p;
     for (uint32_t k = 0; k < nNee; k++)
       P.neeSampleQueue[first + k] = p * (uint32_t)F.maxRec + k;
+  }
+  if (VOL && F.stats) {
+    waveAdd(&P.stats[12], vcnt.samples);
+    waveAdd(&P.stats[13], vcnt.cells);
+    waveAdd(&P.stats[14], vcnt.tracks);
   }
 }
 
@@ -764,6 +779,7 @@ template <bool VOL>
 __global__ __launch_bounds__(256) void k_resolve(DScene S, DFrame F, DPaths P, int qin)
 {
   const int n = P.counters[2 * qin];
+  VolCount vcnt;
   for (int i = globalId(); i < n; i += gridStride()) {
     const uint32_t p = P.queue[qin][i];
     const uint32_t nls = P.neeN[p];
@@ -781,9 +797,9 @@ __global__ __launch_bounds__(256) void k_resolve(DScene S, DFrame F, DPaths P, i
         const float4 d4 = ld4(P.recDir + r4);
         const float4 v4 = ld4(P.recVol + r4);
         const vec3f org = v3(o4.x, o4.y, o4.z), dir = v3(d4.x, d4.y, d4.z);
-        const float T = volumeTransmittance(S, org, dir, v4.z, o4.w, 0.f, o4.w, rs);
+        const float T = volumeTransmittance(S, org, dir, v4.z, o4.w, 0.f, o4.w, rs, vcnt);
         if (T > 0.f) {
-          const float Tmis = volumeTransmittance(S, org, dir, v4.z, o4.w, 0.f, o4.w, rs);
+          const float Tmis = volumeTransmittance(S, org, dir, v4.z, o4.w, 0.f, o4.w, rs, vcnt);
           L = L + (T * v3(r.x, r.y, r.z)) * misHeuristic(v4.x, v4.y * Tmis);
         }
       }
@@ -799,6 +815,11 @@ __global__ __launch_bounds__(256) void k_resolve(DScene S, DFrame F, DPaths P, i
     c[0] = c[0] + L.x;
     c[1] = c[1] + L.y;
     c[2] = c[2] + L.z;
+  }
+  if (VOL && F.stats) {
+    waveAdd(&P.stats[12], vcnt.samples);
+    waveAdd(&P.stats[13], vcnt.cells);
+    waveAdd(&P.stats[14], vcnt.tracks);
   }
 }
 
@@ -1152,10 +1173,10 @@ hipError_t hpt_shadow_trans(
   return hipGetLastError();
 }
 
-hipError_t hpt_volume(const HptScene *S, const HptPaths *P, int queue, int blocks, hipStream_t s)
+hipError_t hpt_volume(const HptScene *S, const HptPaths *P, int queue, int blocks, int stats, hipStream_t s)
 {
   hipLaunchKernelGGL(k_volume, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
-      *reinterpret_cast<const DPaths *>(P), queue);
+      *reinterpret_cast<const DPaths *>(P), queue, stats);
   return hipGetLastError();
 }
 

@@ -136,20 +136,19 @@ struct DLight
 
 // Volumes.  A structured-regular grid of vertex-centred voxels (x fastest)
 // at origin + i * spacing, its piecewise-linear transfer function and the
-// volumetric model's densityScale / anisotropy / id.  `macro` holds the
-// (min, max) voxel value of every macrocell of kMacro^3 cells, the
-// majorant grid of delta tracking; `maxOpacity` the transfer function's
-// kTfSubranges x kTfSubranges table of maximum opacity per value sub-range.
-constexpr int kMacro = 8;
+// volumetric model's densityScale / anisotropy / id.  `majorant` holds, per
+// macrocell of kMacro^3 cells, densityScale times the transfer function's
+// maximum opacity over the macrocell's (min, max) voxel values: the
+// majorant of delta tracking, precomputed per volumetric model.
+constexpr int kMacro = 16;
 constexpr int kTfSubranges = 32;
 
 struct DVolume
 {
   const float *voxels;
-  const float *macro; // 2 per macrocell: min, max
+  const float *majorant; // per macrocell
   const float *tfColor; // vec3f per control point
   const float *tfOpacity;
-  const float *maxOpacity; // [i * kTfSubranges + j]
   int32_t dims[3], mdims[3];
   float origin[3], spacing[3], upper[3];
   int32_t numColors, numOpacities;
@@ -234,7 +233,9 @@ struct DPaths
   int32_t *counters;
   uint32_t *overflow; // traversal stack overflow, per resident thread
   // node visits, primitive tests, traversals: [0..2] extension rays, [3..5]
-  // shadow occlusion pass, [6..8] shadow transparency pass
+  // shadow occlusion pass, [6..8] shadow transparency pass; delta tracking
+  // voxel samples, macrocell intervals, tracked segments: [9..11] free
+  // paths, [12..14] transmittance estimates
   unsigned long long *stats;
 };
 
@@ -301,6 +302,7 @@ struct DFrame
   // exceeds it
   float adaptiveThreshold;
   int32_t hasVolumes; // the paths carry a volume sampler (P.rng)
+  int32_t stats; // count traversal and delta-tracking work into P.stats
 };
 
 } // namespace hpt

@@ -10,7 +10,7 @@
 // a sample is the trilinear interpolation of the vertex-centred voxels
 // (NaN outside the box) and the intervals of a ray are its runs through
 // the macrocells of kMacro^3 cells, each with the macrocell's (min, max)
-// voxel value as value range.
+// voxel value as value range (its majorant precomputed per model).
 #pragma once
 
 #include "pt_shade.h"
@@ -129,14 +129,15 @@ HPT_HD vec3f tfColor(const DVolume &v, float u)
 }
 
 // LinearTransferFunction_getMaxOpacity (LinearTransferFunction.ispc:45-58)
-HPT_HD float tfMaxOpacity(const DVolume &v, float lower, float upper)
+// over a kTfSubranges^2 table (used on the host to build the majorant grid)
+HPT_HD float tfMaxOpacity(const float *maxOpacity, float tfLower, float tfUpper, float lower, float upper)
 {
   const int maxDim = kTfSubranges - 1;
-  const float denom = v.tfUpper - v.tfLower;
-  const float a = (lower - v.tfLower) / denom, b = (upper - v.tfLower) / denom;
+  const float denom = tfUpper - tfLower;
+  const float a = (lower - tfLower) / denom, b = (upper - tfLower) / denom;
   const int i = (int)floorf((!(a > 0.f) ? 0.f : (a > 1.f ? 1.f : a)) * (float)maxDim); // NaN -> 0
   const int j = (int)ceilf((!(b > 0.f) ? 0.f : (b > 1.f ? 1.f : b)) * (float)maxDim);
-  return v.maxOpacity[(i < maxDim ? i : maxDim) * kTfSubranges + (j < maxDim ? j : maxDim)];
+  return maxOpacity[(i < maxDim ? i : maxDim) * kTfSubranges + (j < maxDim ? j : maxDim)];
 }
 
 // the ray's parameter range inside the volume's box, clipped to [t0, t1]
@@ -159,13 +160,22 @@ HPT_HD bool volumeBox(const DVolume &v, vec3f o, vec3f d, float t0, float t1, fl
 // interval then counts as passed)
 constexpr int kMaxDeltaSteps = 1 << 16;
 
+// work counters of delta tracking: voxel samples (tentative collisions),
+// macrocell intervals entered, tracked segments
+struct VolCount
+{
+  uint32_t samples = 0, cells = 0, tracks = 0;
+};
+
 // delta_tracking (VolumeSampler.ispc:13-92) over [t0, t1] (inside the box),
 // walking the macrocells with a 3D DDA: the distance of the first real
 // collision (inf if none) and its albedo
-HPT_HD float deltaTracking(const DVolume &v, vec3f o, vec3f d, float t0, float t1, RandomSampler &rs, vec3f &albedo)
+HPT_HD float deltaTracking(
+    const DVolume &v, vec3f o, vec3f d, float t0, float t1, RandomSampler &rs, vec3f &albedo, VolCount &cnt)
 {
   if (!(t0 < t1))
     return kInf;
+  cnt.tracks++;
   const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
   const float ii[3] = {safeRcp(d.x), safeRcp(d.y), safeRcp(d.z)};
   float msz[3];
@@ -187,7 +197,8 @@ HPT_HD float deltaTracking(const DVolume &v, vec3f o, vec3f d, float t0, float t
     if (tb > t) {
       // one interval [t, tb] of macrocell m
       const size_t mi = ((size_t)m[2] * (size_t)v.mdims[1] + (size_t)m[1]) * (size_t)v.mdims[0] + (size_t)m[0];
-      const float mu_max = v.densityScale * tfMaxOpacity(v, v.macro[2 * mi], v.macro[2 * mi + 1]);
+      cnt.cells++;
+      const float mu_max = v.majorant[mi];
       if (mu_max > 0.f) {
         float tt = t;
         for (int k = 0; k < kMaxDeltaSteps; k++) {
@@ -197,6 +208,7 @@ HPT_HD float deltaTracking(const DVolume &v, vec3f o, vec3f d, float t0, float t
           if (tt > tb)
             break;
           xi = randomFloat(rs);
+          cnt.samples++;
           const vec3f p = o + tt * d;
           const float smp = volumeSample(v, p);
           if (smp != smp)
@@ -231,7 +243,7 @@ HPT_HD void toVolumeSpace(const DInstance &I, vec3f worg, vec3f wdir, vec3f &o, 
 // pass's ray range.  The nearest collision wins and clips the later
 // intervals.  Returns its distance (inf: none), pair index and albedo.
 HPT_HD float volumeFreePath(const DScene &S, vec3f org, vec3f dir, float t0, float tTrace, bool surfaceHit, float r0,
-    float r1, RandomSampler &rs, int &pair, vec3f &albedo)
+    float r1, RandomSampler &rs, int &pair, vec3f &albedo, VolCount &cnt)
 {
   float minDist = kInf;
   float tClip = tTrace;
@@ -253,7 +265,7 @@ HPT_HD float volumeFreePath(const DScene &S, vec3f org, vec3f dir, float t0, flo
     if (hi < r0 || r1 < lo)
       continue;
     vec3f alb;
-    const float dist = deltaTracking(v, o, d, maxf(r0, lo), minf(r1, hi), rs, alb);
+    const float dist = deltaTracking(v, o, d, maxf(r0, lo), minf(r1, hi), rs, alb, cnt);
     if (dist < kInf && dist < minDist) {
       minDist = dist;
       pair = j;
@@ -268,7 +280,7 @@ HPT_HD float volumeFreePath(const DScene &S, vec3f org, vec3f dir, float t0, flo
 // volumeTransmittance (VolumeSampler.ispc:201-277): 1 if delta tracking
 // passes every volume interval of [t0, t1] (within [r0, r1]), else 0
 HPT_HD float volumeTransmittance(
-    const DScene &S, vec3f org, vec3f dir, float t0, float t1, float r0, float r1, RandomSampler &rs)
+    const DScene &S, vec3f org, vec3f dir, float t0, float t1, float r0, float r1, RandomSampler &rs, VolCount &cnt)
 {
   float T = 1.f;
   for (int j = 0; j < S.numVolInsts && T > 0.f; j++) {
@@ -283,7 +295,7 @@ HPT_HD float volumeTransmittance(
       continue;
     const float b = minf(r1, hi);
     vec3f alb;
-    const float dist = deltaTracking(v, o, d, maxf(r0, lo), b, rs, alb);
+    const float dist = deltaTracking(v, o, d, maxf(r0, lo), b, rs, alb, cnt);
     T = T * (dist < b ? 0.f : 1.f);
   }
   return T;

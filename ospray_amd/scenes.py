@@ -7,6 +7,8 @@ pcg32_srandom(seed, 54)); uniform floats are (u32 >> 8) * 2^-24.
         radius U[0.2,0.6], centre U[-5,5]^3 and Lambert kd U[0.1,0.9]^3 plus
         a 20x20 ground quad at y=-6; one quad light (intensity 10) and an
         ambient light (0.2).  Seed 42.
+  c4(): 1280x720, 2 spp, maxPathLength 10; the C0 spheres inside a 512^3
+        fBm density volume (seed 45) with an 8-point transfer function.
   c1(): 1920x1080, 4 spp, maxPathLength 8; 1448^2 fBm heightfield over
         [-50,50]^2 (4,187,618 triangles, 5 octaves, amplitude 2) plus 256
         instances of one subdivision-4 icosphere; 60/25/15 % diffuse / metal
@@ -19,6 +21,7 @@ it through the C API and returns the committed handles.
 from __future__ import annotations
 
 import dataclasses
+import os
 import math
 from typing import Optional
 
@@ -47,6 +50,79 @@ class Pcg32:
     def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
         u = (self.next_u32() >> 8) * (1.0 / 16777216.0)
         return lo + (hi - lo) * u
+
+
+def pcg32_draws(rng: Pcg32, n: int) -> np.ndarray:
+    """The next n outputs of `rng` as a uint32 array (the same values as n
+    calls of rng.next_u32(), which advances rng accordingly): the LCG states
+    s_k = a^k s_0 + c (a^(k-1) + ... + 1) are formed by doubling, so the
+    whole run costs O(log n) vector steps."""
+    a, c = np.uint64(Pcg32.MULT), np.uint64(rng.inc)
+    A = np.ones(1, np.uint64)  # a^k for k in [0, m)
+    Cs = np.zeros(1, np.uint64)  # c (a^(k-1) + ... + 1)
+    am, cm = a, c  # a^m, c (a^(m-1) + ... + 1) for the current block size m
+    with np.errstate(over="ignore"):
+        while A.size < n:
+            A = np.concatenate([A, A * am])
+            Cs = np.concatenate([Cs, Cs * am + cm])
+            cm = cm * am + cm
+            am = am * am
+        A, Cs = A[:n], Cs[:n]
+        old = A * np.uint64(rng.state) + Cs
+        xorshifted = (((old >> np.uint64(18)) ^ old) >> np.uint64(27)) & np.uint64(0xFFFFFFFF)
+        rot = old >> np.uint64(59)
+        out = ((xorshifted >> rot) | (xorshifted << ((np.uint64(32) - rot) & np.uint64(31)))) & np.uint64(0xFFFFFFFF)
+        last = old[-1] if n else np.uint64(rng.state)
+        if n:
+            rng.state = int(last * a + c) & _MASK64
+    return out.astype(np.uint32)
+
+
+def _fade_np(t):
+    return t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+
+
+def value_fbm3(n: int, rng: Pcg32, octaves: int = 6, base_period: int = 128) -> np.ndarray:
+    """n^3 fBm of seeded lattice value noise, float32 (nz, ny, nx).
+
+    Octave o has lattice period base_period / 2^o voxels, gain 0.5^o and
+    lattice values 2u - 1 (u from the PCG32 draws, z-major then y then x);
+    between lattice points the noise is interpolated per axis with the
+    quintic fade 6t^5 - 15t^4 + 10t^3, separably (x, then y, then z), in
+    float32.  Slabs of z are summed in parallel threads."""
+    import concurrent.futures as cf
+
+    i = np.arange(n)
+    planes = []  # per octave: (lattice x/y-interpolated planes, period, amplitude)
+    amp = 1.0
+    for o in range(octaves):
+        per = max(1, base_period >> o)
+        L = (n - 1) // per + 2
+        u = pcg32_draws(rng, L * L * L).astype(np.float64) * (1.0 / 4294967296.0)
+        lat = (2.0 * u - 1.0).astype(np.float32).reshape(L, L, L)
+        c = i // per
+        w = _fade_np((i % per) / per).astype(np.float32)
+        v = lat[:, :, c] * (1.0 - w) + lat[:, :, c + 1] * w  # x
+        vy = np.empty((L, n, n), np.float32)
+        for k in range(L):  # y
+            vy[k] = v[k, c, :] * (1.0 - w)[:, None] + v[k, c + 1, :] * w[:, None]
+        planes.append((vy, per, np.float32(amp)))
+        amp *= 0.5
+    total = np.empty((n, n, n), np.float32)
+
+    def slab(z0, z1):
+        acc = np.zeros((z1 - z0, n, n), np.float32)
+        for vy, per, a in planes:
+            for z in range(z0, z1):
+                k, f = divmod(z, per)
+                wz = np.float32(_fade_np(f / per))
+                acc[z - z0] += a * (vy[k] * (np.float32(1.0) - wz) + vy[k + 1] * wz)
+        total[z0:z1] = acc
+
+    step = 16
+    with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(lambda z0: slab(z0, min(n, z0 + step)), range(0, n, step)))
+    return total
 
 
 @dataclasses.dataclass
@@ -291,6 +367,28 @@ def c1(width: int = 1920, height: int = 1080, spp: int = 4, max_path_length: int
                  camera={"position": pos, "direction": _look_at(pos), "up": (0.0, 1.0, 0.0), "fovy": 40.0,
                          "aspect": width / height},
                  renderer={"pixelSamples": spp, "maxPathLength": max_path_length})
+
+
+def c4(width: int = 1280, height: int = 720, spp: int = 2, max_path_length: int = 10, seed: int = 45,
+       grid: int = 512, base_period: int = 128) -> Scene:
+    """The C0 spheres, ground and lights embedded in a grid^3 fBm density
+    volume over [-6, 6]^3: value-noise fBm (6 octaves, lattice period
+    base_period voxels halving per octave, seed 45) mapped to
+    clamp(0.5 + fBm, 0, 1); piecewise-linear transfer function of 8 points
+    over [0, 1] with seeded colours U[0.3, 1]^3 and opacity 0.5 * k / 7
+    (opacity scale 0.5), densityScale 1, isotropic scattering."""
+    sc = c0(width, height, spp=spp, max_path_length=max_path_length)
+    rng = Pcg32(seed)
+    dens = np.clip(0.5 + value_fbm3(grid, rng, 6, base_period), 0.0, 1.0).astype(np.float32)
+    col = np.array([[rng.uniform(0.3, 1.0) for _ in range(3)] for _ in range(8)], np.float32)
+    op = (0.5 * np.arange(8) / 7.0).astype(np.float32)
+    sp = 12.0 / (grid - 1)
+    sc.volumes = [Volume(dens, col, op, origin=(-6.0, -6.0, -6.0), spacing=(sp, sp, sp))]
+    sc.groups.append([])
+    sc.group_volumes = [[], [0]]
+    sc.instances.append(Instance(1))
+    sc.name = "c4"
+    return sc
 
 
 # ---- OSPRay construction ---------------------------------------------------------------------

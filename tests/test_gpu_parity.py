@@ -246,6 +246,36 @@ def test_volume_only_world_bit_exact(hip_device):
     assert col[..., :3].max() > 0.0
 
 
+def test_c4_small_bit_exact(hip_device):
+    """The surface+volume configuration at reduced size: 64^3 fBm grid,
+    96x54, 2 spp, maxPathLength 10."""
+    sc = scenes.c4(96, 54, spp=2, grid=64, base_period=16)
+    s, fb, ofb, _, _ = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+
+
+def test_c4_full_size_band_bit_exact(hip_device):
+    """The full surface+volume configuration (512^3 grid, 1280x720, 2 spp):
+    the GPU frame's rows 352-367 against the oracle's."""
+    sc = scenes.c4()
+    s = scenes.to_ospray(sc)
+    ch = K.OSP_FB_COLOR | K.OSP_FB_ACCUM | K.OSP_FB_DEPTH | K.OSP_FB_ID_OBJECT | K.OSP_FB_ID_INSTANCE
+    fb = A.framebuffer(sc.width, sc.height, K.OSP_FB_RGBA32F, ch)
+    A.render(fb, s.renderer, s.camera, s.world)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.isfinite(col).all()
+    rows = (352, 368)
+    ofb, _, _ = orc.render_scene(sc, frames=1, channels=ch, rows=rows)
+    o = ofb.channel(K.OSP_FB_COLOR)
+    assert np.array_equal(col[rows[0]:rows[1]], o[rows[0]:rows[1]])
+    for c in (K.OSP_FB_DEPTH, K.OSP_FB_ID_OBJECT, K.OSP_FB_ID_INSTANCE):
+        g = A.map_channel(fb, c, sc.width, sc.height)[rows[0]:rows[1]]
+        assert np.array_equal(g, ofb.channel(c)[rows[0]:rows[1]], equal_nan=True), c
+    assert (A.map_channel(fb, K.OSP_FB_ID_INSTANCE, sc.width, sc.height) == 1).mean() > 0.05
+
+
 def test_pick(hip_device):
     sc = scenes.c0(64, 64)
     s = scenes.to_ospray(sc)

@@ -124,10 +124,10 @@ def test_macrocell_intervals_cover_the_box_and_bound_the_samples():
         assert abs(iv[-1, 1] - exit_) <= 1e-4 * max(1.0, exit_)
         np.testing.assert_array_equal(iv[1:, 0], iv[:-1, 1])  # contiguous
         assert (iv[:, 1] > iv[:, 0]).all()
-        u = rng.uniform(0.05, 0.95, (len(iv), 8))
+        u = rng.uniform(0.05, 0.95, (len(iv), 24))
         t = iv[:, :1] + u * (iv[:, 1:2] - iv[:, :1])
         pts = (o[None, None, :] + t[:, :, None] * d[None, None, :]).reshape(-1, 3).astype(np.float32)
-        smp = os_.volume_sample(0, pts).reshape(len(iv), 8)
+        smp = os_.volume_sample(0, pts).reshape(len(iv), 24)
         ok = ~np.isnan(smp)
         lo = np.broadcast_to(iv[:, 2:3], smp.shape)
         hi = np.broadcast_to(iv[:, 3:4], smp.shape)
