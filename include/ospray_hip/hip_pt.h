@@ -24,7 +24,9 @@ typedef struct HptScene HptScene; /* committed world: BVH, primitives, materials
 typedef struct HptFrame HptFrame; /* camera, renderer parameters, pass range, fb buffers */
 typedef struct HptPaths HptPaths; /* wavefront path state, queues and counters */
 
+#ifndef HPT_API
 #define HPT_API __attribute__((visibility("default")))
+#endif
 
 /* Sets every queue counter to 0 except the entry count of extension queue
  * `keepIndex` (0 or 1; -1: none). */

@@ -1,0 +1,203 @@
+"""The neural radiance cache MLP on the GPU (hpt_nrc_* through the C ABI)
+against the FP64 oracle (oracle/nrc_ref.py, pinned by finite differences
+in test_nrc_cpu.py).
+
+Tolerances: the device stores weights, inputs, hidden activations and
+back-propagated gradients in bfloat16 and accumulates in fp32; the oracle
+rounds the same quantities to bfloat16 and accumulates in FP64, so the
+remaining differences come from summation order and from the rare
+activation whose bf16 rounding flips: outputs within 1 % of the largest
+output, gradients within 2 % (relative L2 per layer), losses within 0.5 %.
+torch provides device memory only.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import nrc_ref as R
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODULE = os.path.join(ROOT, "ospray_amd", "lib", "libospray_module_hip.so")
+
+
+class HptNrc(C.Structure):
+    _fields_ = [("w", C.c_void_p), ("wT", C.c_void_p), ("master", C.c_void_p), ("adamM", C.c_void_p),
+                ("adamV", C.c_void_p), ("partial", C.c_void_p), ("tileLoss", C.c_void_p), ("loss", C.c_void_p),
+                ("maxTiles", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(MODULE)
+        P = C.POINTER(HptNrc)
+        L.hpt_nrc_init.argtypes = [P, C.c_uint32, C.c_void_p]
+        L.hpt_nrc_encode.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hpt_nrc_infer.argtypes = [P, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.hpt_nrc_train.argtypes = [P, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.hpt_nrc_adam.argtypes = [P, C.c_int, C.c_float, C.c_int, C.c_void_p]
+        for f in (L.hpt_nrc_init, L.hpt_nrc_encode, L.hpt_nrc_infer, L.hpt_nrc_train, L.hpt_nrc_adam):
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    assert rc == 0, f"hipError {rc}"
+    torch.cuda.synchronize()
+
+
+class Net:
+    def __init__(self, max_tiles=64):
+        dev = "cuda"
+        self.w = torch.zeros(R.N_PARAMS, dtype=torch.bfloat16, device=dev)
+        self.wT = torch.zeros(14336, dtype=torch.bfloat16, device=dev)
+        self.master = torch.zeros(R.N_PARAMS, dtype=torch.float32, device=dev)
+        self.m = torch.zeros_like(self.master)
+        self.v = torch.zeros_like(self.master)
+        self.partial = torch.zeros(max_tiles * R.N_PARAMS, dtype=torch.float32, device=dev)
+        self.tile_loss = torch.zeros(max_tiles, dtype=torch.float32, device=dev)
+        self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.s = HptNrc(self.w.data_ptr(), self.wT.data_ptr(), self.master.data_ptr(), self.m.data_ptr(),
+                        self.v.data_ptr(), self.partial.data_ptr(), self.tile_loss.data_ptr(), self.loss.data_ptr(),
+                        max_tiles)
+
+    def set_params(self, params):
+        p32 = torch.tensor(np.asarray(params, np.float32))
+        self.master.copy_(p32)
+        self.w.copy_(p32.to(torch.bfloat16))
+        W = R.unpack(np.asarray(params, np.float64))
+        wt = np.zeros(14336, np.float32)
+        off = 0
+        for i in (1, 2, 3):
+            wt[off:off + 4096] = W[i].T.ravel()
+            off += 4096
+        w4t = np.zeros((64, 32))
+        w4t[:, :16] = W[4].T
+        wt[off:] = w4t.ravel()
+        self.wT.copy_(torch.tensor(wt).to(torch.bfloat16))
+        torch.cuda.synchronize()
+
+    def params(self):
+        return self.master.cpu().numpy().astype(np.float64)
+
+    def infer(self, feats_bf16):
+        n = feats_bf16.shape[0]
+        out = torch.zeros(n * 3, dtype=torch.float32, device="cuda")
+        _check(lib().hpt_nrc_infer(C.byref(self.s), feats_bf16.data_ptr(), n, out.data_ptr(), None))
+        return out.cpu().numpy().reshape(n, 3)
+
+    def train(self, feats_bf16, targets):
+        n = feats_bf16.shape[0]
+        t = torch.tensor(np.asarray(targets, np.float32).ravel(), device="cuda")
+        _check(lib().hpt_nrc_train(C.byref(self.s), feats_bf16.data_ptr(), t.data_ptr(), n, None))
+        tiles = (n + 63) // 64
+        g = self.partial[:tiles * R.N_PARAMS].view(tiles, R.N_PARAMS).sum(0).cpu().numpy().astype(np.float64)
+        return float(self.tile_loss[:tiles].sum().item()), g
+
+    def adam(self, n, lr, step):
+        _check(lib().hpt_nrc_adam(C.byref(self.s), n, lr, step, None))
+
+
+def _bf16_dev(x):
+    return torch.tensor(np.asarray(x, np.float32)).to(torch.bfloat16).cuda().contiguous()
+
+
+def _features(n, seed):
+    rng = np.random.default_rng(seed)
+    return R.bf16_round(rng.uniform(-1, 1, (n, R.N_IN)))
+
+
+def test_nrc_encode_matches_oracle():
+    rng = np.random.default_rng(1)
+    n = 1000
+    raw = np.concatenate([rng.uniform(-5, 5, (n, 3)), rng.normal(size=(n, 3)), rng.normal(size=(n, 3)),
+                          rng.uniform(0, 1, (n, 3))], 1)
+    raw[:, 3:6] /= np.linalg.norm(raw[:, 3:6], axis=1, keepdims=True)
+    lo, hi = np.array([-6.0, -6.0, -6.0], np.float32), np.array([6.0, 6.0, 6.0], np.float32)
+    raw_d = torch.tensor(raw.astype(np.float32).ravel(), device="cuda")
+    X = torch.zeros(n * R.N_IN, dtype=torch.bfloat16, device="cuda")
+    _check(lib().hpt_nrc_encode(raw_d.data_ptr(), n, lo.ctypes.data, hi.ctypes.data, X.data_ptr(), None))
+    got = X.float().cpu().numpy().reshape(n, R.N_IN)
+    ref = R.encode(raw.astype(np.float32).astype(np.float64), lo, hi)
+    # fp32 arguments of the 2^11 pi band are good to ~4e-4; then bf16 rounding
+    assert np.max(np.abs(got - ref) - np.abs(ref) * 2 ** -8) < 2e-3
+
+
+def test_nrc_forward_matches_oracle():
+    net = Net()
+    params = R.init_params(7)
+    net.set_params(params)
+    x = _features(1000, 2)  # 15 full tiles and a partial one
+    y = net.infer(_bf16_dev(x))
+    ref, _ = R.forward(params, x, bf16=True)
+    assert np.max(np.abs(y - ref)) <= 1e-2 * np.max(np.abs(ref)), np.max(np.abs(y - ref))
+
+
+def test_nrc_gradients_match_oracle():
+    net = Net()
+    params = R.init_params(8)
+    net.set_params(params)
+    x = _features(200, 3)  # 3 full tiles and one of 8
+    t = np.random.default_rng(4).uniform(0, 1, (200, 3))
+    loss, g = net.train(_bf16_dev(x), t)
+    Lref, gref = R.loss_and_grad(params, x, t, bf16=True)
+    assert abs(loss - Lref) <= 5e-3 * Lref, (loss, Lref)
+    for i in range(5):
+        a, b = R.OFFSETS[i], R.OFFSETS[i + 1]
+        err = np.linalg.norm(g[a:b] - gref[a:b]) / np.linalg.norm(gref[a:b])
+        assert err < 2e-2, (i, err)
+    assert np.all(g[R.OFFSETS[4] + 3 * 64:] == 0)
+
+
+def test_nrc_adam_step():
+    net = Net()
+    params = R.init_params(9)
+    net.set_params(params)
+    x = _features(128, 5)
+    t = np.random.default_rng(6).uniform(0, 1, (128, 3))
+    _, g = net.train(_bf16_dev(x), t)
+    g32 = net.partial[:2 * R.N_PARAMS].view(2, R.N_PARAMS)
+    g32 = (g32[0] + g32[1]).cpu().numpy()  # the kernel's summation order for 2 tiles
+    lr = 1e-2
+    net.adam(128, lr, 1)
+    m = np.float32(0.1) * g32
+    v = np.float32(0.01) * g32 * g32
+    bc1, bc2 = np.float32(1) - np.float32(0.9), np.float32(1) - np.float32(0.99)  # 1 - beta^1 in fp32
+    expect = np.float32(params.astype(np.float32)) - np.float32(lr) * (m / bc1) / (np.sqrt(v / bc2) + np.float32(1e-8))
+    expect[R.OFFSETS[4] + 3 * 64:] = 0
+    got = net.master.cpu().numpy()
+    np.testing.assert_allclose(got, expect, rtol=2e-6, atol=1e-7)
+    assert np.array_equal(net.w.cpu().float().numpy(), torch.tensor(got).to(torch.bfloat16).float().numpy())
+    W1 = got[R.OFFSETS[1]:R.OFFSETS[2]].reshape(64, 64)
+    wt = net.wT.cpu().float().numpy()
+    assert np.array_equal(wt[:4096].reshape(64, 64), torch.tensor(W1.T.copy()).to(torch.bfloat16).float().numpy())
+
+
+def test_nrc_training_fits_a_teacher():
+    """The device network, from its own initial weights, learns the outputs
+    of a fixed random teacher network on 4096 inputs: the loss falls by 10x
+    within 300 Adam steps."""
+    net = Net(max_tiles=64)
+    _check(lib().hpt_nrc_init(C.byref(net.s), 1, None))
+    teacher = R.init_params(11)
+    x = _features(4096, 7)
+    t, _ = R.forward(teacher, x)
+    t = np.abs(t) * 0.5
+    xd = _bf16_dev(x)
+    first = None
+    for step in range(1, 301):
+        loss, _ = net.train(xd, t)
+        net.adam(4096, 1e-2, step)
+        if first is None:
+            first = loss
+    final = R.loss_value(net.params(), x, t)
+    assert final < 0.1 * first, (first, final)
