@@ -115,14 +115,116 @@ def traffic_per_launch(kernel: str):
     return None
 
 
+def bench_c3(args) -> int:
+    """Plain vs radiance-cache rendering of the 1080p scene, 1 spp per frame,
+    each accumulated over --c3-frames frames and compared (RMSE over RGB)
+    with a plain render of --c3-ref-spp samples per pixel.  Frame times are
+    ospRenderFrame + ospWait wall times (the cache's inference and training
+    included); framebuffer maps for the RMSE checkpoints are outside them.
+    value = (plain time to its final RMSE) / (cache time to the same RMSE)."""
+    import numpy as np
+
+    from ospray_amd import api as A
+    from ospray_amd import capi as K
+    from ospray_amd import scenes
+
+    w, h = args.width or 1920, args.height or 1080
+    sc = scenes.c1(width=w, height=h, spp=1)
+    A.init("hip", profileKernels=True)
+    name, _ = A.device_info()
+    L = K.lib()
+
+    def frame(s, fb):
+        f = L.ospRenderFrame(fb.h, s.renderer.h, s.camera.h, s.world.h)
+        if not f:
+            A.check()
+            raise RuntimeError("ospRenderFrame failed")
+        L.ospWait(f, K.OSP_TASK_FINISHED)
+        L.ospRelease(f)
+
+    t = time.perf_counter()
+    ref_spp_frame = 16
+    sref = scenes.to_ospray(sc, {"pixelSamples": ref_spp_frame})
+    fbr = A.framebuffer(w, h, K.OSP_FB_RGBA32F, K.OSP_FB_COLOR | K.OSP_FB_ACCUM)
+    nref = max(1, args.c3_ref_spp // ref_spp_frame)
+    for i in range(nref):
+        frame(sref, fbr)
+        if i % 32 == 31:
+            log(f"reference {16 * (i + 1)} spp")
+    ref = A.map_color(fbr, w, h, K.OSP_FB_RGBA32F)[..., :3].astype(np.float64)
+    log(f"reference: {nref * ref_spp_frame} spp in {time.perf_counter() - t:.1f} s")
+
+    checkpoints = sorted({1, 2, 4, 8, 16, 32, 64, 128, 256, args.c3_frames} & set(range(1, args.c3_frames + 1)))
+
+    def run(rp, label):
+        s = scenes.to_ospray(sc, rp)
+        fb = A.framebuffer(w, h, K.OSP_FB_RGBA32F, K.OSP_FB_COLOR | K.OSP_FB_ACCUM)
+        frame(s, fb)  # warm-up frame (first launch of each kernel), then reset
+        L.ospResetAccumulation(fb.h)
+        A.reset_kernel_stats()
+        curve, elapsed = [], 0.0
+        for i in range(1, args.c3_frames + 1):
+            t0 = time.perf_counter()
+            frame(s, fb)
+            elapsed += time.perf_counter() - t0
+            if i in checkpoints:
+                img = A.map_color(fb, w, h, K.OSP_FB_RGBA32F)[..., :3].astype(np.float64)
+                curve.append((i, elapsed, float(np.sqrt(np.mean((img - ref) ** 2)))))
+                log(f"{label}: {i} frames {elapsed:.3f} s rmse {curve[-1][2]:.5f}")
+        ks = {k: round(v[1] / args.c3_frames, 3) for k, v in sorted(A.kernel_stats().items())}
+        return curve, s, ks
+
+    plain, _, kplain = run({}, "plain")
+    nrc, snrc, knrc = run({"nrc": True, "nrcBounce": 2, "nrcTrainPaths": 65536, "nrcTrainSteps": 4}, "cache")
+    loss, steps = A.nrc_state(snrc.renderer)
+    target = plain[-1][2]
+
+    def time_to(curve, rmse):
+        prev = (0, 0.0, float("inf"))
+        for c in curve:
+            if c[2] <= rmse:
+                if prev[2] == float("inf"):
+                    return c[1]
+                # log-log interpolation between checkpoints (RMSE ~ t^-1/2)
+                import math
+                f = (math.log(prev[2]) - math.log(rmse)) / max(math.log(prev[2]) - math.log(c[2]), 1e-12)
+                return prev[1] * (c[1] / prev[1]) ** f
+            prev = c
+        return None
+
+    tp, tn = time_to(plain, target), time_to(nrc, target)
+    out = {"metric": "plain vs neural-cache time to equal RMSE (1080p scene, 1 spp frames)",
+           "value": round(tp / tn, 3) if tn else None, "unit": "x (plain time / cache time)",
+           "n_gpus": 1, "steps": args.c3_frames, "warmup": 1,
+           "ms_per_step": round(nrc[-1][1] / args.c3_frames * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 path tracer, bf16 MLP (fp32 accumulate, fp32 Adam)",
+           "data": "synthetic",
+           "config": {"workload": f"C1 scene {w}x{h}, 1 spp per frame x {args.c3_frames} frames, cache after bounce "
+                                  f"2, 65536 training paths per frame in 4 Adam steps (lr 1e-2); reference "
+                                  f"{nref * ref_spp_frame} spp plain", "device": name},
+           "target_rmse": target,
+           "plain": {"rmse": [c[2] for c in plain], "seconds": [round(c[1], 4) for c in plain],
+                     "frames": [c[0] for c in plain], "ms_per_frame": round(plain[-1][1] / args.c3_frames * 1e3, 3),
+                     "kernels_ms_per_frame": kplain},
+           "cache": {"rmse": [c[2] for c in nrc], "seconds": [round(c[1], 4) for c in nrc],
+                     "frames": [c[0] for c in nrc], "ms_per_frame": round(nrc[-1][1] / args.c3_frames * 1e3, 3),
+                     "kernels_ms_per_frame": knrc, "final_training_loss": loss, "adam_steps": steps},
+           "time_to_target_s": {"plain": tp, "cache": tn}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample size (0: skip)")
-    ap.add_argument("--workload", choices=("c1", "c4"), default="c1",
-                    help="c1: the 1080p surface scene (default); c4: surfaces inside a 512^3 volume")
+    ap.add_argument("--workload", choices=("c1", "c3", "c4"), default="c1",
+                    help="c1: the 1080p surface scene (default); c3: plain vs radiance-cache time to equal "
+                         "RMSE on it; c4: surfaces inside a 512^3 volume")
+    ap.add_argument("--c3-frames", type=int, default=256, help="c3: 1-spp frames per method")
+    ap.add_argument("--c3-ref-spp", type=int, default=4096, help="c3: samples per pixel of the plain reference")
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)
@@ -140,6 +242,9 @@ def main() -> int:
     from ospray_amd import api as A
     from ospray_amd import capi as K
     from ospray_amd import scenes
+
+    if args.workload == "c3":
+        return bench_c3(args)
 
     if args.copy_calibration:
         A.init("hip")

@@ -33,6 +33,11 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void);
  * (out9[6..8]). */
 OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9]);
 
+/* The radiance cache of a "pathtracer" renderer with "nrc" set: the loss of
+ * its last training step (relative L2) and the number of Adam steps taken
+ * so far (0 and 0 before the first training pass). */
+OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *steps);
+
 /* Delta-tracking counters of the last frame rendered while "traversalStats"
  * was set: voxel samples (tentative collisions), macrocell intervals and
  * tracked segments of the free-path sampling (out6[0..2]) and of the

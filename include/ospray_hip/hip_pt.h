@@ -28,8 +28,9 @@ typedef struct HptPaths HptPaths; /* wavefront path state, queues and counters *
 #define HPT_API __attribute__((visibility("default")))
 #endif
 
-/* Sets every queue counter to 0 except the entry count of extension queue
- * `keepIndex` (0 or 1; -1: none). */
+/* keepIndex -1: sets every counter to 0.  keepIndex 0 or 1: sets the
+ * per-bounce queue counters to 0 except the entry count of extension queue
+ * `keepIndex` (the radiance-cache counters of the pass are kept). */
 HPT_API hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s);
 HPT_API hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s);
 /* dst <- src, bytes a multiple of 16: a plain streaming copy that calibrates
@@ -47,7 +48,9 @@ HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, i
  * macrocell intervals and tracked segments to P->stats[9..11]. */
 HPT_API hipError_t hpt_volume(const HptScene *S, const HptPaths *P, int queue, int blocks, int stats, hipStream_t s);
 /* One samplePath bounce (PathSampler.ispc:143-407) for queue `queue`; the
- * NEE light samples of its smooth vertices are queued for hpt_nee. */
+ * NEE light samples of its smooth vertices are queued for hpt_nee.  With
+ * F->nrcBounce >= 0 the vertices of that depth become radiance-cache
+ * queries or training records (no reference counterpart). */
 HPT_API hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s);
 /* One NEE light sample per thread for the vertices shaded from extension
  * queue `queue`: light selection and sampling, BSDF evaluation, shadow-ray
@@ -64,6 +67,14 @@ HPT_API hipError_t hpt_shadow(
  * surfaces (TransparentShadow.ispc:36-84) for the queued records. */
 HPT_API hipError_t hpt_shadow_trans(
     const HptScene *S, const HptFrame *F, const HptPaths *P, int residentBlocks, int stats, hipStream_t s);
+/* Radiance cache (no reference counterpart): the raw inputs (position,
+ * direction, normal, albedo) of the n queued query (which = 0) or training
+ * (which = 1) records; contribution += throughput * max(prediction, 0) for
+ * the n queries; the n training targets (radiance gathered after the
+ * vertex / throughput at the vertex, clamped to [0, 1e4]). */
+HPT_API hipError_t hpt_nrc_gather(const HptPaths *P, int which, int n, float *raw12, hipStream_t s);
+HPT_API hipError_t hpt_nrc_apply(const HptPaths *P, int n, const float *pred3, hipStream_t s);
+HPT_API hipError_t hpt_nrc_targets(const HptPaths *P, int n, float *target3, hipStream_t s);
 /* Ordered sum of the NEE samples of every vertex shaded from extension
  * queue `queue` (NextEventEstimation.ispc:150-159); with volumes, each
  * visible sample's transmittance estimates (NextEventEstimation.ispc:126-149). */

@@ -322,6 +322,16 @@ def traversal_stats() -> tuple:
     return tuple(int(x) for x in out)
 
 
+def nrc_state(renderer: Handle) -> tuple:
+    """(loss of the last training step, Adam steps so far) of a renderer's
+    radiance cache."""
+    loss, steps = C.c_float(0.0), C.c_int(0)
+    if K.ext().ospHipGetNrcState(renderer.h, C.byref(loss), C.byref(steps)) != 0:
+        check()
+        raise OSPRayError("ospHipGetNrcState failed")
+    return float(loss.value), int(steps.value)
+
+
 def volume_stats() -> tuple:
     """(voxel samples, macrocell intervals, tracked segments) of the
     free-path sampling, then of the transmittance estimates, of the last

@@ -131,6 +131,7 @@ _EXT_SIGS = {
     "ospHipResetKernelStats": (C.c_int, []),
     "ospHipGetTraversalStats": (C.c_int, [C.POINTER(_U64)]),
     "ospHipGetVolumeStats": (C.c_int, [C.POINTER(_U64)]),
+    "ospHipGetNrcState": (C.c_int, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
     "ospHipTraceRays": (C.c_int, [_VP, _VP, C.c_int, _VP, _VP, C.POINTER(_U64)]),
     "ospHipGetDeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
     "ospHipGetWorldInfo": (C.c_int, [_VP, C.POINTER(_I64)]),

@@ -24,6 +24,7 @@
 #include "../core/core.h"
 #include "bvh_build.h"
 #include "ospray/ospray_hip.h"
+#include "ospray_hip/hip_nrc.h"
 #include "ospray_hip/hip_pt.h"
 #include "pt_sampler.h"
 #include "pt_shade.h"
@@ -761,6 +762,19 @@ struct Renderer : public ManagedObject
   float filterWidth = 3.f;
   int lutSize = 0;
   DevBuf dLutX, dLutY;
+  // neural radiance cache (hip-only parameters, no reference counterpart):
+  // "nrc" on/off, the depth of the cache vertex, training paths and Adam
+  // steps per frame, learning rate, initial-weight seed; the network and
+  // its optimiser state persist across frames (online training) until
+  // "nrcReset" is committed or the seed changes
+  bool nrc = false;
+  int nrcBounce = 2, nrcTrainPaths = 65536, nrcTrainSteps = 4;
+  float nrcLr = 1e-2f;
+  uint32_t nrcSeed = 1;
+  bool nrcInit = false;
+  int nrcStep = 0;
+  DevBuf nW, nWT, nMaster, nM, nV, nPartial, nTileLoss, nLoss;
+  HptNrc nrcNet{};
   Renderer(const std::string &t)
   {
     managedType = OSP_RENDERER;
@@ -793,6 +807,17 @@ struct Renderer : public ManagedObject
       throw std::runtime_error("pathtracer: shadow catcher is not supported by the hip device");
     geometryLights = getBool("geometryLights", true);
     (void)getFloat("mipMapBias", 0.f);
+    nrc = getBool("nrc", false);
+    nrcBounce = std::max(1, getInt("nrcBounce", 2));
+    nrcTrainPaths = std::max(64, getInt("nrcTrainPaths", 65536));
+    nrcTrainSteps = std::max(1, getInt("nrcTrainSteps", 4));
+    nrcLr = getFloat("nrcLearningRate", 1e-2f);
+    const uint32_t seed = (uint32_t)getInt("nrcSeed", 1);
+    if (getBool("nrcReset", false) || seed != nrcSeed)
+      nrcInit = false;
+    nrcSeed = seed;
+    if (findParamAny("nrcReset"))
+      removeParam("nrcReset");
     const uint32_t pf = getUInt("pixelFilter", OSP_PIXELFILTER_GAUSS);
     if (pf == OSP_PIXELFILTER_POINT) {
       filterType = 0;
@@ -893,6 +918,10 @@ class HipDevice : public Device
   std::string gpuName;
   // path state, shared by all framebuffers of this device
   DevBuf pathBuf, recBuf, queueBuf, counters, overflow, stats;
+  // radiance cache per-pass buffers: vertex records, queues, raw inputs,
+  // bf16 features, predictions, training targets
+  DevBuf nrcRec, nrcQueues, nrcRaw, nrcFeat, nrcOut, nrcTarget;
+  size_t nrcSlots = 0;
   hpt::DPaths paths{};
   size_t pathSlots = 0, recSlots = 0;
   bool auxAllocated = false;
@@ -1185,6 +1214,8 @@ class HipDevice : public Device
   OSPPickResult pick(ManagedObject *fbo, ManagedObject *ro, ManagedObject *co, ManagedObject *wo, float x, float y) override;
 
   void ensurePaths(size_t slots, int maxRec, bool aux);
+  void ensureNrc(size_t slots);
+  void nrcPass(Renderer *r, World *w);
   int traceRays(World *w, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2);
 };
 
@@ -1727,6 +1758,100 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.stats = (unsigned long long *)stats.ptr;
 }
 
+void HipDevice::ensureNrc(size_t slots)
+{
+  if (slots > nrcSlots) {
+    HIP_CHECK(hipStreamSynchronize(stream));
+    nrcSlots = slots;
+    nrcRec.ensure(slots * 80); // 16 floats record + 4 floats contribution before
+    nrcQueues.ensure(slots * 8);
+    nrcRaw.ensure(slots * 48);
+    nrcFeat.ensure(slots * HPT_NRC_INPUTS * 2);
+    nrcOut.ensure(slots * 12);
+    nrcTarget.ensure(slots * 12);
+  }
+  paths.nrcRec = nrcRec.as<float>();
+  paths.nrcBefore = nrcRec.as<float>() + 16 * nrcSlots;
+  paths.nrcQueryQueue = nrcQueues.as<uint32_t>();
+  paths.nrcTrainQueue = nrcQueues.as<uint32_t>() + nrcSlots;
+}
+
+// The radiance cache after a pass's bounces: predictions for the paths that
+// ended at the cache, then Adam steps on the training records' targets.
+// The record counts are read back (one synchronisation per pass).
+void HipDevice::nrcPass(Renderer *r, World *w)
+{
+  const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);
+  int32_t cnt[2] = {0, 0};
+  HIP_CHECK(hipMemcpyAsync(cnt, paths.counters + hpt::kCntNrcQuery, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+  const int nQ = cnt[0], nT = cnt[1];
+  // scene box for the position encoding, padded so it is never empty
+  float lo[3], hi[3];
+  for (int k = 0; k < 3; k++) {
+    const float pad = 1e-3f * (1.f + std::fabs(w->bounds[k]) + std::fabs(w->bounds[3 + k]));
+    lo[k] = w->bounds[k] - pad;
+    hi[k] = w->bounds[3 + k] + pad;
+  }
+  uint16_t *feat = nrcFeat.as<uint16_t>();
+  // a pass yields at most about 2 x nrcTrainPaths training records; the
+  // per-tile gradient rows of one Adam step are sized for that
+  const int maxTiles = (2 * r->nrcTrainPaths + 63) / 64 + 1;
+  if (!r->nrcInit || maxTiles > r->nrcNet.maxTiles) {
+    const size_t P4 = (size_t)HPT_NRC_PARAMS * 4;
+    r->nW.ensure(HPT_NRC_PARAMS * 2);
+    r->nWT.ensure(HPT_NRC_TPARAMS * 2);
+    r->nMaster.ensure(P4);
+    r->nM.ensure(P4);
+    r->nV.ensure(P4);
+    r->nPartial.ensure((size_t)maxTiles * P4);
+    r->nTileLoss.ensure((size_t)maxTiles * 4);
+    r->nLoss.ensure(16);
+    r->nrcNet = {r->nW.as<uint16_t>(), r->nWT.as<uint16_t>(), r->nMaster.as<float>(), r->nM.as<float>(),
+        r->nV.as<float>(), r->nPartial.as<float>(), r->nTileLoss.as<float>(), r->nLoss.as<float>(), maxTiles};
+  }
+  if (!r->nrcInit) {
+    launch("nrc_init", [&]() { return hpt_nrc_init(&r->nrcNet, r->nrcSeed, stream); });
+    r->nrcInit = true;
+    r->nrcStep = 0;
+  }
+  if (nQ > 0) {
+    launch("nrc_infer", [&]() {
+      hipError_t e = hpt_nrc_gather(P, 0, nQ, nrcRaw.as<float>(), stream);
+      if (e == hipSuccess)
+        e = hpt_nrc_encode(nrcRaw.as<float>(), nQ, lo, hi, feat, stream);
+      if (e == hipSuccess)
+        e = hpt_nrc_infer(&r->nrcNet, feat, nQ, nrcOut.as<float>(), stream);
+      if (e == hipSuccess)
+        e = hpt_nrc_apply(P, nQ, nrcOut.as<float>(), stream);
+      return e;
+    });
+  }
+  if (nT > 0) {
+    int steps = std::min(r->nrcTrainSteps, (nT + 63) / 64);
+    int batch = (nT + steps - 1) / steps;
+    if ((batch + 63) / 64 > r->nrcNet.maxTiles) { // more, smaller steps
+      batch = r->nrcNet.maxTiles * 64;
+      steps = (nT + batch - 1) / batch;
+    }
+    launch("nrc_train", [&]() {
+      hipError_t e = hpt_nrc_targets(P, nT, nrcTarget.as<float>(), stream);
+      if (e == hipSuccess)
+        e = hpt_nrc_gather(P, 1, nT, nrcRaw.as<float>(), stream);
+      if (e == hipSuccess)
+        e = hpt_nrc_encode(nrcRaw.as<float>(), nT, lo, hi, feat, stream);
+      for (int k = 0; k < steps && e == hipSuccess; k++) {
+        const int b0 = k * batch, nb = std::min(batch, nT - b0);
+        e = hpt_nrc_train(&r->nrcNet, feat + (size_t)b0 * HPT_NRC_INPUTS, nrcTarget.as<float>() + 3 * (size_t)b0, nb,
+            stream);
+        if (e == hipSuccess)
+          e = hpt_nrc_adam(&r->nrcNet, nb, r->nrcLr, ++r->nrcStep, stream);
+      }
+      return e;
+    });
+  }
+}
+
 ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, ManagedObject *co, ManagedObject *wo)
 {
   if (fbo->managedType != OSP_FRAMEBUFFER || ro->managedType != OSP_RENDERER || co->managedType != OSP_CAMERA
@@ -1835,14 +1960,18 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   F.maxRec = (int)std::max<uint32_t>(1u, F.numFirstLightSamples);
   F.hasVolumes = sceneF.numVolInsts > 0 ? 1 : 0;
   F.stats = travStats ? 1 : 0;
+  F.nrcBounce = r->nrc && r->nrcBounce <= F.maxDepth ? r->nrcBounce : -1;
 
   if (!finishedAccum) {
     const bool aux = F.accAux != nullptr;
-    const size_t perSlotBytes = 9 * 16 + 56 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 92 + 8;
+    const size_t perSlotBytes = 9 * 16 + 56 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 92 + 8
+        + (F.nrcBounce >= 0 ? 80 + 8 + 48 + 2 * HPT_NRC_INPUTS + 24 : 0);
     const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
     const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
     ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
     paths.aux = aux ? auxBase : nullptr;
+    if (F.nrcBounce >= 0)
+      ensureNrc(pathSlots);
     paths.overflow = (uint32_t *)overflow.ptr;
     F.residentThreads = residentTraceBlocks * hpt_trace_block_size();
     const HptScene *S = reinterpret_cast<const HptScene *>(&sceneF);
@@ -1852,7 +1981,15 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     for (int s0 = 0; s0 < F.spp; s0 += sppPerPass) {
       F.passS0 = s0;
       F.passCount = std::min(sppPerPass, F.spp - s0);
+      if (F.nrcBounce >= 0) { // every stride-th slot of the pass trains the cache
+        const size_t slotsPass = (size_t)F.passCount * F.numPixPadded;
+        F.nrcTrainStride = (uint32_t)std::max<size_t>(1, slotsPass / (size_t)r->nrcTrainPaths);
+        F.nrcTrainOffset = (uint32_t)((size_t)fb->frameID * 7919u + (size_t)s0) % F.nrcTrainStride;
+      }
+      hpt::DFrame Fno = F; // the frame as the bounces without a cache vertex see it
+      Fno.nrcBounce = -1;
       const HptFrame *Fp = reinterpret_cast<const HptFrame *>(&F);
+      const HptFrame *FpNo = reinterpret_cast<const HptFrame *>(&Fno);
       launch("reset", [&]() { return hpt_reset_counters(P, -1, stream); });
       launch("generate", [&]() { return hpt_generate(Fp, P, stream); });
       int q = 0;
@@ -1861,7 +1998,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
         if (F.hasVolumes)
           launch("volume", [&]() { return hpt_volume(S, P, q, shadeBlocks, travStats ? 1 : 0, stream); });
-        launch("shade", [&]() { return hpt_shade(S, Fp, P, q, shadeBlocks, stream); });
+        launch("shade", [&]() { return hpt_shade(S, b == F.nrcBounce ? Fp : FpNo, P, q, shadeBlocks, stream); });
         if (F.numFirstLightSamples > 0) {
           launch("nee", [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
           launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
@@ -1871,6 +2008,8 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         }
         q ^= 1;
       }
+      if (F.nrcBounce >= 0)
+        nrcPass(r, w);
       launch("finish_pass", [&]() { return hpt_finish_pass(Fp, P, stream); });
     }
     HIP_CHECK(hipEventRecord(fut->rendered, stream));
@@ -2020,6 +2159,28 @@ OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9])
     HIP_CHECK(hipMemcpy(s, d->stats.ptr, sizeof(s), hipMemcpyDeviceToHost));
     for (int k = 0; k < 9; k++)
       out9[k] = s[k];
+  } catch (const std::exception &) {
+    return -2;
+  }
+  return 0;
+}
+
+OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *steps)
+{
+  HipDevice *d = current();
+  if (!d || !renderer || !loss || !steps)
+    return -1;
+  ManagedObject *o = (ManagedObject *)renderer;
+  if (o->managedType != OSP_RENDERER)
+    return -1;
+  Renderer *r = (Renderer *)o;
+  *loss = 0.f;
+  *steps = r->nrcInit ? r->nrcStep : 0;
+  if (!d->stream || !r->nrcInit || r->nrcStep == 0)
+    return 0;
+  try {
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+    HIP_CHECK(hipMemcpy(loss, r->nLoss.ptr, sizeof(float), hipMemcpyDeviceToHost));
   } catch (const std::exception &) {
     return -2;
   }

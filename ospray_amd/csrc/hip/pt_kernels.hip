@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void k_volume(DScene S, DPaths P, int qin, int
 // vertex's light-sample slots to the NEE sample queue; k_nee evaluates them
 // (NextEventEstimation.ispc:25-160).  The loop runs over whole waves so the
 // appends can scan across the wave.
-template <bool VOL>
+template <bool VOL, bool NRC>
 __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int qin)
 {
   const int n = P.counters[2 * qin];
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
   VolCount vcnt; // transmittance work of the virtual-light MIS weights
   for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
     const int i = w0 + laneId();
-    bool cont = false;
+    bool cont = false, nrcQuery = false, nrcTrain = false;
     uint32_t nNee = 0;
     uint32_t p = 0;
     if (i < n) {
@@ -475,6 +475,24 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
         BSDF b;
         if (!volHit && !getBSDF(S, dg, S.media[medium], b))
           break;
+        if (NRC && depth == (uint32_t)F.nrcBounce) {
+          // the radiance cache at this vertex (before its NEE): position,
+          // direction towards the previous vertex, shading normal, albedo
+          // and the throughput.  The path ends here with a cache query;
+          // a training slot runs on and becomes a training record.
+          const vec3f alb = volHit ? hit.Ng : b.albedo;
+          float *rec = P.nrcRec + 16 * (size_t)p;
+          st4(rec, dg.P.x, dg.P.y, dg.P.z, thr.x);
+          st4(rec + 4, -rdir.x, -rdir.y, -rdir.z, thr.y);
+          st4(rec + 8, dg.Ns.x, dg.Ns.y, dg.Ns.z, thr.z);
+          st4(rec + 12, alb.x, alb.y, alb.z, 0.f);
+          if ((p + F.nrcTrainOffset) % F.nrcTrainStride != 0) {
+            nrcQuery = true;
+            break;
+          }
+          nrcTrain = true;
+          st4(P.nrcBefore + 4 * (size_t)p, contribution.x, contribution.y, contribution.z, 0.f);
+        }
         ldNextGroup(lds);
         const vec3f wo = neg(rdir);
         uint32_t pvNLS = 0;
@@ -572,6 +590,14 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
       }
       P.neeN[p] = nNee;
     }
+    if (NRC) {
+      const int qs = enqueue(&P.counters[kCntNrcQuery], nrcQuery);
+      if (nrcQuery)
+        P.nrcQueryQueue[qs] = p;
+      const int ts = enqueue(&P.counters[kCntNrcTrain], nrcTrain);
+      if (nrcTrain)
+        P.nrcTrainQueue[ts] = p;
+    }
     // the continuing path to extension queue qout; the light-sample slots of
     // this vertex (records p*maxRec + [0, nNee)) to the NEE sample queue
     int so, first;
@@ -585,6 +611,50 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
     waveAdd(&P.stats[12], vcnt.samples);
     waveAdd(&P.stats[13], vcnt.cells);
     waveAdd(&P.stats[14], vcnt.tracks);
+  }
+}
+
+// radiance cache: the queued vertex records as raw cache inputs
+// (position, direction, normal, albedo)
+__global__ __launch_bounds__(256) void k_nrc_gather(DPaths P, const uint32_t *queue, int n, float *raw12)
+{
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const float *rec = P.nrcRec + 16 * (size_t)queue[i];
+    float *o = raw12 + 12 * (size_t)i;
+    for (int k = 0; k < 4; k++) {
+      o[3 * k] = rec[4 * k];
+      o[3 * k + 1] = rec[4 * k + 1];
+      o[3 * k + 2] = rec[4 * k + 2];
+    }
+  }
+}
+
+// a query's path: contribution += throughput * max(prediction, 0)
+__global__ __launch_bounds__(256) void k_nrc_apply(DPaths P, int n, const float *pred3)
+{
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t p = P.nrcQueryQueue[i];
+    const float *rec = P.nrcRec + 16 * (size_t)p;
+    float *c = P.contrib + 4 * (size_t)p;
+    for (int k = 0; k < 3; k++)
+      c[k] = c[k] + rec[4 * k + 3] * fmaxf(pred3[3 * (size_t)i + k], 0.f);
+  }
+}
+
+// a training record's target: the radiance its path gathered after the
+// vertex, divided by the throughput at the vertex
+__global__ __launch_bounds__(256) void k_nrc_targets(DPaths P, int n, float *target3)
+{
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t p = P.nrcTrainQueue[i];
+    const float *rec = P.nrcRec + 16 * (size_t)p;
+    const float *b = P.nrcBefore + 4 * (size_t)p;
+    const float *c = P.contrib + 4 * (size_t)p;
+    for (int k = 0; k < 3; k++) {
+      const float thr = rec[4 * k + 3];
+      const float t = thr > 0.f ? (c[k] - b[k]) / thr : 0.f;
+      target3[3 * (size_t)i + k] = t == t ? fminf(fmaxf(t, 0.f), 1e4f) : 0.f;
+    }
   }
 }
 
@@ -1086,8 +1156,10 @@ extern "C" {
 hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s)
 {
   const DPaths &p = *reinterpret_cast<const DPaths *>(P);
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, kNumCounters,
-      keepIndex < 0 ? -1 : 2 * keepIndex);
+  // a full reset clears every counter; the per-bounce reset only the queue
+  // counters of a bounce, keeping the extension queue being read
+  const int n = keepIndex < 0 ? kNumCounters : kNumBounceCounters;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, n, keepIndex < 0 ? -1 : 2 * keepIndex);
   return hipGetLastError();
 }
 
@@ -1131,11 +1203,47 @@ hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, in
   const DFrame &f = *reinterpret_cast<const DFrame *>(F);
   const DPaths &p = *reinterpret_cast<const DPaths *>(P);
   // the volume variant carries the sampler and transmittance code (more
-  // registers); surface-only worlds run the lean one
-  if (sc.numVolInsts > 0)
-    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
+  // registers), the cache variant the cache records (the caller passes a
+  // frame with nrcBounce -1 for the other bounces); surface-only worlds run
+  // the lean one
+  const bool vol = sc.numVolInsts > 0;
+  const bool nrc = f.nrcBounce >= 0;
+  if (vol && nrc)
+    hipLaunchKernelGGL((k_shade<true, true>), dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
+  else if (vol)
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
+  else if (nrc)
+    hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
   else
-    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
+    hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(256), 0, s, sc, f, p, queue);
+  return hipGetLastError();
+}
+
+hipError_t hpt_nrc_gather(const HptPaths *P, int which, int n, float *raw12, hipStream_t s)
+{
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_nrc_gather, dim3(gridFor(n, 256, 8192)), dim3(256), 0, s, p,
+      which ? p.nrcTrainQueue : p.nrcQueryQueue, n, raw12);
+  return hipGetLastError();
+}
+
+hipError_t hpt_nrc_apply(const HptPaths *P, int n, const float *pred3, hipStream_t s)
+{
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_nrc_apply, dim3(gridFor(n, 256, 8192)), dim3(256), 0, s, *reinterpret_cast<const DPaths *>(P),
+      n, pred3);
+  return hipGetLastError();
+}
+
+hipError_t hpt_nrc_targets(const HptPaths *P, int n, float *target3, hipStream_t s)
+{
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_nrc_targets, dim3(gridFor(n, 256, 8192)), dim3(256), 0, s,
+      *reinterpret_cast<const DPaths *>(P), n, target3);
   return hipGetLastError();
 }
 

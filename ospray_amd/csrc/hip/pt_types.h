@@ -232,6 +232,13 @@ struct DPaths
   // with a single atomic
   int32_t *counters;
   uint32_t *overflow; // traversal stack overflow, per resident thread
+  // radiance cache (when enabled): per path slot the vertex record
+  // (P, thr.x | wo, thr.y | Ns, thr.z | albedo, -) and the contribution
+  // before the vertex; compact queues of the query and training slots
+  float *nrcRec; // 16 per path
+  float *nrcBefore; // 4 per path
+  uint32_t *nrcQueryQueue;
+  uint32_t *nrcTrainQueue;
   // node visits, primitive tests, traversals: [0..2] extension rays, [3..5]
   // shadow occlusion pass, [6..8] shadow transparency pass; delta tracking
   // voxel samples, macrocell intervals, tracked segments: [9..11] free
@@ -246,7 +253,10 @@ constexpr int kCntFetchExtend = 5; // work fetch of the extension traversal
 constexpr int kCntFetchShadow = 6; // work fetch of the shadow traversal
 constexpr int kCntTrans = 7; // records for the transparency pass
 constexpr int kCntFetchTrans = 8; // work fetch of the transparency pass
-constexpr int kNumCounters = 9;
+constexpr int kNumBounceCounters = 9; // the counters above are reset every bounce
+constexpr int kCntNrcQuery = 9; // radiance-cache queries of the pass
+constexpr int kCntNrcTrain = 10; // radiance-cache training records of the pass
+constexpr int kNumCounters = 11;
 
 // path flags in ld[3]
 constexpr uint32_t FLAG_DEPTH_MASK = 0xffu; // bits 0-7
@@ -303,6 +313,11 @@ struct DFrame
   float adaptiveThreshold;
   int32_t hasVolumes; // the paths carry a volume sampler (P.rng)
   int32_t stats; // count traversal and delta-tracking work into P.stats
+  // radiance cache: paths end at the vertex of depth nrcBounce (-1: off)
+  // with a cache query, except every nrcTrainStride-th slot (offset
+  // nrcTrainOffset), which runs on and becomes a training record
+  int32_t nrcBounce;
+  uint32_t nrcTrainStride, nrcTrainOffset;
 };
 
 } // namespace hpt
