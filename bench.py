@@ -176,7 +176,7 @@ def bench_c3(args) -> int:
 
     plain, _, kplain = run({}, "plain")
     nrc, snrc, knrc = run({"nrc": True, "nrcBounce": 2, "nrcTrainPaths": 65536, "nrcTrainSteps": 4}, "cache")
-    loss, steps = A.nrc_state(snrc.renderer)
+    loss, steps, queries, records = A.nrc_state(snrc.renderer)
     target = plain[-1][2]
 
     def time_to(curve, rmse):
@@ -208,7 +208,8 @@ def bench_c3(args) -> int:
                      "kernels_ms_per_frame": kplain},
            "cache": {"rmse": [c[2] for c in nrc], "seconds": [round(c[1], 4) for c in nrc],
                      "frames": [c[0] for c in nrc], "ms_per_frame": round(nrc[-1][1] / args.c3_frames * 1e3, 3),
-                     "kernels_ms_per_frame": knrc, "final_training_loss": loss, "adam_steps": steps},
+                     "kernels_ms_per_frame": knrc, "final_training_loss": loss, "adam_steps": steps,
+                     "queries_last_frame": queries, "training_records_last_frame": records},
            "time_to_target_s": {"plain": tp, "cache": tn}}
     print(json.dumps(out), flush=True)
     return 0

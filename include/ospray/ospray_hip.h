@@ -34,9 +34,11 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void);
 OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9]);
 
 /* The radiance cache of a "pathtracer" renderer with "nrc" set: the loss of
- * its last training step (relative L2) and the number of Adam steps taken
- * so far (0 and 0 before the first training pass). */
-OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *steps);
+ * its last training step (relative L2), the number of Adam steps taken so
+ * far, and the cache queries and training records of the last frame (all
+ * 0 before the first training pass). */
+OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *steps, int64_t *queries,
+    int64_t *trainingRecords);
 
 /* Delta-tracking counters of the last frame rendered while "traversalStats"
  * was set: voxel samples (tentative collisions), macrocell intervals and

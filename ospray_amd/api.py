@@ -323,13 +323,13 @@ def traversal_stats() -> tuple:
 
 
 def nrc_state(renderer: Handle) -> tuple:
-    """(loss of the last training step, Adam steps so far) of a renderer's
-    radiance cache."""
-    loss, steps = C.c_float(0.0), C.c_int(0)
-    if K.ext().ospHipGetNrcState(renderer.h, C.byref(loss), C.byref(steps)) != 0:
+    """(loss of the last training step, Adam steps so far, cache queries and
+    training records of the last frame) of a renderer's radiance cache."""
+    loss, steps, q, t = C.c_float(0.0), C.c_int(0), C.c_int64(0), C.c_int64(0)
+    if K.ext().ospHipGetNrcState(renderer.h, C.byref(loss), C.byref(steps), C.byref(q), C.byref(t)) != 0:
         check()
         raise OSPRayError("ospHipGetNrcState failed")
-    return float(loss.value), int(steps.value)
+    return float(loss.value), int(steps.value), int(q.value), int(t.value)
 
 
 def volume_stats() -> tuple:

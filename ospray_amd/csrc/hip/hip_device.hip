@@ -773,6 +773,10 @@ struct Renderer : public ManagedObject
   uint32_t nrcSeed = 1;
   bool nrcInit = false;
   int nrcStep = 0;
+  int64_t nrcQueries = 0, nrcRecords = 0; // of the last frame
+  // fraction of training slots whose path reaches the cache vertex (running
+  // estimate): the training stride aims at nrcTrainPaths records per pass
+  double nrcSurvival = 1.0;
   DevBuf nW, nWT, nMaster, nM, nV, nPartial, nTileLoss, nLoss;
   HptNrc nrcNet{};
   Renderer(const std::string &t)
@@ -1786,6 +1790,8 @@ void HipDevice::nrcPass(Renderer *r, World *w)
   HIP_CHECK(hipMemcpyAsync(cnt, paths.counters + hpt::kCntNrcQuery, sizeof(cnt), hipMemcpyDeviceToHost, stream));
   HIP_CHECK(hipStreamSynchronize(stream));
   const int nQ = cnt[0], nT = cnt[1];
+  r->nrcQueries += nQ;
+  r->nrcRecords += nT;
   // scene box for the position encoding, padded so it is never empty
   float lo[3], hi[3];
   for (int k = 0; k < 3; k++) {
@@ -1961,6 +1967,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   F.hasVolumes = sceneF.numVolInsts > 0 ? 1 : 0;
   F.stats = travStats ? 1 : 0;
   F.nrcBounce = r->nrc && r->nrcBounce <= F.maxDepth ? r->nrcBounce : -1;
+  r->nrcQueries = r->nrcRecords = 0;
 
   if (!finishedAccum) {
     const bool aux = F.accAux != nullptr;
@@ -1983,7 +1990,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       F.passCount = std::min(sppPerPass, F.spp - s0);
       if (F.nrcBounce >= 0) { // every stride-th slot of the pass trains the cache
         const size_t slotsPass = (size_t)F.passCount * F.numPixPadded;
-        F.nrcTrainStride = (uint32_t)std::max<size_t>(1, slotsPass / (size_t)r->nrcTrainPaths);
+        F.nrcTrainStride = (uint32_t)std::max<double>(1.0, (double)slotsPass * r->nrcSurvival / r->nrcTrainPaths);
         F.nrcTrainOffset = (uint32_t)((size_t)fb->frameID * 7919u + (size_t)s0) % F.nrcTrainStride;
       }
       hpt::DFrame Fno = F; // the frame as the bounces without a cache vertex see it
@@ -2008,8 +2015,13 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         }
         q ^= 1;
       }
-      if (F.nrcBounce >= 0)
+      if (F.nrcBounce >= 0) {
+        const int64_t before = r->nrcRecords;
         nrcPass(r, w);
+        const double slots = std::ceil((double)F.passCount * F.numPixPadded / F.nrcTrainStride);
+        const double survival = std::min(1.0, std::max(1e-3, (double)(r->nrcRecords - before) / slots));
+        r->nrcSurvival = 0.5 * r->nrcSurvival + 0.5 * survival;
+      }
       launch("finish_pass", [&]() { return hpt_finish_pass(Fp, P, stream); });
     }
     HIP_CHECK(hipEventRecord(fut->rendered, stream));
@@ -2165,10 +2177,11 @@ OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9])
   return 0;
 }
 
-OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *steps)
+OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *steps, int64_t *queries,
+    int64_t *trainingRecords)
 {
   HipDevice *d = current();
-  if (!d || !renderer || !loss || !steps)
+  if (!d || !renderer || !loss || !steps || !queries || !trainingRecords)
     return -1;
   ManagedObject *o = (ManagedObject *)renderer;
   if (o->managedType != OSP_RENDERER)
@@ -2176,6 +2189,8 @@ OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *s
   Renderer *r = (Renderer *)o;
   *loss = 0.f;
   *steps = r->nrcInit ? r->nrcStep : 0;
+  *queries = r->nrcQueries;
+  *trainingRecords = r->nrcRecords;
   if (!d->stream || !r->nrcInit || r->nrcStep == 0)
     return 0;
   try {

@@ -231,7 +231,7 @@ def test_nrc_in_the_path_tracer(hip_device):
     # that reach the third vertex become records, a few hundred per frame
     rp = {"nrc": True, "nrcBounce": 2, "nrcTrainPaths": 2048, "nrcTrainSteps": 4}
     img, s, _ = _render(sc, rp, 1)
-    first, steps = A.nrc_state(s.renderer)
+    first, steps = A.nrc_state(s.renderer)[:2]
     assert 1 <= steps <= 4 and first > 0
     img, s, fb = _render(sc, rp, 179, s=s)
     losses = []
