@@ -115,6 +115,68 @@ def traffic_per_launch(kernel: str):
     return None
 
 
+def bench_c2(args) -> int:
+    """64 views of the 1080p scene (scenes.c2_views: seeded positions on a
+    radius-30 sphere, resolutions {256^2, 512^2, 768x432, 1024^2}, spp
+    {1, 2, 4, 8}, maxPathLength 8) rendered as one batch: the autotuner
+    picks the resident grids and the path budget, then the batch is
+    captured once into a HIP graph and relaunched --steps times.  value =
+    samples of the batch / mean wall time of a launch (host launch + GPU +
+    wait); per-view latency = GPU time from the batch start to the view's
+    end, p50/p99 over the views of the last launch."""
+    import numpy as np
+
+    from ospray_amd import api as A
+    from ospray_amd import capi as K
+    from ospray_amd import scenes
+
+    sc = scenes.c1()
+    views = scenes.c2_views()
+    A.init("hip")
+    name, _ = A.device_info()
+    s = scenes.to_ospray(sc)
+    fbs, rens, cams = [], [], []
+    for v in views:
+        cams.append(A.new("camera", "perspective", dict(v["camera"])))
+        rens.append(A.new("renderer", "pathtracer", {"pixelSamples": v["spp"], "maxPathLength": 8}))
+        fbs.append(A.framebuffer(v["width"], v["height"], K.OSP_FB_RGBA32F, K.OSP_FB_COLOR))
+    samples = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    log(f"{len(views)} views, {samples} samples per batch")
+    t = time.perf_counter()
+    best_ms, best = A.tune_batch(fbs, rens, cams, s.world, reps=1)
+    log(f"autotuner: {time.perf_counter() - t:.1f} s, best batch {best_ms:.2f} ms with traceBlocksPerCU {best[0]}, "
+        f"shadeBlocksPerCU {best[1]}, pathBudgetMB {best[2]}")
+    t = time.perf_counter()
+    A.render_batch(fbs, rens, cams, s.world, graph=True)  # eager sizing pass + capture + first launch
+    log(f"graph captured and launched in {time.perf_counter() - t:.2f} s")
+    for _ in range(args.warmup):
+        A.render_batch(fbs, rens, cams, s.world, graph=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ms = A.render_batch(fbs, rens, cams, s.world, graph=True)
+    elapsed = time.perf_counter() - t0
+    eager_t = time.perf_counter()
+    A.render_batch(fbs, rens, cams, s.world, graph=False)
+    eager_s = time.perf_counter() - eager_t
+    lat = np.array(ms)
+    img = A.map_color(fbs[0], views[0]["width"], views[0]["height"], K.OSP_FB_RGBA32F)
+    if not np.isfinite(img).all():
+        raise RuntimeError("non-finite pixels")
+    out = {"metric": "batched multi-view path samples/s (64 views of the 1080p scene, one graph launch)",
+           "value": round(samples * args.steps / elapsed, 1), "unit": "samples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "C2: 64 views of the C1 scene, seeded positions on a radius-30 sphere, "
+                                  "resolutions {256^2, 512^2, 768x432, 1024^2}, spp {1,2,4,8}, maxPathLength 8",
+                      "samples_per_batch": samples, "device": name},
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+           "autotuner": {"traceBlocksPerCU": best[0], "shadeBlocksPerCU": best[1], "pathBudgetMB": best[2],
+                         "best_eager_batch_ms": round(best_ms, 3)},
+           "eager_batch_ms": round(eager_s * 1e3, 3)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_c3(args) -> int:
     """Plain vs radiance-cache rendering of the 1080p scene, 1 spp per frame,
     each accumulated over --c3-frames frames and compared (RMSE over RGB)
@@ -221,9 +283,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample size (0: skip)")
-    ap.add_argument("--workload", choices=("c1", "c3", "c4"), default="c1",
-                    help="c1: the 1080p surface scene (default); c3: plain vs radiance-cache time to equal "
-                         "RMSE on it; c4: surfaces inside a 512^3 volume")
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4"), default="c1",
+                    help="c1: the 1080p surface scene (default); c2: 64 views of it in one batched graph "
+                         "launch; c3: plain vs radiance-cache time to equal RMSE on it; c4: surfaces inside a "
+                         "512^3 volume")
     ap.add_argument("--c3-frames", type=int, default=256, help="c3: 1-spp frames per method")
     ap.add_argument("--c3-ref-spp", type=int, default=4096, help="c3: samples per pixel of the plain reference")
     ap.add_argument("--width", type=int, default=None)
@@ -246,6 +309,8 @@ def main() -> int:
 
     if args.workload == "c3":
         return bench_c3(args)
+    if args.workload == "c2":
+        return bench_c2(args)
 
     if args.copy_calibration:
         A.init("hip")

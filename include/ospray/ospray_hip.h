@@ -33,6 +33,26 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void);
  * (out9[6..8]). */
 OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9]);
 
+/* Renders n views (framebuffer, renderer, camera) of one world in one
+ * enqueue and waits for them.  mode 0: eager launches; mode 1: one HIP
+ * graph, captured on the first call for this batch (after an eager pass
+ * that sizes every buffer) and relaunched while the batch's objects are
+ * unchanged; relaunches replay the captured frames (same sample indices),
+ * and leave ospGetVariance's estimate on the device.  viewMs (optional,
+ * n floats): GPU milliseconds from the batch start to the end of each
+ * view.  Renderers with "nrc" cannot be captured.  Returns 0 on success. */
+OSPRAY_INTERFACE int ospHipRenderBatch(int n, const OSPFrameBuffer *framebuffers, const OSPRenderer *renderers,
+    const OSPCamera *cameras, OSPWorld world, int mode, float *viewMs);
+
+/* Autotuner of a batch: times the eager batch (best of `reps`) for each
+ * candidate resident traversal grid (8, 12, 16 blocks per CU), shading
+ * grid (4, 8, 16) and path-state budget (2 GiB, 16 GiB: the samples per
+ * pass, i.e. the wavefront queue sizes), keeps the fastest as the device's
+ * setting.  Returns its batch milliseconds; best3 = traceBlocksPerCU,
+ * shadeBlocksPerCU, pathBudgetMB. */
+OSPRAY_INTERFACE float ospHipTuneBatch(int n, const OSPFrameBuffer *framebuffers, const OSPRenderer *renderers,
+    const OSPCamera *cameras, OSPWorld world, int reps, int *best3);
+
 /* The radiance cache of a "pathtracer" renderer with "nrc" set: the loss of
  * its last training step (relative L2), the number of Adam steps taken so
  * far, and the cache queries and training records of the last frame (all

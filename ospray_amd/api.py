@@ -332,6 +332,35 @@ def nrc_state(renderer: Handle) -> tuple:
     return float(loss.value), int(steps.value), int(q.value), int(t.value)
 
 
+def _handles(hs):
+    return (C.c_void_p * len(hs))(*[h.h for h in hs])
+
+
+def render_batch(fbs, renderers, cameras, world: Handle, graph: bool = True):
+    """Render a batch of views in one enqueue (a HIP graph when graph is
+    True); returns each view's GPU milliseconds from the batch start."""
+    n = len(fbs)
+    ms = (C.c_float * n)()
+    rc = K.ext().ospHipRenderBatch(n, _handles(fbs), _handles(renderers), _handles(cameras), world.h,
+                                   1 if graph else 0, ms)
+    if rc != 0:
+        check()
+        raise OSPRayError("ospHipRenderBatch failed")
+    return [float(x) for x in ms]
+
+
+def tune_batch(fbs, renderers, cameras, world: Handle, reps: int = 2):
+    """Run the batch autotuner; returns (best batch ms, (traceBlocksPerCU,
+    shadeBlocksPerCU, pathBudgetMB))."""
+    best = (C.c_int * 3)()
+    n = len(fbs)
+    ms = K.ext().ospHipTuneBatch(n, _handles(fbs), _handles(renderers), _handles(cameras), world.h, reps, best)
+    if ms < 0:
+        check()
+        raise OSPRayError("ospHipTuneBatch failed")
+    return float(ms), tuple(int(x) for x in best)
+
+
 def volume_stats() -> tuple:
     """(voxel samples, macrocell intervals, tracked segments) of the
     free-path sampling, then of the transmittance estimates, of the last

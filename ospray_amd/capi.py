@@ -131,6 +131,10 @@ _EXT_SIGS = {
     "ospHipResetKernelStats": (C.c_int, []),
     "ospHipGetTraversalStats": (C.c_int, [C.POINTER(_U64)]),
     "ospHipGetVolumeStats": (C.c_int, [C.POINTER(_U64)]),
+    "ospHipRenderBatch": (C.c_int, [C.c_int, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP), _VP, C.c_int,
+                                    C.POINTER(C.c_float)]),
+    "ospHipTuneBatch": (C.c_float, [C.c_int, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP), _VP, C.c_int,
+                                    C.POINTER(C.c_int)]),
     "ospHipGetNrcState": (C.c_int, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(_I64),
                                      C.POINTER(_I64)]),
     "ospHipTraceRays": (C.c_int, [_VP, _VP, C.c_int, _VP, _VP, C.POINTER(_U64)]),

@@ -933,6 +933,7 @@ class HipDevice : public Device
   int residentTraceBlocks = 0, shadeBlocks = 0;
   size_t pathBudgetBytes = (size_t)16 << 30;
   bool profile = false, travStats = false;
+  bool capturing = false; // enqueuing a batch of views into a HIP graph
   std::vector<KernelTimer> timers;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pendingEvents;
   std::map<std::vector<std::pair<const Geometry *, uint64_t>>, std::shared_ptr<Blas>> blasCache;
@@ -941,6 +942,11 @@ class HipDevice : public Device
   ~HipDevice() override
   {
     collectTimers();
+    releaseGraph();
+    for (auto e : viewEvents)
+      (void)hipEventDestroy(e);
+    if (batchStart)
+      (void)hipEventDestroy(batchStart);
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -986,6 +992,7 @@ class HipDevice : public Device
   int traceBlocksPerCU = 12, shadeBlocksPerCU = 8;
   void sizeGrids()
   {
+    releaseGraph(); // a captured batch refers to the grid sizes and the overflow rows
     residentTraceBlocks = std::max(1, numCUs) * traceBlocksPerCU;
     shadeBlocks = std::max(1, numCUs) * shadeBlocksPerCU;
     overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
@@ -1012,7 +1019,7 @@ class HipDevice : public Device
   template <typename F>
   void launch(const char *name, F &&fn)
   {
-    if (!profile) {
+    if (!profile || capturing) {
       HIP_CHECK(fn());
       return;
     }
@@ -1218,6 +1225,157 @@ class HipDevice : public Device
   OSPPickResult pick(ManagedObject *fbo, ManagedObject *ro, ManagedObject *co, ManagedObject *wo, float x, float y) override;
 
   void ensurePaths(size_t slots, int maxRec, bool aux);
+
+  // ---- batched views (no reference counterpart) ----
+  // Many views (framebuffer, renderer, camera) of one world in one
+  // enqueue: eagerly (mode 0) or as one captured HIP graph (mode 1), built
+  // once per batch and relaunched; graph launches replay the captured
+  // frames (same sample indices).  outMs[i] = GPU time from the batch start
+  // to the end of view i.
+  struct BatchView
+  {
+    ManagedObject *fb, *renderer, *camera;
+  };
+  std::vector<void *> graphKey;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  std::vector<hipEvent_t> viewEvents;
+  hipEvent_t batchStart = nullptr;
+  void releaseGraph()
+  {
+    if (graphExec)
+      (void)hipGraphExecDestroy(graphExec);
+    if (graph)
+      (void)hipGraphDestroy(graph);
+    graphExec = nullptr;
+    graph = nullptr;
+    graphKey.clear();
+  }
+  void enqueueViews(const std::vector<BatchView> &v, ManagedObject *world, std::vector<ManagedObject *> &futures)
+  {
+    HIP_CHECK(hipEventRecord(batchStart, stream));
+    for (size_t i = 0; i < v.size(); i++) {
+      futures.push_back(renderFrame(v[i].fb, v[i].renderer, v[i].camera, world));
+      HIP_CHECK(hipEventRecord(viewEvents[i], stream));
+    }
+  }
+  void renderBatch(const std::vector<BatchView> &v, ManagedObject *world, int mode, float *outMs)
+  {
+    gpu();
+    while (viewEvents.size() < v.size()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      viewEvents.push_back(e);
+    }
+    if (!batchStart)
+      HIP_CHECK(hipEventCreate(&batchStart));
+    std::vector<ManagedObject *> futures;
+    auto releaseFutures = [&]() {
+      for (auto *f : futures)
+        f->refDec();
+      futures.clear();
+    };
+    try {
+      if (mode == 0) {
+        enqueueViews(v, world, futures);
+      } else {
+        // the graph holds the captured parameters: rebuilt when any object
+        // of the batch is another one or has been committed since
+        std::vector<void *> key{world, (void *)(uintptr_t)world->commitCounter};
+        for (auto &x : v)
+          for (ManagedObject *o : {x.fb, x.renderer, x.camera}) {
+            key.push_back(o);
+            key.push_back((void *)(uintptr_t)o->commitCounter);
+          }
+        if (key != graphKey || !graphExec) {
+          releaseGraph();
+          // an eager pass first: it sizes every buffer, so nothing is
+          // allocated (or synchronised) while capturing
+          enqueueViews(v, world, futures);
+          HIP_CHECK(hipStreamSynchronize(stream));
+          releaseFutures();
+          capturing = true;
+          HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+          try {
+            enqueueViews(v, world, futures);
+          } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g)
+              (void)hipGraphDestroy(g);
+            capturing = false;
+            throw;
+          }
+          HIP_CHECK(hipStreamEndCapture(stream, &graph));
+          capturing = false;
+          releaseFutures();
+          HIP_CHECK(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+          graphKey = key;
+        }
+        HIP_CHECK(hipGraphLaunch(graphExec, stream));
+      }
+      HIP_CHECK(hipStreamSynchronize(stream));
+    } catch (...) {
+      capturing = false;
+      releaseFutures();
+      throw;
+    }
+    releaseFutures();
+    for (size_t i = 0; i < v.size(); i++)
+      if (!outMs || hipEventElapsedTime(&outMs[i], batchStart, viewEvents[i]) != hipSuccess) {
+        if (outMs)
+          outMs[i] = -1.f;
+      }
+  }
+
+  // Autotuner of the batch: times the eager batch for each candidate of
+  // the resident traversal grid, the shading grid and the path-state
+  // budget (which sets the samples per pass, i.e. the wavefront-queue
+  // sizes), keeps the fastest.  Returns its milliseconds; best[3] =
+  // traceBlocksPerCU, shadeBlocksPerCU, pathBudgetMB.
+  float tuneBatch(const std::vector<BatchView> &v, ManagedObject *world, int reps, int *best)
+  {
+    gpu();
+    const int tb0 = traceBlocksPerCU, sb0 = shadeBlocksPerCU;
+    const size_t pb0 = pathBudgetBytes;
+    const int traceCand[] = {8, 12, 16}, shadeCand[] = {4, 8, 16};
+    const size_t budgetCand[] = {(size_t)2048 << 20, (size_t)16384 << 20};
+    float bestMs = 1e30f;
+    int bt = tb0, bs = sb0;
+    size_t bb = pb0;
+    std::vector<float> ms(v.size());
+    for (size_t pb : budgetCand)
+      for (int tb : traceCand)
+        for (int sbk : shadeCand) {
+          traceBlocksPerCU = tb;
+          shadeBlocksPerCU = sbk;
+          pathBudgetBytes = pb;
+          sizeGrids();
+          renderBatch(v, world, 0, ms.data()); // warm-up (allocations)
+          float t = 1e30f;
+          for (int k = 0; k < std::max(1, reps); k++) {
+            renderBatch(v, world, 0, ms.data());
+            t = std::min(t, ms.back());
+          }
+          if (t < bestMs) {
+            bestMs = t;
+            bt = tb;
+            bs = sbk;
+            bb = pb;
+          }
+        }
+    traceBlocksPerCU = bt;
+    shadeBlocksPerCU = bs;
+    pathBudgetBytes = bb;
+    sizeGrids();
+    releaseGraph();
+    if (best) {
+      best[0] = bt;
+      best[1] = bs;
+      best[2] = (int)(bb >> 20);
+    }
+    return bestMs;
+  }
   void ensureNrc(size_t slots);
   void nrcPass(Renderer *r, World *w);
   int traceRays(World *w, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2);
@@ -1707,6 +1865,9 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   const bool grow = slots > pathSlots || (size_t)maxRec * slots > recSlots || (aux && !auxAllocated);
   if (!grow)
     return;
+  if (capturing)
+    throw std::runtime_error("batched views: path buffers would grow while capturing");
+  releaseGraph(); // a captured batch refers to the old buffers
   HIP_CHECK(hipStreamSynchronize(stream));
   pathSlots = std::max(slots, pathSlots);
   recSlots = std::max((size_t)maxRec * pathSlots, recSlots);
@@ -1765,6 +1926,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
 void HipDevice::ensureNrc(size_t slots)
 {
   if (slots > nrcSlots) {
+    releaseGraph();
     HIP_CHECK(hipStreamSynchronize(stream));
     nrcSlots = slots;
     nrcRec.ensure(slots * 80); // 16 floats record + 4 floats contribution before
@@ -1881,7 +2043,10 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   HIP_CHECK(hipEventCreate(&fut->start));
   HIP_CHECK(hipEventCreate(&fut->rendered));
   HIP_CHECK(hipEventCreate(&fut->finished));
-  HIP_CHECK(hipEventRecord(fut->start, stream));
+  if (capturing)
+    fut->empty = true; // a captured frame: its completion is the graph's
+  else
+    HIP_CHECK(hipEventRecord(fut->start, stream));
 
   // FrameBuffer::beginFrame (FrameBuffer.cpp:115-137)
   const bool doAccum = (fb->channels & OSP_FB_ACCUM) != 0;
@@ -1967,6 +2132,8 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   F.hasVolumes = sceneF.numVolInsts > 0 ? 1 : 0;
   F.stats = travStats ? 1 : 0;
   F.nrcBounce = r->nrc && r->nrcBounce <= F.maxDepth ? r->nrcBounce : -1;
+  if (capturing && F.nrcBounce >= 0)
+    throw std::runtime_error("batched views: the radiance cache needs a host synchronisation per pass");
   r->nrcQueries = r->nrcRecords = 0;
 
   if (!finishedAccum) {
@@ -2024,7 +2191,8 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       }
       launch("finish_pass", [&]() { return hpt_finish_pass(Fp, P, stream); });
     }
-    HIP_CHECK(hipEventRecord(fut->rendered, stream));
+    if (!capturing)
+      HIP_CHECK(hipEventRecord(fut->rendered, stream));
     const HptFrame *Fp = reinterpret_cast<const HptFrame *>(&F);
     launch("fb_accumulate", [&]() { return hpt_fb_accumulate(Fp, stream); });
     // LocalFrameBuffer::postProcess (LocalFB.cpp:459-478)
@@ -2034,7 +2202,8 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       launch("variance_reduce", [&]() {
         return hpt_variance_reduce(F.taskVariance, ntask, fb->errorThreshold, fb->varianceOut.as<float>(), stream);
       });
-      HIP_CHECK(hipMemcpyAsync(fb->hostVariance, fb->varianceOut.ptr, sizeof(float), hipMemcpyDeviceToHost, stream));
+      if (!capturing) // a captured batch leaves the error estimate on the device
+        HIP_CHECK(hipMemcpyAsync(fb->hostVariance, fb->varianceOut.ptr, sizeof(float), hipMemcpyDeviceToHost, stream));
       fb->varianceValid = true;
     }
     if (fb->color.ptr && fb->format != OSP_FB_NONE) {
@@ -2043,9 +2212,11 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       });
     }
   } else {
-    HIP_CHECK(hipEventRecord(fut->rendered, stream));
+    if (!capturing)
+      HIP_CHECK(hipEventRecord(fut->rendered, stream));
   }
-  HIP_CHECK(hipEventRecord(fut->finished, stream));
+  if (!capturing)
+    HIP_CHECK(hipEventRecord(fut->finished, stream));
   return fut;
 }
 
@@ -2175,6 +2346,54 @@ OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9])
     return -2;
   }
   return 0;
+}
+
+static void extensionError(HipDevice *d, const char *msg);
+
+static bool batchViews(int n, const OSPFrameBuffer *fbs, const OSPRenderer *rs, const OSPCamera *cams,
+    std::vector<HipDevice::BatchView> &v)
+{
+  if (n <= 0 || !fbs || !rs || !cams)
+    return false;
+  for (int i = 0; i < n; i++) {
+    ManagedObject *f = (ManagedObject *)fbs[i], *r = (ManagedObject *)rs[i], *c = (ManagedObject *)cams[i];
+    if (!f || !r || !c || f->managedType != OSP_FRAMEBUFFER || r->managedType != OSP_RENDERER
+        || c->managedType != OSP_CAMERA)
+      return false;
+    v.push_back({f, r, c});
+  }
+  return true;
+}
+
+OSPRAY_INTERFACE int ospHipRenderBatch(int n, const OSPFrameBuffer *fbs, const OSPRenderer *renderers,
+    const OSPCamera *cameras, OSPWorld world, int mode, float *viewMs)
+{
+  HipDevice *d = current();
+  std::vector<HipDevice::BatchView> v;
+  if (!d || !world || !batchViews(n, fbs, renderers, cameras, v) || (mode != 0 && mode != 1))
+    return -1;
+  try {
+    d->renderBatch(v, (ManagedObject *)world, mode, viewMs);
+  } catch (const std::exception &e) {
+    extensionError(d, e.what());
+    return -2;
+  }
+  return 0;
+}
+
+OSPRAY_INTERFACE float ospHipTuneBatch(int n, const OSPFrameBuffer *fbs, const OSPRenderer *renderers,
+    const OSPCamera *cameras, OSPWorld world, int reps, int *best3)
+{
+  HipDevice *d = current();
+  std::vector<HipDevice::BatchView> v;
+  if (!d || !world || !batchViews(n, fbs, renderers, cameras, v))
+    return -1.f;
+  try {
+    return d->tuneBatch(v, (ManagedObject *)world, reps, best3);
+  } catch (const std::exception &e) {
+    extensionError(d, e.what());
+    return -2.f;
+  }
 }
 
 OSPRAY_INTERFACE int ospHipGetNrcState(OSPRenderer renderer, float *loss, int *steps, int64_t *queries,

@@ -369,6 +369,29 @@ def c1(width: int = 1920, height: int = 1080, spp: int = 4, max_path_length: int
                  renderer={"pixelSamples": spp, "maxPathLength": max_path_length})
 
 
+def c2_views(n: int = 64, seed: int = 44, radius: float = 30.0) -> list:
+    """The C2 batch: n views of the C1 scene with positions uniform on a
+    sphere of `radius` about the origin (z = 2u - 1, phi = 2 pi v), looking
+    at the origin, resolution drawn from {256^2, 512^2, 768x432, 1024^2} and
+    spp from {1, 2, 4, 8}; maxPathLength 8, fovy 40.  Views below the
+    terrain still render (they see its underside)."""
+    rng = Pcg32(seed)
+    sizes = [(256, 256), (512, 512), (768, 432), (1024, 1024)]
+    views = []
+    for _ in range(n):
+        z = 2.0 * rng.uniform() - 1.0
+        phi = 2.0 * math.pi * rng.uniform()
+        r = math.sqrt(max(0.0, 1.0 - z * z))
+        pos = (radius * r * math.cos(phi), radius * z, radius * r * math.sin(phi))
+        w, h = sizes[rng.next_u32() % 4]
+        spp = (1, 2, 4, 8)[rng.next_u32() % 4]
+        up = (0.0, 1.0, 0.0) if abs(z) < 0.999 else (1.0, 0.0, 0.0)
+        views.append({"width": w, "height": h, "spp": spp,
+                      "camera": {"position": pos, "direction": _look_at(pos), "up": up, "fovy": 40.0,
+                                 "aspect": w / h}})
+    return views
+
+
 def c4(width: int = 1280, height: int = 720, spp: int = 2, max_path_length: int = 10, seed: int = 45,
        grid: int = 512, base_period: int = 128) -> Scene:
     """The C0 spheres, ground and lights embedded in a grid^3 fBm density

@@ -1,0 +1,76 @@
+"""Batched views (ospHipRenderBatch, ospHipTuneBatch): a batch of views of
+one world rendered in one enqueue, eagerly or as one captured HIP graph,
+gives each view exactly the image ospRenderFrame gives it; graph relaunches
+replay the same frames; the autotuner's settings leave the images
+unchanged."""
+import numpy as np
+import pytest
+
+from ospray_amd import api as A
+from ospray_amd import capi as K
+from ospray_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _views(s, specs):
+    fbs, rens, cams = [], [], []
+    for (w, h, spp, pos) in specs:
+        cams.append(A.new("camera", "perspective", {"position": pos, "direction": tuple(-x for x in pos),
+                                                     "up": (0.0, 1.0, 0.0), "fovy": 45.0, "aspect": w / h}))
+        rens.append(A.new("renderer", "pathtracer", {"pixelSamples": spp, "maxPathLength": 4}))
+        fbs.append(A.framebuffer(w, h, K.OSP_FB_RGBA32F, K.OSP_FB_COLOR | K.OSP_FB_DEPTH))
+    return fbs, rens, cams
+
+
+SPECS = [(40, 24, 1, (0.0, 4.0, -20.0)), (64, 64, 2, (15.0, 6.0, -10.0)), (24, 40, 4, (-12.0, 10.0, 12.0))]
+
+
+def _images(fbs):
+    return [A.map_color(fb, w, h, K.OSP_FB_RGBA32F) for fb, (w, h, _, _) in zip(fbs, SPECS)]
+
+
+def test_batch_equals_single_frames(hip_device):
+    sc = scenes.c0(32, 32)
+    s = scenes.to_ospray(sc)
+    fbs, rens, cams = _views(s, SPECS)
+    single = []
+    for fb, r, c, (w, h, _, _) in zip(fbs, rens, cams, SPECS):
+        A.render(fb, r, c, s.world)
+        single.append(A.map_color(fb, w, h, K.OSP_FB_RGBA32F))
+    ms = A.render_batch(fbs, rens, cams, s.world, graph=False)
+    assert all(a >= 0 for a in ms) and ms == sorted(ms)  # views finish in order on one stream
+    for a, b in zip(_images(fbs), single):
+        assert np.array_equal(a, b)
+    for _ in range(3):  # capture, then two relaunches
+        A.render_batch(fbs, rens, cams, s.world, graph=True)
+        for a, b in zip(_images(fbs), single):
+            assert np.array_equal(a, b)
+    # a recommitted camera invalidates the captured graph
+    A.set_param(cams[1].h, "position", (16.0, 6.0, -10.0))
+    A.commit(cams[1])
+    A.render_batch(fbs, rens, cams, s.world, graph=True)
+    moved = _images(fbs)[1]
+    fb = A.framebuffer(64, 64, K.OSP_FB_RGBA32F, K.OSP_FB_COLOR | K.OSP_FB_DEPTH)
+    A.render(fb, rens[1], cams[1], s.world)
+    assert np.array_equal(moved, A.map_color(fb, 64, 64, K.OSP_FB_RGBA32F))
+    assert not np.array_equal(moved, single[1])
+
+
+def test_autotuner_keeps_images(hip_device):
+    sc = scenes.c0(32, 32)
+    s = scenes.to_ospray(sc)
+    fbs, rens, cams = _views(s, SPECS)
+    A.render_batch(fbs, rens, cams, s.world, graph=False)
+    before = _images(fbs)
+    ms, best = A.tune_batch(fbs, rens, cams, s.world, reps=1)
+    assert ms > 0 and best[0] in (8, 12, 16) and best[1] in (4, 8, 16) and best[2] in (2048, 16384)
+    A.render_batch(fbs, rens, cams, s.world, graph=True)
+    for a, b in zip(_images(fbs), before):
+        assert np.array_equal(a, b)
+    # back to the defaults for the tests that follow
+    L = K.lib()
+    dev = L.ospGetCurrentDevice()
+    for k, v in (("traceBlocksPerCU", 12), ("shadeBlocksPerCU", 8), ("pathBudgetMB", 16384)):
+        A.set_device_param(dev, k, v)
+    L.ospDeviceCommit(dev)
