@@ -56,6 +56,13 @@ HPT_API hipError_t hpt_nrc_encode(const float *raw12, int n, const float *lo3, c
     hipStream_t s);
 /* out3 = n x 3 network outputs for n x 160 bf16 features. */
 HPT_API hipError_t hpt_nrc_infer(const HptNrc *N, const uint16_t *features, int n, float *out3, hipStream_t s);
+/* The path tracer's cache queries fused in one kernel: for the n path
+ * slots in queue, read the slot's vertex record rec16 (position,
+ * throughput.x | direction, throughput.y | normal, throughput.z | albedo,
+ * unused), encode it, run the network, and add throughput * max(output, 0)
+ * to contrib4[slot].xyz. */
+HPT_API hipError_t hpt_nrc_query(const HptNrc *N, const float *rec16, const uint32_t *queue, int n, const float *lo3,
+    const float *hi3, float *contrib4, hipStream_t s);
 /* Forward + backward of the relative-L2 loss over n <= maxTiles * 64
  * (features, targets) pairs: per-tile gradients (scaled by 1 / (3 n)) and
  * losses into N->partial and N->tileLoss. */

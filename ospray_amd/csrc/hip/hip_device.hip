@@ -1983,18 +1983,10 @@ void HipDevice::nrcPass(Renderer *r, World *w)
     r->nrcInit = true;
     r->nrcStep = 0;
   }
-  if (nQ > 0) {
+  if (nQ > 0) // gather, encode, infer and apply in one kernel
     launch("nrc_infer", [&]() {
-      hipError_t e = hpt_nrc_gather(P, 0, nQ, nrcRaw.as<float>(), stream);
-      if (e == hipSuccess)
-        e = hpt_nrc_encode(nrcRaw.as<float>(), nQ, lo, hi, feat, stream);
-      if (e == hipSuccess)
-        e = hpt_nrc_infer(&r->nrcNet, feat, nQ, nrcOut.as<float>(), stream);
-      if (e == hipSuccess)
-        e = hpt_nrc_apply(P, nQ, nrcOut.as<float>(), stream);
-      return e;
+      return hpt_nrc_query(&r->nrcNet, paths.nrcRec, paths.nrcQueryQueue, nQ, lo, hi, paths.contrib, stream);
     });
-  }
   if (nT > 0) {
     int steps = std::min(r->nrcTrainSteps, (nT + 63) / 64);
     int batch = (nT + steps - 1) / steps;

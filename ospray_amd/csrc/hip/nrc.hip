@@ -327,17 +327,30 @@ __global__ __launch_bounds__(256) void k_nrc_init(HptNrc N, uint32_t seed)
     reinterpret_cast<__bf16 *>(N.wT)[kOffT[3] + (i / 16) * 32 + 16 + (i & 15)] = (__bf16)0.f;
 }
 
+// 64 parameters per block, the tiles' gradient rows split over 4 groups of
+// 64 threads (group k sums tiles k, k + 4, ... in order) and combined in a
+// fixed order: deterministic
 __global__ __launch_bounds__(256) void k_nrc_adam(HptNrc N, int tiles, float lr, float bc1, float bc2)
 {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kP; p += gridDim.x * blockDim.x) {
-    float g = 0.f;
-    for (int t = 0; t < tiles; t++)
-      g += N.partial[(size_t)t * kP + p];
-    const float m = 0.9f * N.adamM[p] + 0.1f * g;
-    const float v = 0.99f * N.adamV[p] + 0.01f * g * g;
-    N.adamM[p] = m;
-    N.adamV[p] = v;
-    writeWeight(N, p, N.master[p] - lr * (m / bc1) / (sqrtf(v / bc2) + 1e-8f));
+  __shared__ float part[4][64];
+  const int col = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  for (int p0 = blockIdx.x * 64; p0 < kP; p0 += gridDim.x * 64) {
+    const int p = p0 + col;
+    float gs = 0.f;
+    if (p < kP)
+      for (int t = grp; t < tiles; t += 4)
+        gs += N.partial[(size_t)t * kP + p];
+    part[grp][col] = gs;
+    __syncthreads();
+    if (grp == 0 && p < kP) {
+      const float g = (part[0][col] + part[1][col]) + (part[2][col] + part[3][col]);
+      const float m = 0.9f * N.adamM[p] + 0.1f * g;
+      const float v = 0.99f * N.adamV[p] + 0.01f * g * g;
+      N.adamM[p] = m;
+      N.adamV[p] = v;
+      writeWeight(N, p, N.master[p] - lr * (m / bc1) / (sqrtf(v / bc2) + 1e-8f));
+    }
+    __syncthreads();
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     float L = 0.f;
@@ -347,34 +360,138 @@ __global__ __launch_bounds__(256) void k_nrc_adam(HptNrc N, int tiles, float lr,
   }
 }
 
+// feature group j (0..19: 8 features) of a query whose raw inputs are
+// position, direction, normal, albedo (raw12): groups 0..17 hold 4 bands
+// x (sin, cos) of coordinate j / 3 (position mapped over [lo, hi], then
+// direction); 18 and 19 the albedo, normal, mapped position, direction, 1.
+// j may differ across lanes: the coordinate is picked by selects, so the
+// raw inputs stay in registers.
+__device__ __forceinline__ bf16x8 encodeGroup(const float (&raw12)[12], int j, const float lo[3], const float inv[3])
+{
+  const float px = (raw12[0] - lo[0]) * inv[0], py = (raw12[1] - lo[1]) * inv[1], pz = (raw12[2] - lo[2]) * inv[2];
+  bf16x8 v;
+  if (j < 18) {
+    const int d = j / 3, h = j - 3 * d;
+    float c = px;
+    c = d == 1 ? py : c;
+    c = d == 2 ? pz : c;
+    c = d == 3 ? raw12[3] : c;
+    c = d == 4 ? raw12[4] : c;
+    c = d == 5 ? raw12[5] : c;
+    // band 4h directly, the next three by the double-angle formulas
+    // sin 2x = 2 sin x cos x, cos 2x = (cos x - sin x)(cos x + sin x)
+    float sn, cs;
+    hpt::fsincos((float)(1 << (4 * h)) * hpt::kPi * c, sn, cs);
+    for (int b = 0; b < 4; b++) {
+      v[2 * b] = (__bf16)sn;
+      v[2 * b + 1] = (__bf16)cs;
+      const float s2 = 2.f * sn * cs, c2 = (cs - sn) * (cs + sn);
+      sn = s2;
+      cs = c2;
+    }
+    return v;
+  }
+  const bool first = j == 18;
+  const float t[8] = {first ? raw12[9] : pz, first ? raw12[10] : raw12[3], first ? raw12[11] : raw12[4],
+      first ? raw12[6] : raw12[5], first ? raw12[7] : 1.f, first ? raw12[8] : 0.f, first ? px : 0.f,
+      first ? py : 0.f};
+  for (int b = 0; b < 8; b++)
+    v[b] = (__bf16)t[b];
+  return v;
+}
+
 __global__ __launch_bounds__(256) void k_nrc_encode(const float *raw, int n, float lx, float ly, float lz, float hx,
     float hy, float hz, __bf16 *X)
 {
+  const float lo[3] = {lx, ly, lz}, inv[3] = {1.f / (hx - lx), 1.f / (hy - ly), 1.f / (hz - lz)};
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const float *rq = raw + 12 * (size_t)q;
-    const float c[6] = {(rq[0] - lx) / (hx - lx), (rq[1] - ly) / (hy - ly), (rq[2] - lz) / (hz - lz), rq[3], rq[4],
-        rq[5]};
+    float rq[12];
+    for (int k = 0; k < 12; k++)
+      rq[k] = raw[12 * (size_t)q + k];
     bf16x8 *out = reinterpret_cast<bf16x8 *>(X + (size_t)q * kIn);
-    for (int d = 0; d < 6; d++)
-      for (int h = 0; h < 3; h++) { // 4 bands per 16-byte store
-        bf16x8 v;
-        for (int j = 0; j < 4; j++) {
-          const int k = 4 * h + j;
-          float s, co;
-          hpt::fsincos((float)(1 << k) * hpt::kPi * c[d], s, co);
-          v[2 * j] = (__bf16)s;
-          v[2 * j + 1] = (__bf16)co;
+    for (int j = 0; j < kIn / 8; j++)
+      out[j] = encodeGroup(rq, j, lo, inv);
+  }
+}
+
+// The path tracer's cache queries in one kernel: gather the queued vertex
+// records (per path slot: P, thr.x | wo, thr.y | Ns, thr.z | albedo, -),
+// encode them in registers, run the network with the weights staged once
+// per block in LDS (8 waves, one 64-query tile each per iteration), and add
+// throughput * max(prediction, 0) to the path's contribution.
+constexpr int kQueryWaves = 8;
+__global__ __launch_bounds__(64 * kQueryWaves) void k_nrc_query(HptNrc N, const float *rec16, const uint32_t *queue,
+    int n, float lx, float ly, float lz, float hx, float hy, float hz, float *contrib4)
+{
+  __shared__ __attribute__((aligned(16))) __bf16 wl[kP];
+  __shared__ __attribute__((aligned(16))) __bf16 actAll[kQueryWaves][64 * kS];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  __bf16 *act = actAll[wave];
+  {
+    const bf16x8 *src = reinterpret_cast<const bf16x8 *>(N.w);
+    bf16x8 *dst = reinterpret_cast<bf16x8 *>(wl);
+    for (int i = threadIdx.x; i < kP / 8; i += blockDim.x)
+      dst[i] = src[i];
+  }
+  __syncthreads();
+  const float lo[3] = {lx, ly, lz}, inv[3] = {1.f / (hx - lx), 1.f / (hy - ly), 1.f / (hz - lz)};
+  // uniform trip count over the block: every wave reaches every barrier
+  for (int base = blockIdx.x * kQueryWaves * 64; base < n; base += gridDim.x * kQueryWaves * 64) {
+    const int q0 = base + wave * 64;
+    // raw inputs of this lane's 4 queries (tile rows qt * 16 + r)
+    float raw[4][12];
+    for (int qt = 0; qt < 4; qt++) {
+      const int q = q0 + qt * 16 + r;
+      if (q < n) {
+        const float *rec = rec16 + 16 * (size_t)queue[q];
+        for (int k = 0; k < 4; k++) {
+          const float4 v = *reinterpret_cast<const float4 *>(rec + 4 * k);
+          raw[qt][3 * k] = v.x;
+          raw[qt][3 * k + 1] = v.y;
+          raw[qt][3 * k + 2] = v.z;
         }
-        out[d * 3 + h] = v;
+      } else {
+        for (int k = 0; k < 12; k++)
+          raw[qt][k] = 0.f;
       }
-    const float tail[16] = {rq[9], rq[10], rq[11], rq[6], rq[7], rq[8], c[0], c[1], c[2], c[3], c[4], c[5], 1.f, 0.f,
-        0.f, 0.f};
-    for (int h = 0; h < 2; h++) {
-      bf16x8 v;
-      for (int j = 0; j < 8; j++)
-        v[j] = (__bf16)tail[8 * h + j];
-      out[18 + h] = v;
     }
+    f32x4 acc[4][4];
+    for (int nt = 0; nt < 4; nt++)
+      for (int qt = 0; qt < 4; qt++)
+        acc[nt][qt] = zero4();
+    for (int kb = 0; kb < kIn / 32; kb++) {
+      bf16x8 b[4];
+      for (int qt = 0; qt < 4; qt++)
+        b[qt] = q0 + qt * 16 + r < n ? encodeGroup(raw[qt], 4 * kb + g, lo, inv) : zero8();
+      for (int nt = 0; nt < 4; nt++) {
+        const bf16x8 a = ld8(wl + kOff[0] + (nt * 16 + r) * kIn + kb * 32 + g * 8);
+        for (int qt = 0; qt < 4; qt++)
+          acc[nt][qt] = mma(a, b[qt], acc[nt][qt]);
+      }
+    }
+    __syncthreads();
+    storeAct(acc, act, nullptr, r, g);
+    __syncthreads();
+    for (int l = 1; l < 4; l++) {
+      layer64(wl + kOff[l], act, acc, r, g);
+      __syncthreads();
+      storeAct(acc, act, nullptr, r, g);
+      __syncthreads();
+    }
+    f32x4 o[4];
+    layerOut(wl + kOff[4], act, o, r, g);
+    if (g == 0)
+      for (int qt = 0; qt < 4; qt++) {
+        const int q = q0 + qt * 16 + r;
+        if (q < n) {
+          const uint32_t p = queue[q];
+          const float *rec = rec16 + 16 * (size_t)p;
+          float *c = contrib4 + 4 * (size_t)p;
+          for (int k = 0; k < 3; k++)
+            c[k] = c[k] + rec[4 * k + 3] * fmaxf(o[qt][k], 0.f);
+        }
+      }
+    __syncthreads();
   }
 }
 
@@ -400,6 +517,16 @@ hipError_t hpt_nrc_encode(const float *raw12, int n, const float *lo3, const flo
     return hipSuccess;
   hipLaunchKernelGGL(k_nrc_encode, dim3(gridFor(n, 256, 8192)), dim3(256), 0, s, raw12, n, lo3[0], lo3[1], lo3[2],
       hi3[0], hi3[1], hi3[2], reinterpret_cast<__bf16 *>(features));
+  return hipGetLastError();
+}
+
+hipError_t hpt_nrc_query(const HptNrc *N, const float *rec16, const uint32_t *queue, int n, const float *lo3,
+    const float *hi3, float *contrib4, hipStream_t s)
+{
+  if (n <= 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(k_nrc_query, dim3(gridFor(n, 64 * kQueryWaves, 4096)), dim3(64 * kQueryWaves), 0, s, *N, rec16,
+      queue, n, lo3[0], lo3[1], lo3[2], hi3[0], hi3[1], hi3[2], contrib4);
   return hipGetLastError();
 }
 
@@ -429,7 +556,7 @@ hipError_t hpt_nrc_adam(const HptNrc *N, int n, float lr, int step, hipStream_t 
   if (tiles <= 0 || tiles > N->maxTiles || step < 1)
     return hipErrorInvalidValue;
   const float bc1 = 1.f - powf(0.9f, (float)step), bc2 = 1.f - powf(0.99f, (float)step);
-  hipLaunchKernelGGL(k_nrc_adam, dim3(gridFor(kP, 256, 1024)), dim3(256), 0, s, *N, tiles, lr, bc1, bc2);
+  hipLaunchKernelGGL(k_nrc_adam, dim3(gridFor(kP, 64, 1024)), dim3(256), 0, s, *N, tiles, lr, bc1, bc2);
   return hipGetLastError();
 }
 

@@ -158,6 +158,45 @@ def test_nrc_gradients_match_oracle():
     assert np.all(g[R.OFFSETS[4] + 3 * 64:] == 0)
 
 
+def test_nrc_fused_query_equals_the_stages():
+    """hpt_nrc_query (records gathered, encoded and run in one kernel, the
+    result added to the contributions) equals encode + infer on the same
+    records, bit for bit."""
+    L = lib()
+    L.hpt_nrc_query.argtypes = [C.POINTER(HptNrc), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p]
+    L.hpt_nrc_query.restype = C.c_int
+    net = Net()
+    net.set_params(R.init_params(12))
+    rng = np.random.default_rng(13)
+    slots, n = 5000, 777
+    rec = np.zeros((slots, 16), np.float32)
+    rec[:, 0:3] = rng.uniform(-5, 5, (slots, 3))
+    d = rng.normal(size=(slots, 3))
+    rec[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rec[:, 8:11] = rng.normal(size=(slots, 3))
+    rec[:, 12:15] = rng.uniform(0, 1, (slots, 3))
+    rec[:, [3, 7, 11]] = rng.uniform(0.1, 2, (slots, 3))  # throughput
+    queue = rng.choice(slots, n, replace=False).astype(np.uint32)
+    contrib = rng.uniform(0, 1, (slots, 4)).astype(np.float32)
+    lo, hi = np.array([-6, -6, -6], np.float32), np.array([6, 6, 6], np.float32)
+    rec_d = torch.tensor(rec.ravel(), device="cuda")
+    q_d = torch.tensor(queue.astype(np.int32), device="cuda")
+    c_d = torch.tensor(contrib.ravel(), device="cuda")
+    _check(L.hpt_nrc_query(C.byref(net.s), rec_d.data_ptr(), q_d.data_ptr(), n, lo.ctypes.data, hi.ctypes.data,
+                           c_d.data_ptr(), None))
+    got = c_d.cpu().numpy().reshape(slots, 4)
+    raw = np.concatenate([rec[queue][:, 0:3], rec[queue][:, 4:7], rec[queue][:, 8:11], rec[queue][:, 12:15]], 1)
+    raw_d = torch.tensor(raw.ravel(), device="cuda")
+    X = torch.zeros(n * R.N_IN, dtype=torch.bfloat16, device="cuda")
+    _check(L.hpt_nrc_encode(raw_d.data_ptr(), n, lo.ctypes.data, hi.ctypes.data, X.data_ptr(), None))
+    y = net.infer(X.view(n, R.N_IN))
+    expect = contrib.copy()
+    thr = rec[queue][:, [3, 7, 11]]
+    expect[queue, :3] = contrib[queue, :3] + thr * np.maximum(y, 0)
+    assert np.array_equal(got, expect)
+
+
 def test_nrc_adam_step():
     net = Net()
     params = R.init_params(9)
@@ -217,22 +256,52 @@ def _render(sc, rp, frames, fb=None, s=None):
     return A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)[..., :3].astype(np.float64), s, fb
 
 
-def test_nrc_in_the_path_tracer(hip_device):
-    """C0 at 64x64 with paths ending at the cache after bounce 2: the online
-    training loss falls, the accumulated image stays within a few percent of
-    the plain path tracer's, and the trained cache beats an untrained one
-    (same renderer, learning rate 0) against the plain reference."""
-    from ospray_amd import api as A
+def _box_scene(w=64, h=64):
+    """A closed Lambertian box (open towards the camera) with a quad light
+    under its ceiling and two icospheres: nearly every path reaches its
+    third vertex, where the cache answers."""
     from ospray_amd import scenes
 
-    sc = scenes.c0(64, 64, spp=1, max_path_length=8)
-    ref, _, _ = _render(sc, {}, 256)
-    # every second path slot trains (4096 padded slots / 2048); only paths
-    # that reach the third vertex become records, a few hundred per frame
-    rp = {"nrc": True, "nrcBounce": 2, "nrcTrainPaths": 2048, "nrcTrainSteps": 4}
+    q = lambda *v: np.array(v, np.float32).reshape(4, 3)
+    walls = [q(-5, -5, -5, 5, -5, -5, 5, -5, 5, -5, -5, 5),  # floor (normal +y)
+             q(-5, 5, 5, 5, 5, 5, 5, 5, -5, -5, 5, -5),  # ceiling (-y)
+             q(-5, -5, 5, 5, -5, 5, 5, 5, 5, -5, 5, 5),  # back (-z)
+             q(-5, -5, -5, -5, -5, 5, -5, 5, 5, -5, 5, -5),  # left (+x)
+             q(5, -5, 5, 5, -5, -5, 5, 5, -5, 5, 5, 5)]  # right (-x)
+    kd = [(0.7, 0.7, 0.7), (0.7, 0.7, 0.7), (0.7, 0.7, 0.7), (0.7, 0.2, 0.2), (0.2, 0.7, 0.2)]
+    geoms, models, mats = [], [], []
+    for i, v in enumerate(walls):
+        geoms.append(scenes.Geometry("quads", vertex=v, index=np.array([[0, 1, 2, 3]], np.uint32)))
+        mats.append({"type": "obj", "kd": kd[i]})
+        models.append(scenes.Model(i, i))
+    unit, faces = scenes.icosphere(2)
+    for c, rad in (((-2.0, -3.5, 1.0), 1.5), ((2.0, -3.0, -1.0), 2.0)):
+        geoms.append(scenes.Geometry("triangles", vertex=(np.array(c) + rad * unit).astype(np.float32),
+                                     index=faces.astype(np.uint32)))
+        mats.append({"type": "obj", "kd": (0.6, 0.6, 0.8)})
+        models.append(scenes.Model(len(geoms) - 1, len(mats) - 1))
+    lights = [{"type": "quad", "position": (-1.5, 4.9, -1.5), "edge1": (3.0, 0.0, 0.0), "edge2": (0.0, 0.0, 3.0),
+               "intensity": 15.0}]
+    cam = {"position": (0.0, 0.0, -14.0), "direction": (0.0, 0.0, 1.0), "up": (0.0, 1.0, 0.0), "fovy": 45.0,
+           "aspect": w / h}
+    return scenes.Scene("box", w, h, geoms, models, [list(range(len(models)))], [scenes.Instance(0)], mats, lights,
+                        cam, {"pixelSamples": 1, "maxPathLength": 8})
+
+
+def test_nrc_in_the_path_tracer(hip_device):
+    """A closed box at 64x64 with paths ending at the cache after bounce 2:
+    the online training loss falls, the accumulated image stays within a few
+    percent of the plain path tracer's, and the trained cache is closer to
+    the plain reference than an untrained one (same set-up, learning rate
+    0)."""
+    from ospray_amd import api as A
+
+    sc = _box_scene()
+    ref, _, _ = _render(sc, {}, 512)
+    rp = {"nrc": True, "nrcBounce": 2, "nrcTrainPaths": 512, "nrcTrainSteps": 4}
     img, s, _ = _render(sc, rp, 1)
-    first, steps = A.nrc_state(s.renderer)[:2]
-    assert 1 <= steps <= 4 and first > 0
+    first, steps, queries, records = A.nrc_state(s.renderer)
+    assert 1 <= steps <= 4 and first > 0 and queries > 1000 and records > 64
     img, s, fb = _render(sc, rp, 179, s=s)
     losses = []
     for _ in range(20):  # single-batch losses are noisy: the median of the last 20
@@ -240,9 +309,8 @@ def test_nrc_in_the_path_tracer(hip_device):
         losses.append(A.nrc_state(s.renderer)[0])
     last, steps = float(np.median(losses)), A.nrc_state(s.renderer)[1]
     assert steps >= 200 and last < 0.5 * first, (first, last, steps)
-    # a fresh accumulation with the trained cache
-    trained, _, _ = _render(sc, rp, 64, s=s)
-    assert abs(trained.mean() / ref.mean() - 1.0) < 0.05
+    trained, _, _ = _render(sc, rp, 64, s=s)  # a fresh accumulation with the trained cache
+    assert abs(trained.mean() / ref.mean() - 1.0) < 0.05, (trained.mean(), ref.mean())
     untrained, _, _ = _render(sc, dict(rp, nrcLearningRate=0.0, nrcSeed=5), 64)
     err = lambda a: float(np.sqrt(np.mean((a - ref) ** 2)))
-    assert err(trained) < err(untrained), (err(trained), err(untrained))
+    assert err(trained) < 0.8 * err(untrained), (err(trained), err(untrained))
