@@ -122,3 +122,83 @@ def test_c_application_compiles_against_headers(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-o", exe, src, f"-I{ROOT}/include", f"-L{K.LIBDIR}",
                     "-lospray", f"-Wl,-rpath,{K.LIBDIR}", "-lm"], check=True)
     assert os.path.exists(exe)
+
+
+@pytest.mark.parametrize("std", ["c++11", "c++17"])
+def test_cpp_application_compiles_against_wrapper(tmp_path, std):
+    """The C++ wrapper (include/ospray/ospray_cpp.h) compiles warning-free
+    in an application and links against libospray.so."""
+    src = os.path.join(ROOT, "tests", "apps", "pathtracer_cpp_app.cpp")
+    exe = str(tmp_path / "app")
+    subprocess.run(["g++", f"-std={std}", "-Wall", "-Wextra", "-Werror", "-o", exe, src, f"-I{ROOT}/include",
+                    f"-L{K.LIBDIR}", "-lospray", f"-Wl,-rpath,{K.LIBDIR}"], check=True)
+    assert os.path.exists(exe)
+
+
+_WRAPPER_CHECKS = r"""
+#include <array>
+#include <vector>
+#include "ospray/ospray_cpp.h"
+#include "ospray/ospray_cpp/ext/glm.h"
+#include "ospray/ospray_cpp/ext/rkcommon.h"
+using namespace ospray;
+static_assert(OSPTypeFor<float>::value == OSP_FLOAT, "");
+static_assert(OSPTypeFor<const char *>::value == OSP_STRING, "");
+static_assert(OSPTypeFor<OSPFrameBufferFormat>::value == OSP_UINT, "");
+static_assert(OSPTypeFor<cpp::Camera>::value == OSP_CAMERA, "");
+static_assert(OSPTypeFor<cpp::GeometricModel>::value == OSP_GEOMETRIC_MODEL, "");
+static_assert(OSPTypeFor<cpp::CopiedData>::value == OSP_DATA, "");
+static_assert(OSPTypeFor<cpp::MovedData>::value == OSP_DATA, "");
+static_assert(OSPTypeFor<glm::vec3>::value == OSP_VEC3F, "");
+static_assert(OSPTypeFor<glm::mat4x3>::value == OSP_AFFINE3F, "");
+static_assert(OSPTypeFor<rkcommon::math::vec3ul>::value == OSP_VEC3UL, "");
+static_assert(OSPTypeFor<rkcommon::math::box1f>::value == OSP_BOX1F, "");
+static_assert(OSPTypeFor<rkcommon::math::affine3f>::value == OSP_AFFINE3F, "");
+static_assert(OSPTypeFor<std::vector<int>>::value == OSP_UNKNOWN, "");
+static_assert(sizeof(cpp::World) == sizeof(OSPWorld) && sizeof(cpp::SharedData) == sizeof(OSPData), "");
+OSPTYPEFOR_DEFINITION(glm::vec3);
+void use(const rkcommon::math::vec3ul &dims, const std::array<float, 8> &a, const float *p)
+{
+  cpp::SharedData d3(p, dims);             // 3D dimensions from a registered vector
+  cpp::CopiedData c(a);                    // std::array
+  cpp::Volume v("structuredRegular");
+  v.setParam("data", d3);
+  v.setParam("gridSpacing", glm::vec3{1.f, 1.f, 1.f});
+  v.setParam("name", std::string("v"));
+  v.removeParam("name");
+  cpp::Device dev("hip");
+  dev.setParam("numThreads", 4);
+  dev.setParam("logOutput", "cerr");
+}
+"""
+
+
+def test_cpp_wrapper_type_registry(tmp_path):
+    """OSPTypeFor of the wrapper classes, the enums and the glm / rkcommon
+    registrations (ext/glm.h, ext/rkcommon.h); glm and rkcommon are not in
+    this image, so minimal types with their names stand in for them, enough
+    to instantiate the registrations."""
+    for sub, names in (("glm", ["vec2.hpp", "vec3.hpp", "vec4.hpp", "mat2x2.hpp", "mat3x2.hpp", "mat3x3.hpp",
+                                "mat4x3.hpp"]), ("rkcommon/math", ["AffineSpace.h", "box.h"])):
+        d = tmp_path / sub
+        d.mkdir(parents=True)
+        for n in names:
+            # distinct text: gcc's #pragma once takes identical files for one
+            (d / n).write_text(f"#pragma once\n// {sub}/{n}\n#include \"types.h\"\n")
+    (tmp_path / "glm" / "types.h").write_text(
+        "#pragma once\nnamespace glm {\n" + "".join(
+            f"struct {t} {{ float v[{n}]; }};\n" for t, n in
+            (("vec2", 2), ("vec3", 3), ("vec4", 4), ("ivec2", 2), ("ivec3", 3), ("ivec4", 4), ("uvec2", 2),
+             ("uvec3", 3), ("uvec4", 4), ("mat2x2", 4), ("mat3x3", 9), ("mat3x2", 6), ("mat4x3", 12))) + "}\n")
+    rk = ["vec2uc", "vec3uc", "vec4uc", "vec2c", "vec3c", "vec4c", "vec2us", "vec3us", "vec4us", "vec2s", "vec3s",
+          "vec4s", "vec2i", "vec3i", "vec4i", "vec2ui", "vec3ui", "vec4ui", "vec2l", "vec3l", "vec4l", "vec2ul",
+          "vec4ul", "vec2f", "vec3f", "vec4f", "vec2d", "vec3d", "vec4d", "box1i", "box2i", "box3i", "box4i",
+          "box1f", "box2f", "box3f", "box4f", "linear2f", "linear3f", "affine2f", "affine3f", "quatf"]
+    (tmp_path / "rkcommon" / "math" / "types.h").write_text(
+        "#pragma once\n#include <cstdint>\nnamespace rkcommon { namespace math {\n"
+        "struct vec3ul { uint64_t v[3]; uint64_t operator[](int i) const { return v[i]; } };\n"
+        + "".join(f"struct {t} {{ float v[4]; }};\n" for t in rk) + "} }\n")
+    src = tmp_path / "checks.cpp"
+    src.write_text(_WRAPPER_CHECKS)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-c", "-o", str(tmp_path / "checks.o"),
+                    str(src), f"-I{ROOT}/include", f"-I{tmp_path}"], check=True)

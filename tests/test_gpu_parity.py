@@ -349,3 +349,29 @@ def test_c_application_runs(hip_device, tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "checksum" in r.stdout and int(r.stdout.split()[-1]) > 0
     assert os.path.getsize(tmp_path / "out.ppm") > 160 * 120 * 3
+
+
+def test_cpp_application_matches_c_application(hip_device, tmp_path):
+    """The same scene written against the C++ wrapper renders the same image
+    as the C application, byte for byte; its pick hits the red triangles and
+    the world's bounds hold the vertices' box."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    link = [f"-I{root}/include", f"-L{K.LIBDIR}", "-lospray", f"-Wl,-rpath,{K.LIBDIR}"]
+    c_exe, cpp_exe = str(tmp_path / "c_app"), str(tmp_path / "cpp_app")
+    subprocess.run(["gcc", "-std=c99", "-o", c_exe, os.path.join(root, "tests", "apps", "pathtracer_app.c")]
+                   + link + ["-lm"], check=True)
+    subprocess.run(["g++", "-std=c++17", "-o", cpp_exe, os.path.join(root, "tests", "apps", "pathtracer_cpp_app.cpp")]
+                   + link, check=True)
+    outs = []
+    for exe, name in ((c_exe, "c.ppm"), (cpp_exe, "cpp.ppm")):
+        r = subprocess.run([exe, "--osp:load-modules=hip", "--osp:device=hip", str(tmp_path / name), "3"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs.append(r.stdout.splitlines())
+    assert outs[1][0] == outs[0][0]  # frames, variance, checksum
+    assert (tmp_path / "c.ppm").read_bytes() == (tmp_path / "cpp.ppm").read_bytes()
+    assert outs[1][1] == "pick hit 1 model 0 primID 0"
+    # bounds are the BVH's root box: the vertices' box, conservatively padded
+    b = np.array([float(x) for x in outs[1][2].split()[1:]])
+    box = np.array([-3.0, -1.5, -1.0, 3.0, 1.0, 6.0])
+    assert np.all(b[:3] <= box[:3]) and np.all(b[3:] >= box[3:]) and np.allclose(b, box, atol=1e-4)
