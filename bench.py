@@ -292,6 +292,9 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)
+    ap.add_argument("--device-param", action="append", default=[], metavar="KEY=INT",
+                    help="integer parameter of the hip device (traceBlocksPerCU, shadeBlocksPerCU, "
+                         "pathBudgetMB); repeatable")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time the plain copy kernel on BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -329,7 +332,11 @@ def main() -> int:
     ntris = sc.num_triangles
     log(f"scene generated in {time.perf_counter() - t:.1f} s ({ntris} instanced triangles)")
 
-    A.init("hip", profileKernels=True)
+    dparams = {}
+    for kv in args.device_param:
+        k, _, v = kv.partition("=")
+        dparams[k] = int(v)
+    A.init("hip", profileKernels=True, **dparams)
     name, cus = A.device_info()
     log(f"device {name}, {cus} CUs")
     t = time.perf_counter()
@@ -431,7 +438,8 @@ def main() -> int:
            "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": workload, "width": sc.width, "height": sc.height, "spp": args.spp,
-                      "max_path_length": mpl, "instanced_triangles": int(ntris), "device": name},
+                      "max_path_length": mpl, "instanced_triangles": int(ntris), "device": name,
+                      **({"device_params": dparams} if dparams else {})},
            "rays_per_s": round(rays_total * args.steps / elapsed, 1),
            "kernels_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats.items())},
            "traversal": {k: {"nodes_per_ray": round(v["nodes_per_ray"], 2), "prims_per_ray": round(v["prims_per_ray"], 2),

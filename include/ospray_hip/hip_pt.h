@@ -97,9 +97,17 @@ HPT_API hipError_t hpt_color_convert(const float *color, int npix, int format, v
 HPT_API hipError_t hpt_trace_rays(const HptScene *S, const float *rays8, int n, float *outTUV, uint32_t *outIDs,
     uint32_t *overflow, int residentBlocks, unsigned long long *stats, hipStream_t s);
 
+/* threads per block of the traversal kernels */
 HPT_API int hpt_trace_block_size(void);
+/* 32-bit words of the scratch row each resident traversal thread needs in
+ * `overflow` (HptPaths and hpt_trace_rays): the traversal-stack overflow,
+ * then the transparent hits an occlusion traversal records */
 HPT_API int hpt_overflow_depth(void);
+/* traversal stack entries per ray (LDS + overflow) */
 HPT_API int hpt_stack_capacity(void);
+/* transparent hits the occlusion pass records per shadow ray for the
+ * transparency pass (HptPaths.transHits holds this many per queue slot) */
+HPT_API int hpt_trans_record_hits(void);
 
 #ifdef __cplusplus
 }

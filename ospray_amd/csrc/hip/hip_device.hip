@@ -921,7 +921,7 @@ class HipDevice : public Device
   int numCUs = 0;
   std::string gpuName;
   // path state, shared by all framebuffers of this device
-  DevBuf pathBuf, recBuf, queueBuf, counters, overflow, stats;
+  DevBuf pathBuf, recBuf, queueBuf, transBuf, counters, overflow, stats;
   // radiance cache per-pass buffers: vertex records, queues, raw inputs,
   // bf16 features, predictions, training targets
   DevBuf nrcRec, nrcQueues, nrcRaw, nrcFeat, nrcOut, nrcTarget;
@@ -1879,6 +1879,11 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   pathBuf.ensure(S * perSlot);
   recBuf.ensure(R * 80);
   queueBuf.ensure((2 * S + 3 * R) * 4);
+  // recorded transparent hits of up to 4M transparency-queue records (the
+  // rest are traversed again); 4 words per hit + a count
+  const size_t transCap = std::min<size_t>(R, (size_t)1 << 22);
+  const size_t transHitBytes = transCap * 16 * hpt_trans_record_hits();
+  transBuf.ensure(transHitBytes + transCap * 4);
   char *b = (char *)pathBuf.ptr;
   auto take = [&](size_t bytes) {
     char *p = b;
@@ -1918,6 +1923,9 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.shadowQueue = q + 2 * S;
   paths.transQueue = q + 2 * S + R;
   paths.neeSampleQueue = q + 2 * S + 2 * R;
+  paths.transHits = (float *)transBuf.ptr;
+  paths.transCount = (uint32_t *)((char *)transBuf.ptr + transHitBytes);
+  paths.transCap = (int32_t)transCap;
   paths.counters = (int32_t *)counters.ptr;
   paths.overflow = (uint32_t *)overflow.ptr;
   paths.stats = (unsigned long long *)stats.ptr;

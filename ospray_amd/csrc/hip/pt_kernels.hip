@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[2 * qin];
-  uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
+  uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
   for (int base = nextBatch(&P.counters[kCntFetchExtend]); base < n; base = nextBatch(&P.counters[kCntFetchExtend])) {
@@ -736,7 +736,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[kCntShadow];
-  uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
+  uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
   for (int base = nextBatch(&P.counters[kCntFetchShadow]); base < n; base = nextBatch(&P.counters[kCntFetchShadow])) {
@@ -749,8 +749,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
       const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
       Hit h;
       bool sawT;
-      const bool occluded =
-          traceRay<MODE_OCCLUSION, STATS>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 0.f, o4.w, h, lds, ovf, sawT, st);
+      *recordedCount(ovf) = 0;
+      const bool occluded = traceRay<MODE_OCCLUSION, STATS, true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 0.f,
+          o4.w, h, lds, ovf, sawT, st);
       rays++;
       if (!occluded) {
         if (sawT) {
@@ -767,8 +768,20 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
       }
     }
     const int slot = enqueue(&P.counters[kCntTrans], toTrans);
-    if (toTrans)
+    if (toTrans) {
       P.transQueue[slot] = r;
+      if (slot < P.transCap) {
+        // hand the hits traceRay recorded to the transparency pass
+        const uint32_t nrec = *recordedCount(ovf);
+        if (nrec <= (uint32_t)kTransRecHits) {
+          const float4 *rec = recordedHits(ovf);
+          float4 *dst = reinterpret_cast<float4 *>(P.transHits) + (size_t)slot * kTransRecHits;
+          for (uint32_t k = 0; k < nrec; k++)
+            dst[k] = rec[k];
+        }
+        P.transCount[slot] = nrec <= (uint32_t)kTransRecHits ? nrec : 0xffffffffu;
+      }
+    }
   }
   if (STATS) {
     waveAdd(&P.stats[3], st.nodes);
@@ -785,7 +798,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[kCntTrans];
-  uint32_t *ovf = P.overflow + (size_t)globalId() * kOverflowDepth;
+  uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
   for (int base = nextBatch(&P.counters[kCntFetchTrans]); base < n; base = nextBatch(&P.counters[kCntFetchTrans])) {
@@ -803,10 +816,41 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
     const float tOriginal = o4.w;
     float t0 = 0.f;
     Hit h;
+    // the hits the occlusion pass recorded, if it recorded them all: the
+    // closest hit in (t0, tOriginal] is the smallest recorded (t, instance,
+    // model, primitive, half) with t > t0 (the first recorded on a full tie:
+    // a sphere's entry before its exit), exactly what the traversal returns
+    const uint32_t nrec = i < P.transCap ? P.transCount[i] : 0xffffffffu;
+    const float4 *rec = reinterpret_cast<const float4 *>(P.transHits) + (size_t)i * kTransRecHits;
     for (;;) {
-      bool dummy;
-      traceRay<MODE_CLOSEST, STATS>(S, org, dir, t0, tOriginal, h, lds, ovf, dummy, st);
-      rays++;
+      if (nrec <= (uint32_t)kTransRecHits) {
+        h.hit = false;
+        float4 sel = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < (int)nrec; k++) {
+          const float4 a = rec[k];
+          if (!(a.x > t0))
+            continue;
+          const DPrim &Pk = S.prims[__float_as_uint(a.z)];
+          const uint32_t prim = __float_as_uint(reinterpret_cast<const float4 *>(&Pk)[0].w);
+          const uint32_t mk = __float_as_uint(reinterpret_cast<const float4 *>(&Pk)[1].w);
+          const uint32_t half = (mk >> 28) == PRIM_QUAD1 ? 1u : 0u;
+          if (better(a.x, __float_as_uint(a.y), mk & 0x0fffffffu, prim, half, h)) {
+            h.hit = true;
+            h.t = a.x;
+            h.inst = __float_as_uint(a.y);
+            h.model = mk & 0x0fffffffu;
+            h.prim = prim;
+            h.half = half;
+            sel = a;
+          }
+        }
+        if (h.hit)
+          hitFromRecord(S, org, dir, sel, h);
+      } else {
+        bool dummy;
+        traceRay<MODE_CLOSEST, STATS>(S, org, dir, t0, tOriginal, h, lds, ovf, dummy, st);
+        rays++;
+      }
       if (!h.hit)
         break;
       DG dg;
@@ -1117,7 +1161,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DScene S, const floa
     uint32_t *outIDs, uint32_t *ovfBase, unsigned long long *stats)
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
-  uint32_t *ovf = ovfBase + (size_t)globalId() * kOverflowDepth;
+  uint32_t *ovf = ovfBase + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   for (int i = globalId(); i < n; i += gridStride()) {
     const float *r = rays8 + 8 * (size_t)i;
@@ -1344,7 +1388,8 @@ hipError_t hpt_trace_rays(const HptScene *S, const float *rays8, int n, float *o
 }
 
 int hpt_trace_block_size() { return kTraceBlock; }
-int hpt_overflow_depth() { return kOverflowDepth; }
+int hpt_overflow_depth() { return kThreadRow; }
+int hpt_trans_record_hits() { return kTransRecHits; }
 int hpt_stack_capacity() { return kLdsDepth + kOverflowDepth; }
 
 } // extern "C"

@@ -25,6 +25,12 @@ namespace hpt {
 constexpr int kTraceBlock = 128;
 constexpr int kLdsDepth = 24;
 constexpr int kOverflowDepth = 104;
+// transparent hits an occlusion traversal records for the transparency pass
+constexpr int kTransRecHits = 4;
+// words of scratch per resident traversal thread: the stack overflow, then
+// kTransRecHits recorded hits of 4 words (one float4, 16-byte aligned),
+// then their count (padded to 4 words)
+constexpr int kThreadRow = kOverflowDepth + 4 * kTransRecHits + 4;
 constexpr int32_t kMarker = 0x7fffffff;
 
 struct Hit
@@ -122,6 +128,27 @@ HPT_HD bool intersectTriangle(vec3f org, float t0, const TriPre &pc, vec3f v0, v
   return true;
 }
 
+// both roots of the ray/sphere test, with the arithmetic of
+// intersectSphere: bit 0 = entry (tE, NE) in (t0, tmax], bit 1 = exit
+HPT_HD int sphereRoots(vec3f org, vec3f d, float t0, vec3f center, float radius, float tmax, float &tE, vec3f &NE,
+    float &tX, vec3f &NX)
+{
+  const float rd2 = 1.0f / dot(d, d);
+  const vec3f CO = center - org;
+  const float projCO = dot(CO, d) * rd2;
+  const vec3f perp = CO - projCO * d;
+  const float l2 = dot(perp, perp);
+  const float r2 = radius * radius;
+  if (l2 > r2)
+    return 0;
+  const float td = sqrtf((r2 - l2) * rd2);
+  tE = projCO - td;
+  tX = projCO + td;
+  NE = (-td) * d - perp;
+  NX = td * d - perp;
+  return (tE > t0 && tE <= tmax ? 1 : 0) | (tX > t0 && tX <= tmax ? 2 : 0);
+}
+
 HPT_HD bool intersectSphere(vec3f org, vec3f d, float t0, vec3f center, float radius, float tmax, float &t, vec3f &N)
 {
   const float rd2 = 1.0f / dot(d, d);
@@ -200,13 +227,40 @@ __device__ __forceinline__ float slabNear(float lox, float loy, float loz, float
 // Traversal modes: MODE_CLOSEST returns the closest hit; MODE_OCCLUSION
 // returns as soon as a primitive of an opaque model is hit and otherwise
 // reports in `sawTransparent` whether any (non-opaque) primitive was hit.
+// With RECORD, an occlusion traversal also records every hit of a
+// non-opaque primitive in (t0, tfar] (both roots of a sphere) in the
+// thread's scratch row after the stack overflow (recordedHits): one float4
+// (t, instance, index into S.prims, root: 1 = a sphere's exit) per hit, up
+// to kTransRecHits, and in the word after them the count of all of them,
+// also those beyond the capacity (the caller sets it to 0; it lives in
+// memory rather than a register since transparent hits are rare).
+// hitFromRecord rebuilds the closest-hit record of one of them.
+
+__device__ __forceinline__ float4 *recordedHits(uint32_t *ovf)
+{
+  return reinterpret_cast<float4 *>(ovf + kOverflowDepth);
+}
+
+__device__ __forceinline__ uint32_t *recordedCount(uint32_t *ovf)
+{
+  return ovf + kOverflowDepth + 4 * kTransRecHits;
+}
+
+__device__ __forceinline__ void recordHit(uint32_t *ovf, float t, uint32_t inst, uint32_t primIndex, uint32_t exitRoot)
+{
+  uint32_t *cnt = recordedCount(ovf);
+  const uint32_t c = *cnt;
+  if (c < (uint32_t)kTransRecHits)
+    recordedHits(ovf)[c] = make_float4(t, __uint_as_float(inst), __uint_as_float(primIndex), __uint_as_float(exitRoot));
+  *cnt = c + 1;
+}
 enum TraceMode
 {
   MODE_CLOSEST = 0,
   MODE_OCCLUSION = 1
 };
 
-template <int MODE, bool STATS>
+template <int MODE, bool STATS, bool RECORD = false>
 __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
     uint32_t *lds, uint32_t *ovf, bool &sawTransparent, TraceStats &st)
 {
@@ -314,6 +368,23 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
       const uint32_t kind = mk >> 28;
       if (STATS)
         st.prims++;
+      if (MODE == MODE_OCCLUSION && kind == PRIM_SPHERE) {
+        float tE, tX;
+        vec3f NE, NX;
+        const int roots = sphereRoots(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, best.t, tE, NE, tX, NX);
+        if (!roots)
+          continue;
+        if (S.models[modelOffset + model].opaque)
+          return true;
+        sawTransparent = true;
+        if (RECORD) {
+          if (roots & 1)
+            recordHit(ovf, tE, (uint32_t)inst, first + k, 0);
+          if (roots & 2)
+            recordHit(ovf, tX, (uint32_t)inst, first + k, 1);
+        }
+        continue;
+      }
       float t, u = 0.f, v = 0.f;
       vec3f Ng;
       bool h;
@@ -338,6 +409,8 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
         if (S.models[modelOffset + model].opaque)
           return true;
         sawTransparent = true;
+        if (RECORD)
+          recordHit(ovf, t, (uint32_t)inst, first + k, 0);
         continue;
       }
       if (better(t, (uint32_t)inst, model, primID, half, best)) {
@@ -357,6 +430,50 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
     cur = pop();
   }
   return best.hit;
+}
+
+// The hit record the closest-hit traversal of (worg, wdir) returns for a
+// hit recorded by the occlusion traversal: the same object ray (instance
+// transform, triPrecompute) and the same primitive test give the same
+// t, u, v and Ng bit for bit.
+__device__ __forceinline__ void hitFromRecord(const DScene &S, vec3f worg, vec3f wdir, float4 r, Hit &h)
+{
+  const uint32_t inst = __float_as_uint(r.y);
+  const DInstance &I = S.instances[inst];
+  const linear3f rl = {{I.rcp[0], I.rcp[1], I.rcp[2]}, {I.rcp[3], I.rcp[4], I.rcp[5]}, {I.rcp[6], I.rcp[7], I.rcp[8]}};
+  const vec3f o = mul(rl, worg) + v3(I.rcp[9], I.rcp[10], I.rcp[11]);
+  const vec3f d = mul(rl, wdir);
+  const DPrim &P = S.prims[__float_as_uint(r.z)];
+  const float4 pa = reinterpret_cast<const float4 *>(&P)[0];
+  const float4 pb = reinterpret_cast<const float4 *>(&P)[1];
+  const float4 pcv = reinterpret_cast<const float4 *>(&P)[2];
+  const uint32_t mk = __float_as_uint(pb.w);
+  const uint32_t kind = mk >> 28;
+  h.hit = true;
+  h.t = r.x;
+  h.inst = inst;
+  h.model = mk & 0x0fffffffu;
+  h.prim = __float_as_uint(pa.w);
+  h.half = 0;
+  h.u = h.v = 0.f;
+  if (kind == PRIM_SPHERE) {
+    float tE, tX;
+    vec3f NE, NX;
+    sphereRoots(o, d, -kInf, v3(pa.x, pa.y, pa.z), pb.x, kInf, tE, NE, tX, NX);
+    h.Ng = __float_as_uint(r.w) ? NX : NE;
+  } else {
+    const vec3f A = v3(pa.x, pa.y, pa.z), B = v3(pb.x, pb.y, pb.z), C = v3(pcv.x, pcv.y, pcv.z);
+    float t, u = 0.f, v = 0.f;
+    intersectTriangle(o, -kInf, triPrecompute(d), A, B, C, kInf, t, u, v);
+    h.Ng = cross(B - A, C - A);
+    h.u = u;
+    h.v = v;
+    if (kind == PRIM_QUAD1) {
+      h.half = 1;
+      h.u = 1.0f - u;
+      h.v = 1.0f - v;
+    }
+  }
 }
 
 } // namespace hpt

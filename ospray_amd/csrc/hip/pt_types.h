@@ -231,7 +231,13 @@ struct DPaths
   // that k_shade appends to extension queue q and to the NEE sample queue
   // with a single atomic
   int32_t *counters;
-  uint32_t *overflow; // traversal stack overflow, per resident thread
+  uint32_t *overflow; // traversal scratch row per resident thread: stack overflow, recorded hits
+  // transparent hits recorded by the occlusion pass for the records of the
+  // transparency queue, by queue slot < transCap: 4 float4 and a count
+  // (0xffffffff: not recorded, the transparency pass traverses)
+  float *transHits;
+  uint32_t *transCount;
+  int32_t transCap;
   // radiance cache (when enabled): per path slot the vertex record
   // (P, thr.x | wo, thr.y | Ns, thr.z | albedo, -) and the contribution
   // before the vertex; compact queues of the query and training slots

@@ -179,6 +179,53 @@ def test_geometry_lights_bit_exact(hip_device, geometry_lights):
     assert col[..., :3].max() > 1.0  # emitters seen directly
 
 
+def _c0_transparent_layers(w=48, h=48, spp=2, max_path_length=3):
+    """C0 under transparent occluders of the quad light's shadow rays: six
+    stacked obj quads (d 0.8), and in a second instance (scaled in x and z)
+    two more layers and two transparent sphere primitives, a large one
+    around part of the ground and a small one above the layers.  Shadow
+    rays from the ground cross more hits than the occlusion pass records,
+    so the transparency pass traverses again; those from the upper layers
+    and the small sphere cross four or fewer, which the occlusion pass
+    records (rays leaving the small sphere's lit side start inside it:
+    exit root only)."""
+    sc = scenes.c0(w, h, spp=spp, max_path_length=max_path_length)
+    def layers(n, y0):
+        v, ix = [], []
+        for k in range(n):
+            y, b = y0 + k, 4 * k
+            v += [[-6, y, -6], [6, y, -6], [6, y, 6], [-6, y, 6]]
+            ix.append([b, b + 1, b + 2, b + 3])
+        return scenes.Geometry("quads", vertex=np.array(v, np.float32), index=np.array(ix, np.uint32))
+    sc.materials.append({"type": "obj", "kd": (0.6, 0.7, 0.8), "d": 0.8, "tf": (0.3, 0.2, 0.1)})
+    mat = len(sc.materials) - 1
+    sc.materials.append({"type": "obj", "kd": (0.5, 0.5, 0.3), "d": 0.6, "tf": (0.4, 0.4, 0.4)})
+    smat = len(sc.materials) - 1
+    sc.geometries += [layers(6, 1.0), layers(2, 1.5),
+                      scenes.Geometry("spheres", spheres=np.array([[0.0, -6.0, 0.0, 3.0], [0.0, 7.0, 0.0, 0.6]],
+                                                                   np.float32))]
+    g = len(sc.geometries)
+    sc.models += [scenes.Model(geometry=g - 3, material=mat), scenes.Model(geometry=g - 2, material=mat),
+                  scenes.Model(geometry=g - 1, material=smat)]
+    m = len(sc.models)
+    sc.groups += [[m - 3], [m - 2, m - 1]]
+    sc.instances += [scenes.Instance(len(sc.groups) - 2),
+                     scenes.Instance(len(sc.groups) - 1, xfm=(0.5, 0, 0, 0, 1, 0, 0, 0, 0.5, 1.0, 0.0, -1.0))]
+    return sc
+
+
+@pytest.mark.parametrize("max_path_length", [3, 8])
+def test_transparent_shadows_bit_exact(hip_device, max_path_length):
+    # recorded hits (up to 4 per shadow ray) and the traversal fallback of
+    # the transparency pass, against the oracle's closest-hit loop
+    sc = _c0_transparent_layers(max_path_length=max_path_length)
+    s, fb, ofb, _, oren = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    assert col[..., :3].mean() > 0.01
+
+
 def _volume(anisotropy=0.3, user_id=None, seed=3):
     """A 24x22x20 random grid (box smoothed) with an 8-point transfer
     function: non-cubic, so a swapped axis shows."""
