@@ -292,9 +292,9 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)
-    ap.add_argument("--device-param", action="append", default=[], metavar="KEY=INT",
-                    help="integer parameter of the hip device (traceBlocksPerCU, shadeBlocksPerCU, "
-                         "pathBudgetMB); repeatable")
+    ap.add_argument("--device-param", action="append", default=[], metavar="KEY=VALUE",
+                    help="integer or boolean (true/false) parameter of the hip device (traceBlocksPerCU, "
+                         "shadeBlocksPerCU, pathBudgetMB, traversalStats); repeatable")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time the plain copy kernel on BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -335,7 +335,7 @@ def main() -> int:
     dparams = {}
     for kv in args.device_param:
         k, _, v = kv.partition("=")
-        dparams[k] = int(v)
+        dparams[k] = v.lower() == "true" if v.lower() in ("true", "false") else int(v)
     A.init("hip", profileKernels=True, **dparams)
     name, cus = A.device_info()
     log(f"device {name}, {cus} CUs")

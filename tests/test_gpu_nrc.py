@@ -290,10 +290,12 @@ def _box_scene(w=64, h=64):
 
 def test_nrc_in_the_path_tracer(hip_device):
     """A closed box at 64x64 with paths ending at the cache after bounce 2:
-    the online training loss falls, the accumulated image stays within a few
-    percent of the plain path tracer's, and the trained cache is closer to
-    the plain reference than an untrained one (same set-up, learning rate
-    0)."""
+    the online training loss falls to under half the initial network's, the
+    accumulated image stays within a few percent of the plain path tracer's,
+    and the trained cache is closer to the plain reference than an untrained
+    one (same set-up, learning rate 0).  Single-batch losses are noisy and
+    the batches' composition follows the nondeterministic queue order, so
+    losses are compared as medians over 20 frames."""
     from ospray_amd import api as A
 
     sc = _box_scene()
@@ -308,9 +310,15 @@ def test_nrc_in_the_path_tracer(hip_device):
         img, s, fb = _render(sc, rp, 1, s=s, fb=fb)
         losses.append(A.nrc_state(s.renderer)[0])
     last, steps = float(np.median(losses)), A.nrc_state(s.renderer)[1]
-    assert steps >= 200 and last < 0.5 * first, (first, last, steps)
     trained, _, _ = _render(sc, rp, 64, s=s)  # a fresh accumulation with the trained cache
     assert abs(trained.mean() / ref.mean() - 1.0) < 0.05, (trained.mean(), ref.mean())
+    # the initial network's loss: learning rate 0 keeps the weights of nrcSeed
+    init_losses, su, fbu = [], None, None
+    for _ in range(20):
+        _, su, fbu = _render(sc, dict(rp, nrcLearningRate=0.0), 1, s=su, fb=fbu)
+        init_losses.append(A.nrc_state(su.renderer)[0])
+    init = float(np.median(init_losses))
+    assert steps >= 200 and last < 0.5 * init, (first, init, last, steps)
     untrained, _, _ = _render(sc, dict(rp, nrcLearningRate=0.0, nrcSeed=5), 64)
     err = lambda a: float(np.sqrt(np.mean((a - ref) ** 2)))
     assert err(trained) < 0.8 * err(untrained), (err(trained), err(untrained))
