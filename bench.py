@@ -132,7 +132,7 @@ def bench_c2(args) -> int:
 
     sc = scenes.c1()
     views = scenes.c2_views()
-    A.init("hip")
+    A.init("hip", **args.dparams)
     name, _ = A.device_info()
     s = scenes.to_ospray(sc)
     fbs, rens, cams = [], [], []
@@ -168,7 +168,8 @@ def bench_c2(args) -> int:
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "C2: 64 views of the C1 scene, seeded positions on a radius-30 sphere, "
                                   "resolutions {256^2, 512^2, 768x432, 1024^2}, spp {1,2,4,8}, maxPathLength 8",
-                      "samples_per_batch": samples, "device": name},
+                      "samples_per_batch": samples, "device": name,
+                      **({"device_params": args.dparams} if args.dparams else {})},
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
            "autotuner": {"traceBlocksPerCU": best[0], "shadeBlocksPerCU": best[1], "pathBudgetMB": best[2],
                          "best_eager_batch_ms": round(best_ms, 3)},
@@ -192,7 +193,7 @@ def bench_c3(args) -> int:
 
     w, h = args.width or 1920, args.height or 1080
     sc = scenes.c1(width=w, height=h, spp=1)
-    A.init("hip", profileKernels=True)
+    A.init("hip", profileKernels=True, **args.dparams)
     name, _ = A.device_info()
     L = K.lib()
 
@@ -310,6 +311,10 @@ def main() -> int:
     from ospray_amd import capi as K
     from ospray_amd import scenes
 
+    args.dparams = {}
+    for kv in args.device_param:
+        k, _, v = kv.partition("=")
+        args.dparams[k] = v.lower() == "true" if v.lower() in ("true", "false") else int(v)
     if args.workload == "c3":
         return bench_c3(args)
     if args.workload == "c2":
@@ -332,10 +337,7 @@ def main() -> int:
     ntris = sc.num_triangles
     log(f"scene generated in {time.perf_counter() - t:.1f} s ({ntris} instanced triangles)")
 
-    dparams = {}
-    for kv in args.device_param:
-        k, _, v = kv.partition("=")
-        dparams[k] = v.lower() == "true" if v.lower() in ("true", "false") else int(v)
+    dparams = args.dparams
     A.init("hip", profileKernels=True, **dparams)
     name, cus = A.device_info()
     log(f"device {name}, {cus} CUs")

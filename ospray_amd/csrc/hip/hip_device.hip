@@ -16,6 +16,7 @@
 #include <cstring>
 #include <cmath>
 #include <map>
+#include <set>
 #include <memory>
 #include <random>
 #include <thread>
@@ -85,6 +86,26 @@ struct DevBuf
 };
 
 class HipDevice;
+
+// The path state of one frame in flight: wavefront buffers, queues and
+// counters, the traversal scratch rows, and the stream the frame runs on.
+// Frames on different lanes run concurrently; lane 0 runs on the device
+// stream.
+struct PathLane
+{
+  hipStream_t stream = nullptr;
+  bool ownsStream = false;
+  DevBuf pathBuf, recBuf, queueBuf, transBuf, counters, overflow;
+  hpt::DPaths paths{};
+  size_t pathSlots = 0, recSlots = 0;
+  bool auxAllocated = false;
+  float *auxBase = nullptr;
+  ~PathLane()
+  {
+    if (ownsStream && stream)
+      (void)hipStreamDestroy(stream);
+  }
+};
 
 // ---- data helpers -------------------------------------------------------------------
 
@@ -920,16 +941,16 @@ class HipDevice : public Device
   hipStream_t stream = nullptr;
   int numCUs = 0;
   std::string gpuName;
-  // path state, shared by all framebuffers of this device
-  DevBuf pathBuf, recBuf, queueBuf, transBuf, counters, overflow, stats;
+  // path state lanes (frames in flight), shared by all framebuffers of
+  // this device; `lane` is the one frames are enqueued on
+  std::vector<std::unique_ptr<PathLane>> lanes;
+  PathLane *lane = nullptr;
+  int batchStreams = 4; // lanes a batch of views spreads over
+  DevBuf stats;
   // radiance cache per-pass buffers: vertex records, queues, raw inputs,
   // bf16 features, predictions, training targets
   DevBuf nrcRec, nrcQueues, nrcRaw, nrcFeat, nrcOut, nrcTarget;
   size_t nrcSlots = 0;
-  hpt::DPaths paths{};
-  size_t pathSlots = 0, recSlots = 0;
-  bool auxAllocated = false;
-  float *auxBase = nullptr;
   int residentTraceBlocks = 0, shadeBlocks = 0;
   size_t pathBudgetBytes = (size_t)16 << 30;
   bool profile = false, travStats = false;
@@ -947,6 +968,9 @@ class HipDevice : public Device
       (void)hipEventDestroy(e);
     if (batchStart)
       (void)hipEventDestroy(batchStart);
+    for (auto e : laneEvents)
+      (void)hipEventDestroy(e);
+    lanes.clear();
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -958,6 +982,7 @@ class HipDevice : public Device
     travStats = getBool("traversalStats", false);
     traceBlocksPerCU = std::max(1, std::min(64, getInt("traceBlocksPerCU", traceBlocksPerCU)));
     shadeBlocksPerCU = std::max(1, std::min(64, getInt("shadeBlocksPerCU", shadeBlocksPerCU)));
+    batchStreams = std::max(1, std::min(8, getInt("batchStreams", batchStreams)));
     if (stream) {
       HIP_CHECK(hipStreamSynchronize(stream));
       sizeGrids();
@@ -979,12 +1004,34 @@ class HipDevice : public Device
       HIP_CHECK(hipGetDeviceProperties(&prop, dev));
       numCUs = prop.multiProcessorCount;
       gpuName = prop.gcnArchName;
-      HIP_CHECK(hipMalloc(&counters.ptr, 64));
-      counters.bytes = 64;
       stats.ensure(128);
       HIP_CHECK(hipMemset(stats.ptr, 0, 128));
+      ensureLane(0);
+      lane = lanes[0].get();
       sizeGrids();
     }
+  }
+
+  // lane i (created with its stream, counters and scratch rows on first use)
+  PathLane &ensureLane(int i)
+  {
+    while ((int)lanes.size() <= i) {
+      auto L = std::make_unique<PathLane>();
+      if (lanes.empty()) {
+        L->stream = stream;
+      } else {
+        HIP_CHECK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+        L->ownsStream = true;
+      }
+      L->counters.ensure(64);
+      if (residentTraceBlocks > 0)
+        L->overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
+      L->paths.counters = L->counters.as<int32_t>();
+      L->paths.overflow = L->overflow.as<uint32_t>();
+      L->paths.stats = (unsigned long long *)stats.ptr;
+      lanes.push_back(std::move(L));
+    }
+    return *lanes[i];
   }
 
   // resident grids of the bounce kernels (grid-stride loops) and the
@@ -995,7 +1042,11 @@ class HipDevice : public Device
     releaseGraph(); // a captured batch refers to the grid sizes and the overflow rows
     residentTraceBlocks = std::max(1, numCUs) * traceBlocksPerCU;
     shadeBlocks = std::max(1, numCUs) * shadeBlocksPerCU;
-    overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
+    for (auto &L : lanes) {
+      HIP_CHECK(hipStreamSynchronize(L->stream));
+      L->overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
+      L->paths.overflow = L->overflow.as<uint32_t>();
+    }
   }
 
   void commitObject(ManagedObject *obj) override
@@ -1026,9 +1077,9 @@ class HipDevice : public Device
     hipEvent_t a, b;
     HIP_CHECK(hipEventCreate(&a));
     HIP_CHECK(hipEventCreate(&b));
-    HIP_CHECK(hipEventRecord(a, stream));
+    HIP_CHECK(hipEventRecord(a, lane->stream));
     HIP_CHECK(fn());
-    HIP_CHECK(hipEventRecord(b, stream));
+    HIP_CHECK(hipEventRecord(b, lane->stream));
     pendingEvents.push_back({timerIndex(name), {a, b}});
   }
   void collectTimers()
@@ -1240,6 +1291,7 @@ class HipDevice : public Device
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   std::vector<hipEvent_t> viewEvents;
+  std::vector<hipEvent_t> laneEvents; // join of lane k into the device stream
   hipEvent_t batchStart = nullptr;
   void releaseGraph()
   {
@@ -1251,12 +1303,49 @@ class HipDevice : public Device
     graph = nullptr;
     graphKey.clear();
   }
+  // Lanes a batch runs on: views go round-robin to min(batchStreams,
+  // views) lanes, which fork from and join into the device stream.  One
+  // lane when two views share a framebuffer (their accumulation is
+  // ordered), when a renderer uses the radiance cache (one cache state
+  // per renderer, trained in view order) or with traversal statistics.
+  int batchLanes(const std::vector<BatchView> &v) const
+  {
+    if (travStats)
+      return 1;
+    std::set<ManagedObject *> fbs;
+    for (auto &x : v) {
+      if (!fbs.insert(x.fb).second || ((Renderer *)x.renderer)->nrc)
+        return 1;
+    }
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)batchStreams, v.size()));
+  }
   void enqueueViews(const std::vector<BatchView> &v, ManagedObject *world, std::vector<ManagedObject *> &futures)
   {
+    const int K = batchLanes(v);
+    for (int k = 0; k < K; k++)
+      ensureLane(k);
+    while ((int)laneEvents.size() < K) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      laneEvents.push_back(e);
+    }
     HIP_CHECK(hipEventRecord(batchStart, stream));
-    for (size_t i = 0; i < v.size(); i++) {
-      futures.push_back(renderFrame(v[i].fb, v[i].renderer, v[i].camera, world));
-      HIP_CHECK(hipEventRecord(viewEvents[i], stream));
+    for (int k = 1; k < K; k++) // fork: the lanes start after everything queued so far
+      HIP_CHECK(hipStreamWaitEvent(lanes[k]->stream, batchStart, 0));
+    try {
+      for (size_t i = 0; i < v.size(); i++) {
+        lane = lanes[i % K].get();
+        futures.push_back(renderFrame(v[i].fb, v[i].renderer, v[i].camera, world));
+        HIP_CHECK(hipEventRecord(viewEvents[i], lane->stream));
+      }
+    } catch (...) {
+      lane = lanes[0].get();
+      throw;
+    }
+    lane = lanes[0].get();
+    for (int k = 1; k < K; k++) { // join
+      HIP_CHECK(hipEventRecord(laneEvents[k], lanes[k]->stream));
+      HIP_CHECK(hipStreamWaitEvent(stream, laneEvents[k], 0));
     }
   }
   void renderBatch(const std::vector<BatchView> &v, ManagedObject *world, int mode, float *outMs)
@@ -1861,6 +1950,10 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
 
 void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
 {
+  PathLane &L = *lane;
+  size_t &pathSlots = L.pathSlots, &recSlots = L.recSlots;
+  bool &auxAllocated = L.auxAllocated;
+  hpt::DPaths &paths = L.paths;
   maxRec = std::max(1, maxRec);
   const bool grow = slots > pathSlots || (size_t)maxRec * slots > recSlots || (aux && !auxAllocated);
   if (!grow)
@@ -1868,7 +1961,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   if (capturing)
     throw std::runtime_error("batched views: path buffers would grow while capturing");
   releaseGraph(); // a captured batch refers to the old buffers
-  HIP_CHECK(hipStreamSynchronize(stream));
+  HIP_CHECK(hipStreamSynchronize(L.stream));
   pathSlots = std::max(slots, pathSlots);
   recSlots = std::max((size_t)maxRec * pathSlots, recSlots);
   auxAllocated = auxAllocated || aux;
@@ -1876,15 +1969,15 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   // per-slot arrays: 9 x float4 + hitPrim + ids(uint4) + neeN + volume
   // sampler (uint4) + NEE vertex record (6 x float4) (+ aux 16 floats)
   const size_t perSlot = 9 * 16 + 4 + 16 + 4 + 16 + 6 * 16 + (auxAllocated ? 64 : 0);
-  pathBuf.ensure(S * perSlot);
-  recBuf.ensure(R * 80);
-  queueBuf.ensure((2 * S + 3 * R) * 4);
+  L.pathBuf.ensure(S * perSlot);
+  L.recBuf.ensure(R * 80);
+  L.queueBuf.ensure((2 * S + 3 * R) * 4);
   // recorded transparent hits of up to 4M transparency-queue records (the
   // rest are traversed again); 4 words per hit + a count
   const size_t transCap = std::min<size_t>(R, (size_t)1 << 22);
   const size_t transHitBytes = transCap * 16 * hpt_trans_record_hits();
-  transBuf.ensure(transHitBytes + transCap * 4);
-  char *b = (char *)pathBuf.ptr;
+  L.transBuf.ensure(transHitBytes + transCap * 4);
+  char *b = (char *)L.pathBuf.ptr;
   auto take = [&](size_t bytes) {
     char *p = b;
     b += bytes;
@@ -1909,25 +2002,25 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.vtxThr = (float *)take(S * 16);
   paths.vtxWo = (float *)take(S * 16);
   paths.vtxLd = (uint32_t *)take(S * 16);
-  auxBase = auxAllocated ? (float *)take(S * 64) : nullptr;
-  paths.aux = auxBase;
-  char *r = (char *)recBuf.ptr;
+  L.auxBase = auxAllocated ? (float *)take(S * 64) : nullptr;
+  paths.aux = L.auxBase;
+  char *r = (char *)L.recBuf.ptr;
   paths.recOrg = (float *)r;
   paths.recDir = (float *)(r + R * 16);
   paths.recContrib = (float *)(r + R * 32);
   paths.recResult = (float *)(r + R * 48);
   paths.recVol = (float *)(r + R * 64);
-  uint32_t *q = (uint32_t *)queueBuf.ptr;
+  uint32_t *q = (uint32_t *)L.queueBuf.ptr;
   paths.queue[0] = q;
   paths.queue[1] = q + S;
   paths.shadowQueue = q + 2 * S;
   paths.transQueue = q + 2 * S + R;
   paths.neeSampleQueue = q + 2 * S + 2 * R;
-  paths.transHits = (float *)transBuf.ptr;
-  paths.transCount = (uint32_t *)((char *)transBuf.ptr + transHitBytes);
+  paths.transHits = (float *)L.transBuf.ptr;
+  paths.transCount = (uint32_t *)((char *)L.transBuf.ptr + transHitBytes);
   paths.transCap = (int32_t)transCap;
-  paths.counters = (int32_t *)counters.ptr;
-  paths.overflow = (uint32_t *)overflow.ptr;
+  paths.counters = L.counters.as<int32_t>();
+  paths.overflow = L.overflow.as<uint32_t>();
   paths.stats = (unsigned long long *)stats.ptr;
 }
 
@@ -1935,7 +2028,7 @@ void HipDevice::ensureNrc(size_t slots)
 {
   if (slots > nrcSlots) {
     releaseGraph();
-    HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(hipStreamSynchronize(lane->stream));
     nrcSlots = slots;
     nrcRec.ensure(slots * 80); // 16 floats record + 4 floats contribution before
     nrcQueues.ensure(slots * 8);
@@ -1944,6 +2037,7 @@ void HipDevice::ensureNrc(size_t slots)
     nrcOut.ensure(slots * 12);
     nrcTarget.ensure(slots * 12);
   }
+  hpt::DPaths &paths = lane->paths;
   paths.nrcRec = nrcRec.as<float>();
   paths.nrcBefore = nrcRec.as<float>() + 16 * nrcSlots;
   paths.nrcQueryQueue = nrcQueues.as<uint32_t>();
@@ -1955,6 +2049,8 @@ void HipDevice::ensureNrc(size_t slots)
 // The record counts are read back (one synchronisation per pass).
 void HipDevice::nrcPass(Renderer *r, World *w)
 {
+  const hpt::DPaths &paths = lane->paths;
+  const hipStream_t stream = lane->stream;
   const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);
   int32_t cnt[2] = {0, 0};
   HIP_CHECK(hipMemcpyAsync(cnt, paths.counters + hpt::kCntNrcQuery, sizeof(cnt), hipMemcpyDeviceToHost, stream));
@@ -2035,6 +2131,9 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   if (r->commitCounter == 0 || cam->commitCounter == 0)
     throw std::runtime_error("ospRenderFrame: renderer and camera must be committed");
 
+  // this frame's lane: its stream and path state
+  const hipStream_t stream = lane->stream;
+  hpt::DPaths &paths = lane->paths;
   auto *fut = new Future(this);
   fut->fb.reset(fbo);
   fut->renderer.reset(ro);
@@ -2143,10 +2242,10 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
     const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
     ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
-    paths.aux = aux ? auxBase : nullptr;
+    paths.aux = aux ? lane->auxBase : nullptr;
     if (F.nrcBounce >= 0)
-      ensureNrc(pathSlots);
-    paths.overflow = (uint32_t *)overflow.ptr;
+      ensureNrc(lane->pathSlots);
+    paths.overflow = lane->overflow.as<uint32_t>();
     F.residentThreads = residentTraceBlocks * hpt_trace_block_size();
     const HptScene *S = reinterpret_cast<const HptScene *>(&sceneF);
     const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);
@@ -2232,7 +2331,8 @@ int HipDevice::traceRays(World *w, const float *rays8, int n, float *outTUV, uin
   HIP_CHECK(hipMemcpy(dr.ptr, rays8, (size_t)n * 32, hipMemcpyHostToDevice));
   HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 32, 0u, stream));
   HIP_CHECK(hpt_trace_rays(reinterpret_cast<const HptScene *>(&w->scene), dr.as<float>(), n, dt.as<float>(),
-      di.as<uint32_t>(), (uint32_t *)overflow.ptr, residentTraceBlocks, (unsigned long long *)stats.ptr, stream));
+      di.as<uint32_t>(), lanes[0]->overflow.as<uint32_t>(), residentTraceBlocks, (unsigned long long *)stats.ptr,
+      stream));
   HIP_CHECK(hipStreamSynchronize(stream));
   HIP_CHECK(hipMemcpy(outTUV, dt.ptr, (size_t)n * 12, hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(outIDs, di.ptr, (size_t)n * 12, hipMemcpyDeviceToHost));

@@ -47,16 +47,12 @@ constexpr SobolRevTable makeSobolRevTable()
   return t;
 }
 
-#ifdef __HIP_DEVICE_COMPILE__
-__constant__ const SobolRevTable kSobol = makeSobolRevTable();
-#define HPT_SOBOL kSobol.v
-#else
+#ifndef __HIP_DEVICE_COMPILE__
 inline const uint32_t *hostSobol()
 {
   static const SobolRevTable t = makeSobolRevTable();
   return t.v;
 }
-#define HPT_SOBOL hostSobol()
 #endif
 
 HPT_HD uint32_t tripleHash(uint32_t x)
@@ -105,10 +101,48 @@ HPT_HD uint32_t partitionZOrder8(uint32_t n)
 
 HPT_HD uint32_t interleaveZOrder16(uint32_t x, uint32_t y) { return partitionZOrder8(x) | (partitionZOrder8(y) << 1); }
 
+#ifdef __HIP_DEVICE_COMPILE__
+template <int N>
+struct SobolRevResult
+{
+  uint32_t v[N];
+};
+
+// Fully unrolled over the 32 bits: the matrix rows fold into immediate
+// masks, so there are no table loads and no data-dependent loop.  Out of
+// line and returning by value: inlined, the compiler hoists the 32 N
+// immediates into registers across the caller's loops.
+template <int N>
+__device__ __attribute__((noinline)) SobolRevResult<N> sobolRevSampleDevice(uint32_t revIndex)
+{
+  constexpr SobolRevTable T = makeSobolRevTable();
+  SobolRevResult<N> r;
+#pragma unroll
+  for (int d = 0; d < N; d++)
+    r.v[d] = 0;
+#pragma unroll
+  for (int k = 0; k < 32; k++) {
+    const uint32_t sel = (uint32_t)((int32_t)(revIndex << k) >> 31); // all ones if bit 31 - k is set
+#pragma unroll
+    for (int d = 0; d < N; d++)
+      r.v[d] ^= sel & T.v[4 * k + d];
+  }
+  return r;
+}
+#endif
+
+// out[d] = XOR of the generator-matrix rows k of dimension d for which bit
+// 31 - k of revIndex is set
 template <int N>
 HPT_HD void sobolRevSample(uint32_t revIndex, uint32_t *out)
 {
-  const uint32_t *M = HPT_SOBOL;
+#ifdef __HIP_DEVICE_COMPILE__
+  const SobolRevResult<N> r = sobolRevSampleDevice<N>(revIndex);
+#pragma unroll
+  for (int d = 0; d < N; d++)
+    out[d] = r.v[d];
+#else
+  const uint32_t *M = hostSobol();
   for (int d = 0; d < N; d++)
     out[d] = 0;
   for (uint32_t i = 0; revIndex; revIndex <<= 1, i += 4) {
@@ -116,6 +150,7 @@ HPT_HD void sobolRevSample(uint32_t revIndex, uint32_t *out)
       for (int d = 0; d < N; d++)
         out[d] ^= M[i + d];
   }
+#endif
 }
 
 constexpr uint32_t SCRAMBLE0 = 0x9374b0d3u;
@@ -192,10 +227,24 @@ struct Sample1D
   float frac, pdf, prob;
 };
 
+// the first index whose CDF value exceeds s (upper bound)
 HPT_HD Sample1D distribution1DSample(int size, const float *cdf, int start, float s)
 {
   int first = start;
   int len = size;
+#ifdef __HIP_DEVICE_COMPILE__
+  if (size <= 8) {
+    // short CDFs (the light lists of most scenes): the CDF does not
+    // decrease, so the upper bound is the count of entries <= s; the
+    // entries load independently instead of one per search step
+    int below = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      below += (k < size && !(s < cdf[start + k])) ? 1 : 0;
+    first = start + below;
+    len = 0;
+  }
+#endif
   while (len > 0) {
     const int half = len >> 1;
     const int mid = first + half;

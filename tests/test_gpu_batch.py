@@ -1,8 +1,8 @@
 """Batched views (ospHipRenderBatch, ospHipTuneBatch): a batch of views of
-one world rendered in one enqueue, eagerly or as one captured HIP graph,
-gives each view exactly the image ospRenderFrame gives it; graph relaunches
-replay the same frames; the autotuner's settings leave the images
-unchanged."""
+one world rendered in one enqueue, eagerly or as one captured HIP graph, on
+concurrent lanes or on one stream, gives each view exactly the image
+ospRenderFrame gives it; graph relaunches replay the same frames; the
+autotuner's settings leave the images unchanged."""
 import numpy as np
 import pytest
 
@@ -26,6 +26,13 @@ def _views(s, specs):
 SPECS = [(40, 24, 1, (0.0, 4.0, -20.0)), (64, 64, 2, (15.0, 6.0, -10.0)), (24, 40, 4, (-12.0, 10.0, 12.0))]
 
 
+def _set_batch_streams(n):
+    L = K.lib()
+    dev = L.ospGetCurrentDevice()
+    A.set_device_param(dev, "batchStreams", n)
+    L.ospDeviceCommit(dev)
+
+
 def _images(fbs):
     return [A.map_color(fb, w, h, K.OSP_FB_RGBA32F) for fb, (w, h, _, _) in zip(fbs, SPECS)]
 
@@ -38,14 +45,26 @@ def test_batch_equals_single_frames(hip_device):
     for fb, r, c, (w, h, _, _) in zip(fbs, rens, cams, SPECS):
         A.render(fb, r, c, s.world)
         single.append(A.map_color(fb, w, h, K.OSP_FB_RGBA32F))
+    # the views on concurrent lanes (the default), then on one stream
     ms = A.render_batch(fbs, rens, cams, s.world, graph=False)
-    assert all(a >= 0 for a in ms) and ms == sorted(ms)  # views finish in order on one stream
+    assert all(a > 0 for a in ms)
     for a, b in zip(_images(fbs), single):
         assert np.array_equal(a, b)
     for _ in range(3):  # capture, then two relaunches
         A.render_batch(fbs, rens, cams, s.world, graph=True)
         for a, b in zip(_images(fbs), single):
             assert np.array_equal(a, b)
+    _set_batch_streams(1)
+    try:
+        ms = A.render_batch(fbs, rens, cams, s.world, graph=False)
+        assert all(a > 0 for a in ms) and ms == sorted(ms)  # views finish in order on one stream
+        for a, b in zip(_images(fbs), single):
+            assert np.array_equal(a, b)
+        A.render_batch(fbs, rens, cams, s.world, graph=True)
+        for a, b in zip(_images(fbs), single):
+            assert np.array_equal(a, b)
+    finally:
+        _set_batch_streams(4)
     # a recommitted camera invalidates the captured graph
     A.set_param(cams[1].h, "position", (16.0, 6.0, -10.0))
     A.commit(cams[1])
