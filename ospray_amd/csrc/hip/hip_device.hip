@@ -946,6 +946,9 @@ class HipDevice : public Device
   std::vector<std::unique_ptr<PathLane>> lanes;
   PathLane *lane = nullptr;
   int batchStreams = 4; // lanes a batch of views spreads over
+  int frameLanes = 1; // lanes the passes of one frame alternate over (1: one stream)
+  bool inBatch = false; // enqueuing the views of a batch
+  hipEvent_t passEvents[2] = {nullptr, nullptr};
   DevBuf stats;
   // radiance cache per-pass buffers: vertex records, queues, raw inputs,
   // bf16 features, predictions, training targets
@@ -970,6 +973,9 @@ class HipDevice : public Device
       (void)hipEventDestroy(batchStart);
     for (auto e : laneEvents)
       (void)hipEventDestroy(e);
+    for (auto e : passEvents)
+      if (e)
+        (void)hipEventDestroy(e);
     lanes.clear();
     if (stream)
       (void)hipStreamDestroy(stream);
@@ -983,6 +989,7 @@ class HipDevice : public Device
     traceBlocksPerCU = std::max(1, std::min(64, getInt("traceBlocksPerCU", traceBlocksPerCU)));
     shadeBlocksPerCU = std::max(1, std::min(64, getInt("shadeBlocksPerCU", shadeBlocksPerCU)));
     batchStreams = std::max(1, std::min(8, getInt("batchStreams", batchStreams)));
+    frameLanes = std::max(1, std::min(8, getInt("frameLanes", frameLanes)));
     if (stream) {
       HIP_CHECK(hipStreamSynchronize(stream));
       sizeGrids();
@@ -1332,6 +1339,7 @@ class HipDevice : public Device
     HIP_CHECK(hipEventRecord(batchStart, stream));
     for (int k = 1; k < K; k++) // fork: the lanes start after everything queued so far
       HIP_CHECK(hipStreamWaitEvent(lanes[k]->stream, batchStart, 0));
+    inBatch = true;
     try {
       for (size_t i = 0; i < v.size(); i++) {
         lane = lanes[i % K].get();
@@ -1340,9 +1348,11 @@ class HipDevice : public Device
       }
     } catch (...) {
       lane = lanes[0].get();
+      inBatch = false;
       throw;
     }
     lane = lanes[0].get();
+    inBatch = false;
     for (int k = 1; k < K; k++) { // join
       HIP_CHECK(hipEventRecord(laneEvents[k], lanes[k]->stream));
       HIP_CHECK(hipStreamWaitEvent(stream, laneEvents[k], 0));
@@ -2240,18 +2250,49 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     const size_t perSlotBytes = 9 * 16 + 56 + 96 + (aux ? 64 : 0) + (size_t)F.maxRec * 92 + 8
         + (F.nrcBounce >= 0 ? 80 + 8 + 48 + 2 * HPT_NRC_INPUTS + 24 : 0);
     const size_t budgetSlots = std::max<size_t>(F.numPixPadded, pathBudgetBytes / perSlotBytes);
-    const int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
-    ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
-    paths.aux = aux ? lane->auxBase : nullptr;
+    int sppPerPass = (int)std::max<size_t>(1, std::min<size_t>((size_t)F.spp, budgetSlots / F.numPixPadded));
+    // Pass pipelining: a frame of several samples per pixel (outside a
+    // batch, without the radiance cache) runs its passes alternately on
+    // frameLanes lanes, so that one pass's sparse late bounces overlap the
+    // next pass's dense early ones.  Each pass's accumulation waits for the
+    // previous pass's: the samples are summed in index order as in one pass.
+    int nl = 1;
+    if (!inBatch && F.nrcBounce < 0 && frameLanes > 1 && F.spp > 1) {
+      nl = std::min(frameLanes, F.spp);
+      sppPerPass = std::min(sppPerPass, (F.spp + nl - 1) / nl);
+      nl = std::min(nl, (F.spp + sppPerPass - 1) / sppPerPass);
+    }
+    PathLane *const frameLane = lane;
+    for (int k = nl - 1; k >= 0; k--) {
+      lane = k == 0 ? frameLane : &ensureLane(k);
+      ensurePaths((size_t)sppPerPass * F.numPixPadded, F.maxRec, aux);
+      lane->paths.aux = aux ? lane->auxBase : nullptr;
+      lane->paths.overflow = lane->overflow.as<uint32_t>();
+    }
     if (F.nrcBounce >= 0)
       ensureNrc(lane->pathSlots);
-    paths.overflow = lane->overflow.as<uint32_t>();
+    for (int k = 0; k < 2; k++)
+      if (!passEvents[k])
+        HIP_CHECK(hipEventCreateWithFlags(&passEvents[k], hipEventDisableTiming));
+    if ((int)laneEvents.size() < nl)
+      for (int k = (int)laneEvents.size(); k < nl; k++) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        laneEvents.push_back(e);
+      }
     F.residentThreads = residentTraceBlocks * hpt_trace_block_size();
     const HptScene *S = reinterpret_cast<const HptScene *>(&sceneF);
-    const HptPaths *P = reinterpret_cast<const HptPaths *>(&paths);
     if (travStats)
       HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 32, 0u, stream));
-    for (int s0 = 0; s0 < F.spp; s0 += sppPerPass) {
+    if (nl > 1) { // fork: the other lanes start after the frame's earlier work
+      HIP_CHECK(hipEventRecord(laneEvents[0], stream));
+      for (int k = 1; k < nl; k++)
+        HIP_CHECK(hipStreamWaitEvent(lanes[k]->stream, laneEvents[0], 0));
+    }
+    for (int s0 = 0, pass = 0; s0 < F.spp; s0 += sppPerPass, pass++) {
+      lane = pass % nl == 0 ? frameLane : lanes[pass % nl].get();
+      const hipStream_t stream = lane->stream;
+      const HptPaths *P = reinterpret_cast<const HptPaths *>(&lane->paths);
       F.passS0 = s0;
       F.passCount = std::min(sppPerPass, F.spp - s0);
       if (F.nrcBounce >= 0) { // every stride-th slot of the pass trains the cache
@@ -2288,7 +2329,16 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         const double survival = std::min(1.0, std::max(1e-3, (double)(r->nrcRecords - before) / slots));
         r->nrcSurvival = 0.5 * r->nrcSurvival + 0.5 * survival;
       }
+      if (nl > 1 && pass > 0) // accumulate after the previous pass
+        HIP_CHECK(hipStreamWaitEvent(stream, passEvents[(pass - 1) & 1], 0));
       launch("finish_pass", [&]() { return hpt_finish_pass(Fp, P, stream); });
+      if (nl > 1)
+        HIP_CHECK(hipEventRecord(passEvents[pass & 1], stream));
+    }
+    lane = frameLane;
+    for (int k = 1; k < nl; k++) { // join
+      HIP_CHECK(hipEventRecord(laneEvents[k], lanes[k]->stream));
+      HIP_CHECK(hipStreamWaitEvent(stream, laneEvents[k], 0));
     }
     if (!capturing)
       HIP_CHECK(hipEventRecord(fut->rendered, stream));

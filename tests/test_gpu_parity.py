@@ -226,6 +226,39 @@ def test_transparent_shadows_bit_exact(hip_device, max_path_length):
     assert col[..., :3].mean() > 0.01
 
 
+def _set_device_params(**kv):
+    L = K.lib()
+    dev = L.ospGetCurrentDevice()
+    for k, v in kv.items():
+        A.set_device_param(dev, k, v)
+    L.ospDeviceCommit(dev)
+
+
+@pytest.mark.parametrize("lanes,budget_mb", [(2, 16384), (3, 16384), (2, 1)])
+def test_pass_pipelining_bit_exact(hip_device, lanes, budget_mb):
+    # a frame's passes alternate over lanes (the accumulation stays in
+    # sample order): every channel equals the one-lane frames and the
+    # oracle; a 1 MB path budget gives one sample per pass, more passes
+    # than lanes
+    sc = scenes.c0(40, 36, spp=5, max_path_length=4)
+    images = {}
+    try:
+        for fl in (1, lanes):
+            _set_device_params(frameLanes=fl, pathBudgetMB=budget_mb)
+            s = scenes.to_ospray(sc)
+            fb = A.framebuffer(sc.width, sc.height, K.OSP_FB_RGBA32F, ALL_CH)
+            for _ in range(2):
+                A.render(fb, s.renderer, s.camera, s.world)
+            images[fl] = [A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)] + \
+                [A.map_channel(fb, ch, sc.width, sc.height) for ch in AOV] + [K.lib().ospGetVariance(fb.h)]
+    finally:
+        _set_device_params(frameLanes=1, pathBudgetMB=16384)
+    for a, b in zip(images[1], images[lanes]):
+        assert np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+    ofb, _, _ = orc.render_scene(sc, frames=2, fmt=K.OSP_FB_RGBA32F, channels=ALL_CH)
+    assert np.array_equal(images[lanes][0], ofb.channel(K.OSP_FB_COLOR))
+
+
 def _volume(anisotropy=0.3, user_id=None, seed=3):
     """A 24x22x20 random grid (box smoothed) with an 8-point transfer
     function: non-cubic, so a swapped axis shows."""
