@@ -1032,7 +1032,7 @@ class HipDevice : public Device
       }
       L->counters.ensure(64);
       if (residentTraceBlocks > 0)
-        L->overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
+        ensureScratch(*L);
       L->paths.counters = L->counters.as<int32_t>();
       L->paths.overflow = L->overflow.as<uint32_t>();
       L->paths.stats = (unsigned long long *)stats.ptr;
@@ -1051,9 +1051,22 @@ class HipDevice : public Device
     shadeBlocks = std::max(1, numCUs) * shadeBlocksPerCU;
     for (auto &L : lanes) {
       HIP_CHECK(hipStreamSynchronize(L->stream));
-      L->overflow.ensure((size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4);
+      ensureScratch(*L);
       L->paths.overflow = L->overflow.as<uint32_t>();
     }
+  }
+
+  // the traversal scratch rows of a lane, cleared when (re)allocated: the
+  // occlusion pass keeps each row's recorded-hit count at 0 between rays
+  void ensureScratch(PathLane &L)
+  {
+    const size_t bytes = (size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4;
+    if (L.overflow.ptr && L.overflow.bytes >= bytes)
+      return;
+    if (capturing)
+      throw std::runtime_error("batched views: traversal scratch would grow while capturing");
+    L.overflow.ensure(bytes);
+    HIP_CHECK(hipMemset(L.overflow.ptr, 0, bytes));
   }
 
   void commitObject(ManagedObject *obj) override

@@ -741,15 +741,13 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
   uint32_t rays = 0;
   for (int base = nextBatch(&P.counters[kCntFetchShadow]); base < n; base = nextBatch(&P.counters[kCntFetchShadow])) {
     const int i = base + laneId();
-    bool toTrans = false;
+    bool toTrans = false, sawT = false;
     uint32_t r = 0;
     if (i < n) {
       r = P.shadowQueue[i];
       const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
       const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
       Hit h;
-      bool sawT;
-      *recordedCount(ovf) = 0;
       const bool occluded = traceRay<MODE_OCCLUSION, STATS, true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 0.f,
           o4.w, h, lds, ovf, sawT, st);
       rays++;
@@ -782,6 +780,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
         P.transCount[slot] = nrec <= (uint32_t)kTransRecHits ? nrec : 0xffffffffu;
       }
     }
+    // the recorded-hit count is 0 when a traversal starts: the rows are
+    // cleared when allocated, and only rays that recorded hits reset it
+    if (sawT)
+      *recordedCount(ovf) = 0;
   }
   if (STATS) {
     waveAdd(&P.stats[3], st.nodes);

@@ -232,8 +232,9 @@ __device__ __forceinline__ float slabNear(float lox, float loy, float loz, float
 // thread's scratch row after the stack overflow (recordedHits): one float4
 // (t, instance, index into S.prims, root: 1 = a sphere's exit) per hit, up
 // to kTransRecHits, and in the word after them the count of all of them,
-// also those beyond the capacity (the caller sets it to 0; it lives in
-// memory rather than a register since transparent hits are rare).
+// also those beyond the capacity (0 when the traversal starts: the caller
+// resets it after each ray that recorded hits; it lives in memory rather
+// than a register since transparent hits are rare).
 // hitFromRecord rebuilds the closest-hit record of one of them.
 
 __device__ __forceinline__ float4 *recordedHits(uint32_t *ovf)
