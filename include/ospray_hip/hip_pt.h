@@ -101,7 +101,9 @@ HPT_API hipError_t hpt_trace_rays(const HptScene *S, const float *rays8, int n, 
 HPT_API int hpt_trace_block_size(void);
 /* 32-bit words of the scratch row each resident traversal thread needs in
  * `overflow` (HptPaths and hpt_trace_rays): the traversal-stack overflow,
- * then the transparent hits an occlusion traversal records */
+ * then the transparent hits an occlusion traversal records and their
+ * count.  The rows must be zero when first used (hpt_shadow keeps each
+ * row's count at zero between rays). */
 HPT_API int hpt_overflow_depth(void);
 /* traversal stack entries per ray (LDS + overflow) */
 HPT_API int hpt_stack_capacity(void);
