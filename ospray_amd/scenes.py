@@ -491,7 +491,9 @@ def to_ospray(scene: Scene, renderer_params: Optional[dict] = None) -> OspScene:
     for l in scene.lights:
         p = {k: v for k, v in l.items() if k != "type"}
         light_h.append(A.new("light", l["type"], p))
-    world_params = {"instance": A.object_data(inst_h, K.OSP_INSTANCE)}
+    world_params = {}
+    if inst_h:
+        world_params["instance"] = A.object_data(inst_h, K.OSP_INSTANCE)
     if light_h:
         world_params["light"] = A.object_data(light_h, K.OSP_LIGHT)
     world = A.new("world", params=world_params)

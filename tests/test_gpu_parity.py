@@ -226,6 +226,35 @@ def test_transparent_shadows_bit_exact(hip_device, max_path_length):
     assert col[..., :3].mean() > 0.01
 
 
+def _edge_scene(name):
+    import dataclasses
+    if name == "empty_world":  # lights only: the background and the ambient light
+        sc = scenes.c0(16, 12, spp=2, max_path_length=3)
+        return dataclasses.replace(sc, geometries=[], models=[], groups=[], instances=[], materials=[])
+    if name == "no_lights":
+        return dataclasses.replace(scenes.c0(16, 12, spp=2, max_path_length=3), lights=[])
+    if name == "1x1":
+        return scenes.c0(1, 1, spp=3, max_path_length=4)
+    if name == "3x2":
+        return scenes.c0(3, 2, spp=2, max_path_length=4)
+    if name == "depth0":
+        return scenes.c0(16, 12, spp=2, max_path_length=0)
+    return scenes.c0(16, 12, spp=2, max_path_length=1)
+
+
+@pytest.mark.parametrize("name", ["empty_world", "no_lights", "1x1", "3x2", "depth0", "depth1"])
+def test_edge_cases_bit_exact(hip_device, name):
+    # an empty world, a world without lights, framebuffers of one pixel and
+    # of less than one 8x8 task, and paths of length 0 and 1: every channel
+    # and the variance equal the oracle's
+    sc = _edge_scene(name)
+    s, fb, ofb, _, oren = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    assert K.lib().ospGetVariance(fb.h) == pytest.approx(ofb.variance(oren), rel=1e-6, nan_ok=True)
+
+
 def _set_device_params(**kv):
     L = K.lib()
     dev = L.ospGetCurrentDevice()
