@@ -54,13 +54,15 @@ __device__ __forceinline__ void waveAdd(unsigned long long *counter, uint32_t v)
     atomicAdd(counter, s);
 }
 
-// wave-aggregated append of up to one entry per lane to an extension
-// queue and of `nee` entries per lane to the NEE sample queue, with one
-// 64-bit atomic per wave on the counter pair (ext, nee); every lane of the
-// wave calls it
-__device__ __forceinline__ void enqueuePair(int32_t *pair, bool ext, uint32_t nee, int &extSlot, int &neeSlot)
+// Block-aggregated append of up to one entry per thread to an extension
+// queue and of `nee` entries per thread to the NEE sample queue, with one
+// 64-bit atomic per block and call on the counter pair (ext, nee): atomics
+// on one counter serialise in the L2, so one per wave costs milliseconds
+// per frame.  Every thread of the block calls it, the same number of times.
+__device__ __forceinline__ void blockEnqueuePair(int32_t *pair, bool ext, uint32_t nee, int &extSlot, int &neeSlot)
 {
-  const int lane = laneId();
+  __shared__ unsigned long long waveBase[16];
+  const int lane = laneId(), wave = (int)(threadIdx.x >> 6), nw = (int)((blockDim.x + 63) >> 6);
   const unsigned long long mask = __ballot(ext);
   uint32_t x = nee; // inclusive scan of nee over the wave
   for (int o = 1; o < 64; o <<= 1) {
@@ -68,15 +70,26 @@ __device__ __forceinline__ void enqueuePair(int32_t *pair, bool ext, uint32_t ne
     if (lane >= o)
       x += y;
   }
-  const uint32_t totalNee = __shfl(x, 63);
-  const uint32_t totalExt = (uint32_t)__popcll(mask);
-  unsigned long long base = 0;
-  if (lane == 0 && (totalExt | totalNee))
-    base = atomicAdd(reinterpret_cast<unsigned long long *>(pair),
-        (unsigned long long)totalExt | ((unsigned long long)totalNee << 32));
-  base = __shfl(base, 0);
-  extSlot = (int)(uint32_t)base + __popcll(mask & ((1ull << lane) - 1ull));
-  neeSlot = (int)(uint32_t)(base >> 32) + (int)(x - nee);
+  const uint32_t totalNee = __shfl(x, 63); // all lanes take part in the shuffle
+  if (lane == 0)
+    waveBase[wave] = (unsigned long long)__popcll(mask) | ((unsigned long long)totalNee << 32);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long total = 0; // (ext, nee) counts packed: neither half overflows
+    for (int w = 0; w < nw; w++)
+      total += waveBase[w];
+    unsigned long long run = total ? atomicAdd(reinterpret_cast<unsigned long long *>(pair), total) : 0ull;
+    for (int w = 0; w < nw; w++) {
+      const unsigned long long t = waveBase[w];
+      waveBase[w] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  const unsigned long long b = waveBase[wave];
+  extSlot = (int)(uint32_t)b + __popcll(mask & ((1ull << lane) - 1ull));
+  neeSlot = (int)(uint32_t)(b >> 32) + (int)(x - nee);
+  __syncthreads(); // waveBase is reused by the next call
 }
 
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
@@ -130,16 +143,16 @@ __device__ __forceinline__ void storeRng(const DPaths &P, uint32_t p, const Rand
 }
 
 // pixel filter (PixelFilterDispatch.ispc:7-19, LUTPixelFilter.ispc:9-15,
-// Distribution2D.ispc:9-25)
-__device__ vec2f pixelFilterSample(const DFrame &F, vec2f s)
+// Distribution2D.ispc:9-25); cdfX / cdfY: the LUT's CDFs (n x n, n)
+__device__ __forceinline__ vec2f pixelFilterSample(const DFrame &F, vec2f s, const float *cdfX, const float *cdfY)
 {
   if (F.filterType == 0)
     return v2(0.f, 0.f);
   if (F.filterType == 1)
     return v2((s.x + -0.5f) * F.filterWidth, (s.y + -0.5f) * F.filterWidth);
   const int n = F.lutSize;
-  const Sample1D sy = distribution1DSample(n, F.lutCdfY, 0, s.y);
-  const Sample1D sx = distribution1DSample(n, F.lutCdfX, sy.idx * n, s.x);
+  const Sample1D sy = distribution1DSample(n, cdfY, 0, s.y);
+  const Sample1D sx = distribution1DSample(n, cdfX, sy.idx * n, s.x);
   const float rcpN = 1.f / (float)n;
   const vec2f uv = v2(((float)sx.idx + sx.frac) * rcpN, ((float)sy.idx + sy.frac) * rcpN);
   return v2((uv.x + -0.5f) * F.filterWidth, (uv.y + -0.5f) * F.filterWidth);
@@ -172,8 +185,22 @@ __global__ __launch_bounds__(256) void k_copy(uint4 *dst, const uint4 *src, size
 
 // PathTracer_renderPixel up to the first samplePath iteration
 // (PathTracer.ispc:75-170) for samples [passS0, passS0+passCount)
+constexpr int kLutLds = 1024; // floats of pixel-filter LUT staged in LDS
+
 __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
 {
+  // the filter LUT's CDFs in LDS: every camera sample runs two binary
+  // searches over them
+  __shared__ float lut[kLutLds];
+  const float *cdfX = F.lutCdfX, *cdfY = F.lutCdfY;
+  if (F.filterType == 2 && F.lutSize * (F.lutSize + 1) <= kLutLds) { // uniform over the grid
+    const int nn = F.lutSize * F.lutSize;
+    for (int k = threadIdx.x; k < nn + F.lutSize; k += blockDim.x)
+      lut[k] = k < nn ? F.lutCdfX[k] : F.lutCdfY[k - nn];
+    __syncthreads();
+    cdfX = lut;
+    cdfY = lut + nn;
+  }
   const int total = F.passCount * F.numPixPadded;
   for (int i = globalId(); i < total; i += gridStride()) {
     const int sLocal = i / F.numPixPadded;
@@ -203,7 +230,7 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
       lds.idxMaskLight = F.idxMaskLight;
       const vec2f center = v2(((float)ix + 0.5f) * (1.f / (float)F.width), ((float)iy + 0.5f) * (1.f / (float)F.height));
       const vec2f pixelSample = ldGetNext5Samples(lds);
-      const vec2f pf = pixelFilterSample(F, pixelSample);
+      const vec2f pf = pixelFilterSample(F, pixelSample, cdfX, cdfY);
       const vec2f screen = v2(center.x + pf.x * (1.f / (float)F.width), center.y + pf.y * (1.f / (float)F.height));
       // PerspectiveCamera_initRay (PerspectiveCamera.ispc:16-64), default image region
       const float sx = (1.f - screen.x) * 0.f + screen.x * 1.f;
@@ -230,10 +257,24 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
           st4(a + k, 0.f, 0.f, 0.f, 0.f);
       }
     }
-    const int slot = enqueue(&P.counters[0], valid); // extension queue 0
-    if (valid)
-      P.queue[0][slot] = (uint32_t)i;
+    if (F.adaptiveThreshold == 0.f) {
+      // every pixel of the image is queued: the compacted position follows
+      // from the slot layout (whole 8-row tile rows above, full-width tiles
+      // to the left, then row-major inside the tile)
+      if (valid) {
+        const int tx = ix >> 3, ty = iy >> 3, px = ix & 7, py = iy & 7;
+        const int h = min(8, F.height - 8 * ty), wt = min(8, F.width - 8 * tx);
+        const int pos = sLocal * F.width * F.height + 8 * ty * F.width + 8 * tx * h + py * wt + px;
+        P.queue[0][pos] = (uint32_t)i;
+      }
+    } else {
+      const int slot = enqueue(&P.counters[0], valid); // extension queue 0
+      if (valid)
+        P.queue[0][slot] = (uint32_t)i;
+    }
   }
+  if (F.adaptiveThreshold == 0.f && globalId() == 0)
+    P.counters[0] = F.passCount * F.width * F.height;
 }
 
 // Work distribution of the traversal kernels: each wave takes batches of
@@ -327,8 +368,9 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
   const int n = P.counters[2 * qin];
   const int qout = qin ^ 1;
   VolCount vcnt; // transmittance work of the virtual-light MIS weights
-  for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
-    const int i = w0 + laneId();
+  // block-uniform iterations: the queue appends are aggregated per block
+  for (int b0 = (int)(blockIdx.x * blockDim.x); b0 < n; b0 += (int)(gridDim.x * blockDim.x)) {
+    const int i = b0 + (int)threadIdx.x;
     bool cont = false, nrcQuery = false, nrcTrain = false;
     uint32_t nNee = 0;
     uint32_t p = 0;
@@ -601,7 +643,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int
     // the continuing path to extension queue qout; the light-sample slots of
     // this vertex (records p*maxRec + [0, nNee)) to the NEE sample queue
     int so, first;
-    enqueuePair(&P.counters[2 * qout], cont, nNee, so, first);
+    blockEnqueuePair(&P.counters[2 * qout], cont, nNee, so, first);
     if (cont)
       P.queue[qout][so] = p;
     for (uint32_t k = 0; k < nNee; k++)
@@ -661,11 +703,18 @@ __global__ __launch_bounds__(256) void k_nrc_targets(DPaths P, int n, float *tar
 // One NEE light sample per thread (NextEventEstimation.ispc:54-146): pick a
 // light, sample it, evaluate the BSDF and write the shadow-ray record;
 // records that can contribute go to the shadow queue.
+constexpr int kStage = 2048;
+
 __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int qin)
 {
+  __shared__ uint32_t stage[kStage];
+  __shared__ uint32_t stageCount, stageBase;
+  if (threadIdx.x == 0)
+    stageCount = 0;
+  __syncthreads();
   const int n = P.counters[2 * (qin ^ 1) + 1];
-  for (int w0 = globalId() - laneId(); w0 < n; w0 += gridStride()) {
-    const int i = w0 + laneId();
+  for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    const int i = b0 + (int)threadIdx.x;
     bool emit = false;
     uint32_t r = 0;
     if (i < n) {
@@ -721,9 +770,34 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
           st4(P.recVol + 4 * (size_t)r, ls.pdf, fe.pdf, 0.f, 0.f);
       }
     }
-    const int slot = enqueue(&P.counters[kCntShadow], emit);
-    if (emit)
-      P.shadowQueue[slot] = r;
+    {
+      const unsigned long long mask = __ballot(emit);
+      uint32_t wpos = 0;
+      if (laneId() == 0 && mask)
+        wpos = atomicAdd(&stageCount, (uint32_t)__popcll(mask));
+      wpos = __shfl(wpos, 0);
+      if (emit)
+        stage[wpos + __popcll(mask & ((1ull << laneId()) - 1ull))] = r;
+    }
+    __syncthreads();
+    if (stageCount > (uint32_t)(kStage - blockDim.x)) { // block-uniform
+      if (threadIdx.x == 0)
+        stageBase = (uint32_t)atomicAdd(&P.counters[kCntShadow], (int)stageCount);
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k < stageCount; k += blockDim.x)
+        P.shadowQueue[stageBase + k] = stage[k];
+      __syncthreads();
+      if (threadIdx.x == 0)
+        stageCount = 0;
+      __syncthreads();
+    }
+  }
+  if (stageCount > 0) {
+    if (threadIdx.x == 0)
+      stageBase = (uint32_t)atomicAdd(&P.counters[kCntShadow], (int)stageCount);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < stageCount; k += blockDim.x)
+      P.shadowQueue[stageBase + k] = stage[k];
   }
 }
 
