@@ -1960,11 +1960,12 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
     bounds[k] = T.bounds.lo[k];
     bounds[3 + k] = T.bounds.hi[k];
   }
-  // the traversal stack holds the TLAS entries, the instance marker and the
-  // BLAS entries of one root-to-leaf path
-  if (T.stackNeed + 1 + blasStackNeed > hpt_stack_capacity())
+  // the traversal stack holds the TLAS entries of one root-to-leaf path,
+  // the next TLAS node (a held instance leaf is entered after the lane went
+  // on in the TLAS), the instance marker and the BLAS entries
+  if (T.stackNeed + 2 + blasStackNeed > hpt_stack_capacity())
     throw std::runtime_error("BVH too deep for the traversal stack");
-  stackNeed = T.stackNeed + 1 + blasStackNeed;
+  stackNeed = T.stackNeed + 2 + blasStackNeed;
   maxDepth = std::max(maxDepth, T.maxDepth);
   built = true;
 }

@@ -4,7 +4,10 @@
 // modules/cpu/common/World.ih:29-52, 247-286).  Here:
 //   * two-level 4-wide BVH (TLAS over instances, one BLAS per group) in
 //     128-byte nodes holding four child boxes; one lane per ray, children
-//     visited nearest first;
+//     visited nearest first, speculative while-while order (Aila and Laine,
+//     "Understanding the efficiency of ray traversal on GPUs", 2009): a
+//     lane holds a leaf it reaches and goes on with inner nodes until every
+//     lane of the wave holds one;
 //   * the traversal stack lives in LDS (LDS_DEPTH entries per lane, laid out
 //     [depth][lane] so lanes hit distinct banks) and spills to a per-thread
 //     global overflow row beyond that;
@@ -294,56 +297,71 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
       return (int)lds[sp * kTraceBlock + lane];
     return (int)ovf[sp - kLdsDepth];
   };
+  // speculative while-while traversal: a lane that reaches a leaf keeps it
+  // aside (post) and goes on with inner nodes until every lane of the wave
+  // holds a leaf; then the wave processes the leaves together.  A held BLAS
+  // leaf stops the lane at the instance's exit marker (it needs the object
+  // ray); a held instance leaf is entered from the world level, with the
+  // next world node pushed below the marker.
+  constexpr int kDone = 0x7ffffffe;
+  int post = 0; // a held leaf (leaves are negative), 0: none
   for (;;) {
-    if (cur >= 0 && cur != kMarker) {
-      const Node4 nd = loadNode(S.nodes, cur);
-      if (STATS)
-        st.nodes++;
-      const float tb = best.t;
-      float t0c = slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tb);
-      float t1c = slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tb);
-      float t2c = slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tb);
-      float t3c = slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, t0, tb);
-      int c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
-      // nearest first: sorting network on the four entry distances
-      cswap(t0c, c0, t1c, c1);
-      cswap(t2c, c2, t3c, c3);
-      cswap(t0c, c0, t2c, c2);
-      cswap(t1c, c1, t3c, c3);
-      cswap(t1c, c1, t2c, c2);
-      if (t0c == kInf) {
-        if (sp == 0)
-          break;
-        cur = pop();
-        continue;
+    for (;;) {
+      if (cur >= 0 && cur < kDone) {
+        const Node4 nd = loadNode(S.nodes, cur);
+        if (STATS)
+          st.nodes++;
+        const float tb = best.t;
+        float t0c = slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tb);
+        float t1c = slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tb);
+        float t2c = slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tb);
+        float t3c = slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, t0, tb);
+        int c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
+        cswap(t0c, c0, t1c, c1);
+        cswap(t2c, c2, t3c, c3);
+        cswap(t0c, c0, t2c, c2);
+        cswap(t1c, c1, t3c, c3);
+        cswap(t1c, c1, t2c, c2);
+        if (t0c == kInf) {
+          cur = sp == 0 ? kDone : pop();
+        } else {
+          const int npush = (t1c != kInf) + (t2c != kInf) + (t3c != kInf);
+          if (sp + npush > kLdsDepth + kOverflowDepth) // cannot happen: the world commit checks the stack need
+            return best.hit;
+          if (t3c != kInf)
+            push(c3);
+          if (t2c != kInf)
+            push(c2);
+          if (t1c != kInf)
+            push(c1);
+          cur = c0;
+        }
+      } else if (cur == kMarker && post == 0) {
+        o = worg;
+        d = wdir;
+        inv = winv;
+        inst = -1;
+        cur = sp == 0 ? kDone : pop();
       }
-      const int npush = (t1c != kInf) + (t2c != kInf) + (t3c != kInf);
-      if (sp + npush > kLdsDepth + kOverflowDepth) // cannot happen: the world commit checks the stack need
-        return best.hit;
-      if (t3c != kInf)
-        push(c3);
-      if (t2c != kInf)
-        push(c2);
-      if (t1c != kInf)
-        push(c1);
-      cur = c0;
-      continue;
-    }
-    if (cur == kMarker) {
-      o = worg;
-      d = wdir;
-      inv = winv;
-      inst = -1;
-      if (sp == 0)
+      if (cur < 0 && post == 0) { // hold the leaf, go on
+        post = cur;
+        cur = sp == 0 ? kDone : pop();
+      }
+      const bool canStep = (cur >= 0 && cur < kDone) || (cur == kMarker && post == 0);
+      if (__all(!canStep || post != 0))
         break;
-      cur = pop();
-      continue;
     }
-    const uint32_t enc = ~(uint32_t)cur;
+    if (post == 0) { // nothing held: the lane has no work left
+      if (cur == kDone)
+        break;
+      continue; // (cannot happen: a lane without a held leaf can step)
+    }
+    const uint32_t enc = ~(uint32_t)post;
+    post = 0;
     const uint32_t first = enc >> 3;
     const uint32_t count = (enc & 7u) + 1u;
     if (inst < 0) {
-      // TLAS leaf: one instance; continue in its BLAS with the object ray
+      // world leaf: one instance; continue in its BLAS with the object ray
       inst = (int)first;
       const DInstance &I = S.instances[inst];
       const linear3f rl = {{I.rcp[0], I.rcp[1], I.rcp[2]}, {I.rcp[3], I.rcp[4], I.rcp[5]}, {I.rcp[6], I.rcp[7], I.rcp[8]}};
@@ -352,8 +370,10 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
       inv = v3(safeInv(d.x), safeInv(d.y), safeInv(d.z));
       pc = triPrecompute(d);
       modelOffset = I.modelOffset;
-      if (sp >= kLdsDepth + kOverflowDepth)
+      if (sp + 2 > kLdsDepth + kOverflowDepth)
         return best.hit;
+      if (cur != kDone)
+        push(cur);
       push(kMarker);
       cur = I.blasRoot;
       continue;
@@ -426,9 +446,8 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
         best.half = half;
       }
     }
-    if (sp == 0)
+    if (cur == kDone)
       break;
-    cur = pop();
   }
   return best.hit;
 }
