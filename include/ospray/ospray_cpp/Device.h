@@ -4,6 +4,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 
 #include "ospray/ospray.h"
 
@@ -44,6 +45,19 @@ class Device
       move.ospHandle = nullptr;
     }
     return *this;
+  }
+
+  // a callable that converts to OSPErrorCallback / OSPStatusCallback (a
+  // function or a lambda without captures); no user data
+  template <typename FCN_T>
+  void setErrorCallback(FCN_T &&fcn) const
+  {
+    ospDeviceSetErrorCallback(ospHandle, std::forward<FCN_T>(fcn), nullptr);
+  }
+  template <typename FCN_T>
+  void setStatusCallback(FCN_T &&fcn) const
+  {
+    ospDeviceSetStatusCallback(ospHandle, std::forward<FCN_T>(fcn), nullptr);
   }
 
   void setParam(const std::string &name, const int v) const

@@ -49,13 +49,10 @@ int main(int argc, const char **argv)
   int rc = 0;
   {
     cpp::Device dev = cpp::Device::current();
-    ospDeviceSetErrorCallback(
-        dev.handle(),
-        [](void *, OSPError e, const char *msg) {
-          std::fprintf(stderr, "OSPRay error %d: %s\n", (int)e, msg);
-          std::exit(1);
-        },
-        nullptr);
+    dev.setErrorCallback([](void *, OSPError e, const char *msg) {
+      std::fprintf(stderr, "OSPRay error %d: %s\n", (int)e, msg);
+      std::exit(1);
+    });
     const char *out = argc > 1 ? argv[1] : "out.ppm";
     const int frames = argc > 2 ? std::atoi(argv[2]) : 4;
     const int W = 160, H = 120;
