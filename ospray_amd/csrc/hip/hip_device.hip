@@ -1607,7 +1607,9 @@ static std::shared_ptr<Blas> buildBlas(const std::vector<Geometry *> &geoms)
       }
   }
   const int nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
-  hpt::BuildResult R = hpt::buildBVH(boxes, 4, std::min(nthreads, 16));
+  // leaves of at most 2 primitives: measured fastest on the 1080p scene
+  // (leaf size 1, 2, 4, 6, 8: 260, 275, 266, 259, 251 M samples/s)
+  hpt::BuildResult R = hpt::buildBVH(boxes, 2, std::min(nthreads, 16));
   auto B = std::make_shared<Blas>();
   B->nodes = std::move(R.nodes);
   B->bounds = R.bounds;
