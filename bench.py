@@ -238,7 +238,8 @@ def bench_c3(args) -> int:
         return curve, s, ks
 
     plain, _, kplain = run({}, "plain")
-    nrc, snrc, knrc = run({"nrc": True, "nrcBounce": 2, "nrcTrainPaths": 65536, "nrcTrainSteps": 4}, "cache")
+    nrc, snrc, knrc = run({"nrc": True, "nrcBounce": args.c3_bounce, "nrcTrainPaths": args.c3_train_paths,
+                           "nrcTrainSteps": 4}, "cache")
     loss, steps, queries, records = A.nrc_state(snrc.renderer)
     target = plain[-1][2]
 
@@ -263,7 +264,8 @@ def bench_c3(args) -> int:
            "scaling": "weak", "vs_baseline": None, "dtype": "f32 path tracer, bf16 MLP (fp32 accumulate, fp32 Adam)",
            "data": "synthetic",
            "config": {"workload": f"C1 scene {w}x{h}, 1 spp per frame x {args.c3_frames} frames, cache after bounce "
-                                  f"2, 65536 training paths per frame in 4 Adam steps (lr 1e-2); reference "
+                                  f"{args.c3_bounce}, {args.c3_train_paths} training paths per frame in 4 Adam steps "
+                                  f"(lr 1e-2); reference "
                                   f"{nref * ref_spp_frame} spp plain", "device": name},
            "target_rmse": target,
            "plain": {"rmse": [c[2] for c in plain], "seconds": [round(c[1], 4) for c in plain],
@@ -290,6 +292,8 @@ def main() -> int:
                          "512^3 volume")
     ap.add_argument("--c3-frames", type=int, default=256, help="c3: 1-spp frames per method")
     ap.add_argument("--c3-ref-spp", type=int, default=4096, help="c3: samples per pixel of the plain reference")
+    ap.add_argument("--c3-bounce", type=int, default=1, help="c3: depth of the cache vertex (renderer nrcBounce)")
+    ap.add_argument("--c3-train-paths", type=int, default=8192, help="c3: training paths per frame (nrcTrainPaths)")
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)

@@ -789,7 +789,7 @@ struct Renderer : public ManagedObject
   // its optimiser state persist across frames (online training) until
   // "nrcReset" is committed or the seed changes
   bool nrc = false;
-  int nrcBounce = 2, nrcTrainPaths = 65536, nrcTrainSteps = 4;
+  int nrcBounce = 1, nrcTrainPaths = 8192, nrcTrainSteps = 4;
   float nrcLr = 1e-2f;
   uint32_t nrcSeed = 1;
   bool nrcInit = false;
@@ -833,8 +833,8 @@ struct Renderer : public ManagedObject
     geometryLights = getBool("geometryLights", true);
     (void)getFloat("mipMapBias", 0.f);
     nrc = getBool("nrc", false);
-    nrcBounce = std::max(1, getInt("nrcBounce", 2));
-    nrcTrainPaths = std::max(64, getInt("nrcTrainPaths", 65536));
+    nrcBounce = std::max(1, getInt("nrcBounce", 1));
+    nrcTrainPaths = std::max(64, getInt("nrcTrainPaths", 8192));
     nrcTrainSteps = std::max(1, getInt("nrcTrainSteps", 4));
     nrcLr = getFloat("nrcLearningRate", 1e-2f);
     const uint32_t seed = (uint32_t)getInt("nrcSeed", 1);
