@@ -51,7 +51,8 @@ class OrcMaterial(C.Structure):
                 ("eta", C.c_float * 3), ("k", C.c_float * 3), ("roughness", C.c_float), ("etaInside", C.c_float),
                 ("etaOutside", C.c_float), ("attenuationColorInside", C.c_float * 3),
                 ("attenuationColorOutside", C.c_float * 3), ("attenuationDistance", C.c_float),
-                ("color", C.c_float * 3), ("intensity", C.c_float), ("transparency", C.c_float)]
+                ("color", C.c_float * 3), ("intensity", C.c_float), ("transparency", C.c_float),
+                ("ks", C.c_float * 3), ("ns", C.c_float)]
 
 
 class OrcLight(C.Structure):
@@ -263,6 +264,8 @@ class OracleScene:
             mats[i].color = _f3(m.get("color"), (1.0, 1.0, 1.0))  # Luminous.cpp:37-48
             mats[i].intensity = m.get("intensity", 1.0)
             mats[i].transparency = m.get("transparency", 0.0)
+            mats[i].ks = _f3(m.get("ks"))  # OBJ.cpp:32-37
+            mats[i].ns = m.get("ns", 10.0)
         lights = (OrcLight * max(1, len(scene.lights)))()
         for i, l in enumerate(scene.lights):
             lights[i].type = ORC_LIGHT[l["type"]]

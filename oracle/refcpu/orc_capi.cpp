@@ -177,6 +177,20 @@ void orc_fmath(int kind, const float *in, float *out, int64_t n)
   }
 }
 
+void orc_specular(const float *ks3, float ns, const float *wo3, const float *wi3, const float *s2, float *out11,
+    int64_t n)
+{
+  for (int64_t i = 0; i < n; i++)
+    specularHook(v3(ks3[0], ks3[1], ks3[2]), ns, v3(wo3[3 * i], wo3[3 * i + 1], wo3[3 * i + 2]),
+        v3(wi3[3 * i], wi3[3 * i + 1], wi3[3 * i + 2]), v2(s2[2 * i], s2[2 * i + 1]), out11 + 11 * i);
+}
+
+void orc_fpow(const float *x, const float *y, float *out, int64_t n)
+{
+  for (int64_t i = 0; i < n; i++)
+    out[i] = fpow(x[i], y[i]);
+}
+
 void orc_pixel_filter(const OrcRenderer *ren, const float *s2, float *out2, int64_t n)
 {
   RendererData r;

@@ -185,11 +185,14 @@ void setupScene(Scene &s, const OrcScene &d)
     md.type = m.type;
     if (m.type == ORC_MAT_OBJ) { // materials/OBJ.cpp:32-57
       md.Kd = v3(m.kd[0], m.kd[1], m.kd[2]);
+      md.Ks = v3(m.ks[0], m.ks[1], m.ks[2]);
       md.Tf = v3(m.tf[0], m.tf[1], m.tf[2]);
       md.d = m.d;
-      const float total = reduce_max(md.Kd + md.Tf);
+      md.Ns = m.ns;
+      const float total = reduce_max(md.Kd + md.Ks + md.Tf);
       if (total > 1.f) {
         md.Kd = md.Kd / total;
+        md.Ks = md.Ks / total;
         md.Tf = md.Tf / total;
       }
     } else if (m.type == ORC_MAT_METAL) { // materials/Metal.cpp:30-66
@@ -565,9 +568,13 @@ struct BSDF
   uint32_t scatteringType;
   vec3f albedo;
   linear3f frame;
-  // OBJ children: 0 Lambert, 1 Transmission
-  float weight[2], importance[2], importanceSum;
-  vec3f childAlbedo[2];
+  // OBJ children: 0 Lambert, 1 Transmission, 2 Specular
+  float weight[3], importance[3], importanceSum;
+  vec3f childAlbedo[3];
+  // Specular (Specular.ih:14-19): normalised reflectivity, sample weight,
+  // Blinn exponent
+  vec3f specKs, specWeight;
+  float specNs;
   // conductors
   vec3f eta, k;
   float roughness;
@@ -589,6 +596,82 @@ struct SampleRes
   float pdf;
   uint32_t type;
 };
+
+// ---- the OBJ material's specular lobe: a normalised Blinn-Phong BRDF
+// f = r (n+2) / (4 pi (2 - 2^(-n/2))) cos(HN)^n / cos(LN) with the Phong
+// exponent converted to Blinn (x4), sampled by the power-cosine
+// distribution of the half vector (bsdfs/Specular.ih:14-78,
+// math/sampling.ih:70-87)
+static constexpr float kOneOverFourPi = 0.0795774715459476678844f;
+static constexpr float kLnTwoOverTwo = 0.346573590279972654709f;
+
+static float powerCosineSampleHemispherePDF(float cosTheta, float n)
+{
+  return (n + 1.0f) * kOneOverTwoPi * fpow(cosTheta, n);
+}
+
+static vec3f powerCosineSampleHemisphere(float n, vec2f s)
+{
+  const float phi = kTwoPi * s.x;
+  const float cosTheta = fpow(s.y, 1.0f / (n + 1.0f));
+  return cartesian(phi, cos2sin(cosTheta), cosTheta);
+}
+
+// Specular_Constructor and its MultiBSDF_add as child 2 (OBJ.ispc:86-95)
+static void specularLobe(const MaterialData &m, float d, BSDF &b)
+{
+  const float Ns = m.Ns;
+  const vec3f Ks = d * m.Ks;
+  if (!(reduce_max(Ks) > 0.0f))
+    return;
+  b.specNs = Ns * 4.f;
+  const float tmp = (b.specNs + 2.f) * rcp(2.f - fexp(b.specNs * -kLnTwoOverTwo));
+  b.specKs = Ks * (kOneOverFourPi * tmp);
+  b.specWeight = Ks * (2.f * tmp * rcp(b.specNs + 1.f));
+  b.childAlbedo[2] = Ks;
+  b.albedo = b.albedo + Ks * 1.f;
+  b.scatteringType |= GLOSSY_REFL;
+  b.weight[2] = 1.f;
+  b.importance[2] = luminance(Ks);
+  b.importanceSum += b.importance[2];
+}
+
+static EvalRes specularEval(const BSDF &b, vec3f wo, vec3f wi) // Specular.ih:23-41
+{
+  const vec3f N = b.frame.vz;
+  const float cosThetaO = dot(wo, N);
+  const float cosThetaI = dot(wi, N);
+  if (cosThetaO <= 0.0f || cosThetaI <= 0.0f)
+    return {v3(0.f), 0.f};
+  const vec3f wh = normalize(wo + wi);
+  const float cosThetaH = dot(wh, N);
+  EvalRes res;
+  res.pdf = powerCosineSampleHemispherePDF(cosThetaH, b.specNs) * rcp(4.0f * dot(wo, wh));
+  res.value = b.specKs * fpow(cosThetaH, b.specNs);
+  return res;
+}
+
+static SampleRes specularSample(const BSDF &b, vec3f wo, vec2f s) // Specular.ih:43-66
+{
+  SampleRes res;
+  const vec3f whLocal = powerCosineSampleHemisphere(b.specNs, s);
+  const vec3f wh = mul(b.frame, whLocal);
+  const float cosThetaOH = dot(wo, wh);
+  res.wi = (2.0f * cosThetaOH) * wh - wo; // reflect(wo, wh, cosThetaOH), Optics.ih:13-16
+  const float cosThetaI = dot(res.wi, b.frame.vz);
+  if (cosThetaI <= 0.0f) {
+    res.weight = v3(0.f);
+    res.wi = v3(0.f);
+    res.pdf = 0.f;
+    res.type = 0;
+    return res;
+  }
+  const float cosThetaH = whLocal.z;
+  res.pdf = powerCosineSampleHemispherePDF(cosThetaH, b.specNs) * rcp(4.0f * cosThetaOH);
+  res.type = GLOSSY_REFL;
+  res.weight = b.specWeight * cosThetaOH;
+  return res;
+}
 
 static float fresnelConductor1(float cosI, float eta, float k) // Fresnel.ih:75-84
 {
@@ -684,7 +767,7 @@ static bool getBSDF(const Scene &s, const DG &dg, const Medium &medium, BSDF &b)
     b.albedo = v3(0.f);
     b.scatteringType = 0;
     b.importanceSum = 0.f;
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 3; i++)
       b.weight[i] = b.importance[i] = 0.f;
     // makeShadingFrame (render/Material.ih:25-37)
     {
@@ -718,6 +801,7 @@ static bool getBSDF(const Scene &s, const DG &dg, const Medium &medium, BSDF &b)
       b.importance[1] = luminance(T);
       b.importanceSum += b.importance[1];
     }
+    specularLobe(m, d, b);
     return true;
   }
   if (m.type == ORC_MAT_METAL) { // Metal.ispc:18-46
@@ -768,6 +852,13 @@ static EvalRes bsdfEval(const BSDF &b, vec3f wo, vec3f wi)
       cur.value = cur.value * b.weight[0];
       e.value = e.value + cur.value;
       e.pdf += cur.pdf * b.importance[0];
+    }
+    // child 1 (transmission) has no evaluation (OBJ.ispc:120-130)
+    if (b.importance[2] > 0.f) {
+      EvalRes cur = specularEval(b, wo, wi);
+      cur.value = cur.value * b.weight[2];
+      e.value = e.value + cur.value;
+      e.pdf += cur.pdf * b.importance[2];
     }
     e.pdf *= rcp(b.importanceSum);
     return e;
@@ -835,6 +926,11 @@ static SampleRes bsdfSample(const BSDF &b, vec3f wo, vec2f s, float ss)
       res.type = SPEC_TRANS;
       res.weight = b.childAlbedo[1];
       res.weight = res.weight * b.weight[1];
+    } else if (x < (prefixSum += b.importance[2])) {
+      choice = 2;
+      ss = (x + b.importance[2] - prefixSum) * rcp(b.importance[2]);
+      res = specularSample(b, wo, s);
+      res.weight = res.weight * b.weight[2];
     }
     if (eq(res.weight, v3(0.f)) || res.pdf == 0.f)
       return zero;
@@ -852,6 +948,12 @@ static SampleRes bsdfSample(const BSDF &b, vec3f wo, vec2f s, float ss)
       cur.value = cur.value * b.weight[0];
       e.value = e.value + cur.value;
       e.pdf += cur.pdf * b.importance[0];
+    }
+    if (b.importance[2] > 0.f && choice != 2) {
+      EvalRes cur = specularEval(b, wo, wi);
+      cur.value = cur.value * b.weight[2];
+      e.value = e.value + cur.value;
+      e.pdf += cur.pdf * b.importance[2];
     }
     res.pdf = e.pdf;
     res.pdf *= rcp(b.importanceSum);
@@ -1447,6 +1549,22 @@ float volumeTransmittanceHook(const Scene &s, const Ray &ray, float r0, float r1
 float volumeFreePathHook(const Scene &s, Ray &ray, float r0, float r1, RandomSampler &rs, int &volInst, vec3f &albedo)
 {
   return volumeSampleFreePath(s, ray, false, r0, r1, rs, volInst, albedo);
+}
+
+void specularHook(vec3f ks, float ns, vec3f wo, vec3f wi, vec2f s, float out[11])
+{
+  MaterialData m{};
+  m.Ks = ks;
+  m.Ns = ns;
+  BSDF b{};
+  b.frame = {v3(1.f, 0.f, 0.f), v3(0.f, 1.f, 0.f), v3(0.f, 0.f, 1.f)};
+  specularLobe(m, 1.f, b);
+  const EvalRes e = specularEval(b, wo, wi);
+  const SampleRes r = specularSample(b, wo, s);
+  const float v[11] = {e.value.x, e.value.y, e.value.z, e.pdf, r.wi.x, r.wi.y, r.wi.z, r.pdf, r.weight.x, r.weight.y,
+      r.weight.z};
+  for (int k = 0; k < 11; k++)
+    out[k] = v[k];
 }
 
 void hgSampleHook(float g, vec3f wo, vec2f s, vec3f &wi, float &pdf, uint32_t &type)

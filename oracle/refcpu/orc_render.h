@@ -44,8 +44,8 @@ struct MaterialData
 {
   int type;
   // obj
-  vec3f Kd, Tf;
-  float d;
+  vec3f Kd, Tf, Ks;
+  float d, Ns;
   // metal
   vec3f eta, k;
   float roughness;
@@ -159,6 +159,10 @@ float hostExp(float x);
 float volumeTransmittanceHook(const Scene &s, const Ray &ray, float r0, float r1, RandomSampler &rs);
 float volumeFreePathHook(const Scene &s, Ray &ray, float r0, float r1, RandomSampler &rs, int &volInst, vec3f &albedo);
 void hgSampleHook(float g, vec3f wo, vec2f s, vec3f &wi, float &pdf, uint32_t &type);
+// the OBJ specular lobe of reflectivity ks and Phong exponent ns with the
+// shading normal +z: eval(wo, wi) -> out[0..3] (value, pdf); sample(wo, s)
+// -> out[4..10] (wi, pdf, weight)
+void specularHook(vec3f ks, float ns, vec3f wo, vec3f wi, vec2f s, float out[11]);
 float hgEvalHook(float g, vec3f wo, vec3f wi);
 
 } // namespace orc

@@ -89,6 +89,7 @@ typedef struct
   float attenuationColorInside[3], attenuationColorOutside[3];
   float attenuationDistance;
   float color[3], intensity, transparency; // luminous
+  float ks[3], ns; // obj specular
 } OrcMaterial;
 
 enum OrcLightType

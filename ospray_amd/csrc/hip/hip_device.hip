@@ -272,16 +272,16 @@ struct Material : public ManagedObject
       getVec3f("tf", tf, zero);
       getVec3f("ks", ks, zero);
       m.d = getFloat("d", 1.f);
-      (void)getFloat("ns", 10.f);
-      if (std::max(ks[0], std::max(ks[1], ks[2])) > 0.f)
-        throw std::runtime_error("obj material: specular 'ks' is not supported by the hip device");
-      vec3f Kd = v3(kd[0], kd[1], kd[2]), Tf = v3(tf[0], tf[1], tf[2]);
-      const float total = hpt::reduce_max(Kd + v3(0.f) + Tf);
+      m.Ns = getFloat("ns", 10.f);
+      vec3f Kd = v3(kd[0], kd[1], kd[2]), Ks = v3(ks[0], ks[1], ks[2]), Tf = v3(tf[0], tf[1], tf[2]);
+      const float total = hpt::reduce_max(Kd + Ks + Tf);
       if (total > 1.f) {
         Kd = Kd / total;
+        Ks = Ks / total;
         Tf = Tf / total;
       }
       m.Kd[0] = Kd.x, m.Kd[1] = Kd.y, m.Kd[2] = Kd.z;
+      m.Ks[0] = Ks.x, m.Ks[1] = Ks.y, m.Ks[2] = Ks.z;
       m.Tf[0] = Tf.x, m.Tf[1] = Tf.y, m.Tf[2] = Tf.z;
     } else if (subtype == "metal") { // Metal.cpp:30-66
       m.type = hpt::MAT_METAL;

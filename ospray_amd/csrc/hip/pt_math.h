@@ -140,6 +140,17 @@ HPT_HD float fexp(float x)
   return fldexp(y, (int)n);
 }
 
+HPT_HD float flog(float x);
+
+// x^y for x >= 0 as exp(y log x) (the power-cosine lobe of the OBJ specular
+// BSDF, math/sampling.ih:70-87, Specular.ih:23-78)
+HPT_HD float fpow(float x, float y)
+{
+  if (x == 0.0f)
+    return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : kInf);
+  return fexp(y * flog(x));
+}
+
 HPT_HD float flog(float x)
 {
   if (x != x || x < 0.0f)

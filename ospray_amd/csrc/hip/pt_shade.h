@@ -158,12 +158,16 @@ HPT_HD void postIntersect(const DScene &S, vec3f rorg, vec3f rdir, const Hit &hi
 
 struct BSDF
 {
-  int kind; // 1 OBJ (Lambert + Transmission), 2 conductor, 3 microfacet conductor, 4 dielectric
+  int kind; // 1 OBJ (Lambert + Transmission + Specular), 2 conductor, 3 microfacet conductor, 4 dielectric
   uint32_t scatteringType;
   vec3f albedo;
   linear3f frame;
-  float weight[2], importance[2], importanceSum;
-  vec3f childAlbedo[2];
+  float weight[3], importance[3], importanceSum;
+  vec3f childAlbedo[3];
+  // OBJ specular child (Specular.ih:14-19): its reflectivity R is
+  // childAlbedo[2]; the normalised reflectivity and the sample weight are R
+  // times these two factors; specNs is the Blinn exponent
+  float specKsScale, specWeightScale, specNs;
   vec3f eta, k;
   vec2f alpha;
   float etaRel;
@@ -257,17 +261,76 @@ HPT_HD vec3f ggxSampleVisible(vec2f a, vec3f wo, float &pdf, vec2f s)
   return wh;
 }
 
+// ---- the OBJ material's specular lobe: a normalised Blinn-Phong BRDF
+// f = r (n+2) / (4 pi (2 - 2^(-n/2))) cos(HN)^n / cos(LN) with the Phong
+// exponent converted to Blinn (x4), sampled by the power-cosine
+// distribution of the half vector (bsdfs/Specular.ih:14-78,
+// math/sampling.ih:70-87)
+constexpr float kOneOverFourPi = 0.0795774715459476678844f;
+constexpr float kOneOverTwoPi = 0.159154943091895335769f;
+constexpr float kLnTwoOverTwo = 0.346573590279972654709f;
+
+HPT_HD float powerCosineSampleHemispherePDF(float cosTheta, float n)
+{
+  return (n + 1.0f) * kOneOverTwoPi * fpow(cosTheta, n);
+}
+
+HPT_HD vec3f powerCosineSampleHemisphere(float n, vec2f s)
+{
+  const float phi = kTwoPi * s.x;
+  const float cosTheta = fpow(s.y, 1.0f / (n + 1.0f));
+  return cartesian(phi, cos2sin(cosTheta), cosTheta);
+}
+
+HPT_HD EvalRes specularEval(const BSDF &b, vec3f wo, vec3f wi) // Specular.ih:23-41
+{
+  const vec3f N = b.frame.vz;
+  const float cosThetaO = dot(wo, N);
+  const float cosThetaI = dot(wi, N);
+  if (cosThetaO <= 0.0f || cosThetaI <= 0.0f)
+    return {v3(0.f), 0.f};
+  const vec3f wh = normalize(wo + wi);
+  const float cosThetaH = dot(wh, N);
+  EvalRes res;
+  res.pdf = powerCosineSampleHemispherePDF(cosThetaH, b.specNs) * rcp(4.0f * dot(wo, wh));
+  res.value = (b.childAlbedo[2] * b.specKsScale) * fpow(cosThetaH, b.specNs);
+  return res;
+}
+
+HPT_HD SampleRes specularSample(const BSDF &b, vec3f wo, vec2f s) // Specular.ih:43-66
+{
+  SampleRes res;
+  const vec3f whLocal = powerCosineSampleHemisphere(b.specNs, s);
+  const vec3f wh = mul(b.frame, whLocal);
+  const float cosThetaOH = dot(wo, wh);
+  res.wi = (2.0f * cosThetaOH) * wh - wo; // reflect(wo, wh, cosThetaOH), Optics.ih:13-16
+  const float cosThetaI = dot(res.wi, b.frame.vz);
+  if (cosThetaI <= 0.0f) {
+    res.weight = v3(0.f);
+    res.wi = v3(0.f);
+    res.pdf = 0.f;
+    res.type = 0;
+    return res;
+  }
+  const float cosThetaH = whLocal.z;
+  res.pdf = powerCosineSampleHemispherePDF(cosThetaH, b.specNs) * rcp(4.0f * cosThetaOH);
+  res.type = GLOSSY_REFL;
+  res.weight = (b.childAlbedo[2] * b.specWeightScale) * cosThetaOH;
+  return res;
+}
+
 // Material_dispatch_getBSDF (render/MaterialDispatch.ispc:23-72)
-// the lobes of an OBJ material: Lambert (kd) and specular transmission
-// (1 - d, tf) (OBJ.ispc:55-98, MultiBSDF.ih:11-40)
+// the lobes of an OBJ material: Lambert (kd), specular transmission
+// (1 - d, tf) and the specular reflection (ks, ns) (OBJ.ispc:55-98,
+// MultiBSDF.ih:11-40)
 HPT_HD void objLobes(const DMaterial &m, float colorW, BSDF &b)
 {
   b.kind = 1;
   b.albedo = v3(0.f);
   b.scatteringType = 0;
   b.importanceSum = 0.f;
-  b.weight[0] = b.weight[1] = 0.f;
-  b.importance[0] = b.importance[1] = 0.f;
+  b.weight[0] = b.weight[1] = b.weight[2] = 0.f;
+  b.importance[0] = b.importance[1] = b.importance[2] = 0.f;
   const float d = m.d * colorW;
   const vec3f Kd = ld3(m.Kd) * d * v3(1.f, 1.f, 1.f);
   if (reduce_max(Kd) > 0.0f) {
@@ -286,6 +349,20 @@ HPT_HD void objLobes(const DMaterial &m, float colorW, BSDF &b)
     b.weight[1] = 1.f;
     b.importance[1] = luminance(T);
     b.importanceSum += b.importance[1];
+  }
+  // Specular_Constructor and its MultiBSDF_add as child 2 (OBJ.ispc:86-95)
+  const vec3f Ks = d * ld3(m.Ks);
+  if (reduce_max(Ks) > 0.0f) {
+    b.specNs = m.Ns * 4.f;
+    const float tmp = (b.specNs + 2.f) * rcp(2.f - fexp(b.specNs * -kLnTwoOverTwo));
+    b.specKsScale = kOneOverFourPi * tmp;
+    b.specWeightScale = 2.f * tmp * rcp(b.specNs + 1.f);
+    b.childAlbedo[2] = Ks;
+    b.albedo = b.albedo + Ks * 1.f;
+    b.scatteringType |= GLOSSY_REFL;
+    b.weight[2] = 1.f;
+    b.importance[2] = luminance(Ks);
+    b.importanceSum += b.importance[2];
   }
 }
 
@@ -326,9 +403,6 @@ HPT_HD bool getBSDF(const DScene &S, const DG &dg, const DMedium &medium, BSDF &
     b.kind = 1;
     b.albedo = v3(0.f);
     b.scatteringType = 0;
-    b.importanceSum = 0.f;
-    b.weight[0] = b.weight[1] = 0.f;
-    b.importance[0] = b.importance[1] = 0.f;
     { // makeShadingFrame (render/Material.ih:25-37)
       const vec3f N = dg.Ns;
       vec3f U = normalize(dg.dPds);
@@ -380,6 +454,13 @@ HPT_HD EvalRes bsdfEval(const BSDF &b, vec3f wo, vec3f wi)
       e.value = e.value + cur.value;
       e.pdf += cur.pdf * b.importance[0];
     }
+    // child 1 (transmission) has no evaluation (OBJ.ispc:120-130)
+    if (b.importance[2] > 0.f) {
+      EvalRes cur = specularEval(b, wo, wi);
+      cur.value = cur.value * b.weight[2];
+      e.value = e.value + cur.value;
+      e.pdf += cur.pdf * b.importance[2];
+    }
     e.pdf *= rcp(b.importanceSum);
     return e;
   }
@@ -406,7 +487,7 @@ HPT_HD EvalRes bsdfEval(const BSDF &b, vec3f wo, vec3f wi)
   return z;
 }
 
-// A smooth BSDF (OBJ Lambert, MicrofacetConductor) as far as bsdfEval reads
+// A smooth BSDF (OBJ Lambert + Specular, MicrofacetConductor) as far as bsdfEval reads
 // it, rebuilt from the material and the shading normal: bsdfEval uses only
 // frame.vz of the OBJ shading frame, which is Ns whichever tangents built it
 // (Material.ih:25-37), and metal's frame(Ns).  colorW is 1: postIntersect
@@ -459,16 +540,21 @@ HPT_HD SampleRes bsdfSample(const BSDF &b, vec3f wo, vec2f s, float ss)
       res.type = SPEC_TRANS;
       res.weight = b.childAlbedo[1];
       res.weight = res.weight * b.weight[1];
+    } else if (x < (prefixSum += b.importance[2])) {
+      choice = 2;
+      res = specularSample(b, wo, s);
+      res.weight = res.weight * b.weight[2];
     }
     if (eq(res.weight, v3(0.f)) || res.pdf == 0.f)
       return zero;
+    const float chosen = choice == 0 ? b.importance[0] : choice == 1 ? b.importance[1] : b.importance[2];
     if (res.type & S_SPECULAR) {
-      res.weight = res.weight * (b.importanceSum * rcp(choice == 0 ? b.importance[0] : b.importance[1]));
+      res.weight = res.weight * (b.importanceSum * rcp(chosen));
       return res;
     }
     EvalRes e;
     e.value = res.weight * res.pdf;
-    res.pdf *= choice == 0 ? b.importance[0] : b.importance[1];
+    res.pdf *= chosen;
     e.pdf = res.pdf;
     const vec3f wi = res.wi;
     if (b.importance[0] > 0.f && choice != 0) {
@@ -476,6 +562,12 @@ HPT_HD SampleRes bsdfSample(const BSDF &b, vec3f wo, vec2f s, float ss)
       cur.value = cur.value * b.weight[0];
       e.value = e.value + cur.value;
       e.pdf += cur.pdf * b.importance[0];
+    }
+    if (b.importance[2] > 0.f && choice != 2) {
+      EvalRes cur = specularEval(b, wo, wi);
+      cur.value = cur.value * b.weight[2];
+      e.value = e.value + cur.value;
+      e.pdf += cur.pdf * b.importance[2];
     }
     res.pdf = e.pdf;
     res.pdf *= rcp(b.importanceSum);

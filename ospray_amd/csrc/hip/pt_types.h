@@ -99,6 +99,7 @@ struct DMaterial
 {
   int32_t type;
   float Kd[3], Tf[3], d; // obj
+  float Ks[3], Ns; // obj specular
   float eta[3], k[3], roughness; // metal
   int32_t inside, outside; // glass: indices into the media table
   float radiance[3], transparency; // luminous

@@ -84,16 +84,14 @@ def test_object_layer_and_errors():
         "        print('accepted', kind, t)\n"
         "    except A.OSPRayError as e:\n"
         "        print('rejected', 'Could not find' in str(e))\n"
-        "try:\n"
-        "    A.new('material', 'obj', {'ks': (0.5, 0.5, 0.5)})\n"
-        "except A.OSPRayError as e:\n"
-        "    print('ks', 'not supported' in str(e))\n"
+        "A.new('material', 'obj', {'ks': (0.5, 0.5, 0.5), 'ns': 20.0})\n"
+        "print('ks accepted')\n"
         "cam = A.new('camera', 'perspective', {'fovy': 30.0})\n"
         "L.ospRelease(cam.h)\n"
         "print('property', L.ospDeviceGetProperty(L.ospGetCurrentDevice(), 1))\n")
     lines = out.strip().splitlines()
     assert lines[:4] == ["rejected True"] * 4
-    assert lines[4] == "ks True"
+    assert lines[4] == "ks accepted"
     assert lines[5] == "property 3"
 
 

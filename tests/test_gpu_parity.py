@@ -255,6 +255,36 @@ def test_edge_cases_bit_exact(hip_device, name):
     assert K.lib().ospGetVariance(fb.h) == pytest.approx(ofb.variance(oren), rel=1e-6, nan_ok=True)
 
 
+def _c0_specular(max_path_length):
+    # the C0 spheres with OBJ specular lobes: ks, ns, partial opacity d and
+    # transmission tf, and one material whose kd + ks exceeds 1 (the energy
+    # normalisation of OBJ.cpp:39-47)
+    sc = scenes.c0(24, 16, spp=2, max_path_length=max_path_length)
+    for i, m in enumerate(sc.materials[:-1]):
+        k = i % 4
+        if k == 0:
+            m.update(ks=(0.3, 0.3, 0.3), ns=10.0)
+        elif k == 1:
+            m.update(ks=(0.5, 0.4, 0.2), ns=50.0, d=0.7, tf=(0.2, 0.3, 0.4))
+        elif k == 2:
+            m.update(kd=(0.9, 0.8, 0.7), ks=(0.6, 0.6, 0.6), ns=3.0)
+    sc.materials[-1].update(ks=(0.2, 0.2, 0.2), ns=100.0)
+    return sc
+
+
+@pytest.mark.parametrize("max_path_length", [2, 6])
+def test_obj_specular_bit_exact(hip_device, max_path_length):
+    # the MultiBSDF with the Blinn-Phong child: sampled by choice, evaluated
+    # in the sample's combination and in next-event estimation
+    sc = _c0_specular(max_path_length)
+    s, fb, ofb, _, oren = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    assert K.lib().ospGetVariance(fb.h) == pytest.approx(ofb.variance(oren), rel=1e-6, nan_ok=True)
+    assert col[..., :3].mean() > 0.01
+
+
 def _set_device_params(**kv):
     L = K.lib()
     dev = L.ospGetCurrentDevice()
