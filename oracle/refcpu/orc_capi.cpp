@@ -185,6 +185,14 @@ void orc_specular(const float *ks3, float ns, const float *wo3, const float *wi3
         v3(wi3[3 * i], wi3[3 * i + 1], wi3[3 * i + 2]), v2(s2[2 * i], s2[2 * i + 1]), out11 + 11 * i);
 }
 
+void orc_bsdf(const OrcMaterial *m, const float *wo3, const float *wi3, const float *s2, const float *ss,
+    float *out13, int64_t n)
+{
+  for (int64_t i = 0; i < n; i++)
+    bsdfHook(*m, v3(wo3[3 * i], wo3[3 * i + 1], wo3[3 * i + 2]), v3(wi3[3 * i], wi3[3 * i + 1], wi3[3 * i + 2]),
+        v2(s2[2 * i], s2[2 * i + 1]), ss[i], out13 + 13 * i);
+}
+
 void orc_fpow(const float *x, const float *y, float *out, int64_t n)
 {
   for (int64_t i = 0; i < n; i++)

@@ -175,13 +175,9 @@ static SampleAreaRes sampleArea(const OrcGeometry &g, int64_t p, const affine3f 
   return res;
 }
 
-void setupScene(Scene &s, const OrcScene &d)
+static void setupMaterial(MaterialData &md, const OrcMaterial &m)
 {
-  s.world.build(d);
-  s.materials.resize(d.numMaterials);
-  for (int i = 0; i < d.numMaterials; i++) {
-    const OrcMaterial &m = d.materials[i];
-    MaterialData &md = s.materials[i];
+  {
     md.type = m.type;
     if (m.type == ORC_MAT_OBJ) { // materials/OBJ.cpp:32-57
       md.Kd = v3(m.kd[0], m.kd[1], m.kd[2]);
@@ -213,6 +209,14 @@ void setupScene(Scene &s, const OrcScene &d)
                                    std::log(m.attenuationColorOutside[2])) / dist;
     }
   }
+}
+
+void setupScene(Scene &s, const OrcScene &d)
+{
+  s.world.build(d);
+  s.materials.resize(d.numMaterials);
+  for (int i = 0; i < d.numMaterials; i++)
+    setupMaterial(s.materials[i], d.materials[i]);
   // geometry lights (PathTracerData.cpp:121-162): every emissive model of
   // every instance, in instance and group order; the model's areaPDF is the
   // one of the last instance that gathers it (GeometricModel.ispc:14-55)
@@ -1549,6 +1553,31 @@ float volumeTransmittanceHook(const Scene &s, const Ray &ray, float r0, float r1
 float volumeFreePathHook(const Scene &s, Ray &ray, float r0, float r1, RandomSampler &rs, int &volInst, vec3f &albedo)
 {
   return volumeSampleFreePath(s, ray, false, r0, r1, rs, volInst, albedo);
+}
+
+void bsdfHook(const OrcMaterial &om, vec3f wo, vec3f wi, vec2f s, float ss, float out[13])
+{
+  Scene sc;
+  sc.materials.resize(1);
+  setupMaterial(sc.materials[0], om);
+  DG dg{};
+  dg.Ng = dg.Ns = v3(0.f, 0.f, 1.f);
+  dg.dPds = v3(1.f, 0.f, 0.f);
+  dg.dPdt = v3(0.f, 1.f, 0.f);
+  dg.color = {1.f, 1.f, 1.f, 1.f};
+  dg.material = 0;
+  BSDF b{};
+  if (!getBSDF(sc, dg, sc.materials[0].outside, b)) {
+    for (int k = 0; k < 13; k++)
+      out[k] = 0.f;
+    return;
+  }
+  const EvalRes e = bsdfEval(b, wo, wi);
+  const SampleRes r = bsdfSample(b, wo, s, ss);
+  const float v[13] = {e.value.x, e.value.y, e.value.z, e.pdf, r.wi.x, r.wi.y, r.wi.z, r.pdf, r.weight.x,
+      r.weight.y, r.weight.z, (float)r.type, (float)b.scatteringType};
+  for (int k = 0; k < 13; k++)
+    out[k] = v[k];
 }
 
 void specularHook(vec3f ks, float ns, vec3f wo, vec3f wi, vec2f s, float out[11])

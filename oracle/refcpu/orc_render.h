@@ -163,6 +163,10 @@ void hgSampleHook(float g, vec3f wo, vec2f s, vec3f &wi, float &pdf, uint32_t &t
 // shading normal +z: eval(wo, wi) -> out[0..3] (value, pdf); sample(wo, s)
 // -> out[4..10] (wi, pdf, weight)
 void specularHook(vec3f ks, float ns, vec3f wo, vec3f wi, vec2f s, float out[11]);
+// the BSDF of one material (shading frame = world axes, entered from the
+// outside medium): eval(wo, wi) value and pdf, then sample(wo, s, ss) wi,
+// pdf, weight, type, and the BSDF's scattering type
+void bsdfHook(const OrcMaterial &m, vec3f wo, vec3f wi, vec2f s, float ss, float out[13]);
 float hgEvalHook(float g, vec3f wo, vec3f wi);
 
 } // namespace orc
