@@ -52,7 +52,7 @@ class OrcMaterial(C.Structure):
                 ("etaOutside", C.c_float), ("attenuationColorInside", C.c_float * 3),
                 ("attenuationColorOutside", C.c_float * 3), ("attenuationDistance", C.c_float),
                 ("color", C.c_float * 3), ("intensity", C.c_float), ("transparency", C.c_float),
-                ("ks", C.c_float * 3), ("ns", C.c_float)]
+                ("ks", C.c_float * 3), ("ns", C.c_float), ("ior", C.c_void_p), ("numIor", C.c_int64)]
 
 
 class OrcLight(C.Structure):
@@ -266,6 +266,11 @@ class OracleScene:
             mats[i].transparency = m.get("transparency", 0.0)
             mats[i].ks = _f3(m.get("ks"))  # OBJ.cpp:32-37
             mats[i].ns = m.get("ns", 10.0)
+            if t == "metal" and m.get("ior") is not None:  # Metal.cpp:30-57
+                ior = np.ascontiguousarray(m["ior"], np.float32).reshape(-1, 3)
+                self._keep.append(ior)
+                mats[i].ior = ior.ctypes.data
+                mats[i].numIor = len(ior)
         lights = (OrcLight * max(1, len(scene.lights)))()
         for i, l in enumerate(scene.lights):
             lights[i].type = ORC_LIGHT[l["type"]]

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <stdexcept>
 #include <thread>
 
 namespace orc {
@@ -195,6 +196,28 @@ static void setupMaterial(MaterialData &md, const OrcMaterial &m)
       md.eta = v3(m.eta[0], m.eta[1], m.eta[2]);
       md.k = v3(m.k[0], m.k[1], m.k[2]);
       md.roughness = m.roughness;
+      md.spectral = m.ior != nullptr;
+      if (md.spectral) { // Metal.cpp:34-57: linear resampling to 430 + 35 l nm
+        if (m.numIor <= 0)
+          throw std::runtime_error("metal: empty ior");
+        int64_t j = 0;
+        int64_t prev = 0;
+        for (int l = 0; l < 8; l++) {
+          const float wl = 430.f + 35.f * (float)l;
+          while (j < m.numIor && m.ior[3 * j] < wl)
+            prev = j++;
+          const int64_t next = j < m.numIor ? j : prev; // past the table: the last entry holds
+          const float *a = m.ior + 3 * prev, *c = m.ior + 3 * next;
+          if (c[0] == a[0]) {
+            md.etaS[l] = a[1];
+            md.kS[l] = a[2];
+          } else {
+            const float f = (wl - a[0]) / (c[0] - a[0]);
+            md.etaS[l] = (1.f - f) * a[1] + f * c[1];
+            md.kS[l] = (1.f - f) * a[2] + f * c[2];
+          }
+        }
+      }
     } else if (m.type == ORC_MAT_LUMINOUS) { // materials/Luminous.cpp:37-48, Luminous.h:25-28
       md.radiance = v3(m.color[0], m.color[1], m.color[2]) * m.intensity;
       md.transparency = m.transparency;
@@ -581,6 +604,7 @@ struct BSDF
   float specNs;
   // conductors
   vec3f eta, k;
+  const MaterialData *spectralMat; // a spectral conductor, else null
   float roughness;
   vec2f alpha;
   // dielectric
@@ -684,8 +708,24 @@ static float fresnelConductor1(float cosI, float eta, float k) // Fresnel.ih:75-
   const float Rper = (tmp - 2.0f * eta * cosI + sqr(cosI)) * rcp(tmp + 2.0f * eta * cosI + sqr(cosI));
   return 0.5f * (Rpar + Rper);
 }
+// spectrum_sRGB (math/spectrum.ispc:11-38): linear sRGB of each of the 8
+// wavelength samples
+static const float kSpectrumRGB[3][8] = {
+    {0.0598548f, -0.0234574f, -0.220138f, -0.238902f, 0.316327f, 0.738315f, 0.323302f, 0.0446981f},
+    {-0.0567346f, -0.0160361f, 0.223861f, 0.531185f, 0.337221f, 0.0149718f, -0.0296053f, -0.00486239f},
+    {0.420693f, 0.616597f, 0.0796766f, -0.0496266f, -0.0473149f, -0.0167536f, -0.00295686f, -0.000314818f}};
+
+// Fresnel_dispatch_eval for FresnelConductorRGBUniform (Fresnel.ih:131-141)
+// and FresnelConductorSpectral (Fresnel.ih:83-90)
 static vec3f fresnelRGB(const BSDF &b, float cosI)
 {
+  if (b.spectralMat) {
+    vec3f rgb = v3(0.f);
+    for (int l = 0; l < 8; l++)
+      rgb = rgb + fresnelConductor1(cosI, b.spectralMat->etaS[l], b.spectralMat->kS[l]) *
+          v3(kSpectrumRGB[0][l], kSpectrumRGB[1][l], kSpectrumRGB[2][l]);
+    return v3(clampf(rgb.x), clampf(rgb.y), clampf(rgb.z));
+  }
   return v3(fresnelConductor1(cosI, b.eta.x, b.k.x), fresnelConductor1(cosI, b.eta.y, b.k.y),
       fresnelConductor1(cosI, b.eta.z, b.k.z));
 }
@@ -811,6 +851,7 @@ static bool getBSDF(const Scene &s, const DG &dg, const Medium &medium, BSDF &b)
   if (m.type == ORC_MAT_METAL) { // Metal.ispc:18-46
     b.frame = frame(dg.Ns);
     b.eta = m.eta;
+    b.spectralMat = m.spectral ? &m : nullptr;
     b.k = m.k;
     // FresnelConductorRGBUniform has no evalAvg (Fresnel.ih:307-319): Favg = 0,
     // hence albedo 0 and no Kulla-Conty energy compensation term.

@@ -49,6 +49,8 @@ struct MaterialData
   // metal
   vec3f eta, k;
   float roughness;
+  bool spectral; // eta and k sampled at 430, 465, ..., 675 nm
+  float etaS[8], kS[8];
   // glass
   Medium inside, outside;
   // luminous (Luminous.cpp:37-48)

@@ -90,6 +90,8 @@ typedef struct
   float attenuationDistance;
   float color[3], intensity, transparency; // luminous
   float ks[3], ns; // obj specular
+  const float *ior; // metal: (wavelength nm, eta, k) triples ordered by wavelength, or null
+  int64_t numIor;
 } OrcMaterial;
 
 enum OrcLightType

@@ -114,6 +114,35 @@ static Data *getData(ManagedObject *o, const char *name)
   return (Data *)o->getObject(name, OSP_DATA);
 }
 
+static std::vector<float> dataFloats(Data *d, int comps, const char *what);
+
+// Resamples a metal's (wavelength nm, eta, k) table, ordered by wavelength,
+// to the 8 samples at 430, 465, ..., 675 nm by linear interpolation
+// (Metal.cpp:34-57).  Before the first and after the last entry the
+// nearest entry's values hold.
+static void resampleIor(const std::vector<float> &t, float etaS[8], float kS[8])
+{
+  const size_t n = t.size() / 3;
+  if (n == 0)
+    throw std::runtime_error("metal material: 'ior' is empty");
+  size_t p = 0;
+  const float *prev = &t[0];
+  float wl = 430.f;
+  for (int l = 0; l < 8; wl += 35.f, l++) {
+    for (; p < n && t[3 * p] < wl; p++)
+      prev = &t[3 * p];
+    const float *cur = p < n ? &t[3 * p] : prev;
+    if (cur[0] == prev[0]) {
+      etaS[l] = prev[1];
+      kS[l] = prev[2];
+    } else {
+      const float f = (wl - prev[0]) / (cur[0] - prev[0]);
+      etaS[l] = (1.f - f) * prev[1] + f * cur[1];
+      kS[l] = (1.f - f) * prev[2] + f * cur[2];
+    }
+  }
+}
+
 static std::vector<float> dataFloats(Data *d, int comps, const char *what)
 {
   const size_t es = sizeOfDataType(d->elemType);
@@ -289,8 +318,13 @@ struct Material : public ManagedObject
       getVec3f("eta", m.eta, etaDef);
       getVec3f("k", m.k, kDef);
       m.roughness = getFloat("roughness", 0.1f);
-      if (getData(this, "ior"))
-        throw std::runtime_error("metal material: spectral 'ior' is not supported by the hip device");
+      m.spectral = 0;
+      if (Data *ior = getData(this, "ior")) {
+        if (ior->elemType != OSP_VEC3F)
+          throw std::runtime_error("metal material: 'ior' must be an array of vec3f (wavelength, eta, k)");
+        resampleIor(dataFloats(ior, 3, "metal 'ior'"), m.etaS, m.kS);
+        m.spectral = 1;
+      }
     } else if (subtype == "glass") { // Glass.cpp:36-62
       m.type = hpt::MAT_GLASS;
       const float etaInside = getFloat("etaInside", getFloat("eta", 1.5f));

@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void k_volume(DScene S, DPaths P, int qin, int
 // (NextEventEstimation.ispc:25-160).  The loop runs over whole waves so the
 // appends can scan across the wave.
 template <bool VOL, bool NRC>
-__global__ __launch_bounds__(256) void k_shade(DScene S, DFrame F, DPaths P, int qin)
+__global__ __launch_bounds__(256, VOL ? 2 : 3) void k_shade(DScene S, DFrame F, DPaths P, int qin)
 {
   const int n = P.counters[2 * qin];
   const int qout = qin ^ 1;

@@ -169,6 +169,7 @@ struct BSDF
   // times these two factors; specNs is the Blinn exponent
   float specKsScale, specWeightScale, specNs;
   vec3f eta, k;
+  const float *spec; // spectral conductor: eta[8] then k[8], else null
   vec2f alpha;
   float etaRel;
   vec3f Ng, Ns;
@@ -194,8 +195,32 @@ HPT_HD float fresnelConductor1(float cosI, float eta, float k) // Fresnel.ih:75-
   const float Rper = (tmp - 2.0f * eta * cosI + sqr(cosI)) * rcp(tmp + 2.0f * eta * cosI + sqr(cosI));
   return 0.5f * (Rpar + Rper);
 }
+// the CIE-to-linear-sRGB weights of the eight spectral samples at 430,
+// 465, ..., 675 nm (math/spectrum.ispc:11-38)
+constexpr float kSpectrumSRGB[3][8] = {
+    {0.0598548f, -0.0234574f, -0.220138f, -0.238902f, 0.316327f, 0.738315f, 0.323302f, 0.0446981f},
+    {-0.0567346f, -0.0160361f, 0.223861f, 0.531185f, 0.337221f, 0.0149718f, -0.0296053f, -0.00486239f},
+    {0.420693f, 0.616597f, 0.0796766f, -0.0496266f, -0.0473149f, -0.0167536f, -0.00295686f, -0.000314818f}};
+
+// the conductor's Fresnel term: per RGB channel (FresnelConductorRGBUniform,
+// Fresnel.ih:131-141) or summed over the spectrum and clamped to [0, 1]
+// (FresnelConductorSpectral, Fresnel.ih:83-90, 199-213)
+// a rolled loop, so the rarely taken spectral sum holds few registers in
+// the shading kernels
+HPT_HD vec3f fresnelSpectral(const float *spec, float cosI)
+{
+  vec3f rgb = v3(0.f);
+#pragma unroll 1
+  for (int l = 0; l < 8; l++) {
+    const float F = fresnelConductor1(cosI, spec[l], spec[8 + l]);
+    rgb = rgb + F * v3(kSpectrumSRGB[0][l], kSpectrumSRGB[1][l], kSpectrumSRGB[2][l]);
+  }
+  return v3(clampf(rgb.x), clampf(rgb.y), clampf(rgb.z));
+}
 HPT_HD vec3f fresnelRGB(const BSDF &b, float cosI)
 {
+  if (b.spec)
+    return fresnelSpectral(b.spec, cosI);
   return v3(fresnelConductor1(cosI, b.eta.x, b.k.x), fresnelConductor1(cosI, b.eta.y, b.k.y),
       fresnelConductor1(cosI, b.eta.z, b.k.z));
 }
@@ -371,7 +396,8 @@ HPT_HD void metalLobe(const DMaterial &m, BSDF &b)
 {
   b.eta = ld3(m.eta);
   b.k = ld3(m.k);
-  // RGB conductor Fresnel has no evalAvg (Fresnel.ih:307-319): Favg = 0,
+  b.spec = m.spectral ? m.etaS : nullptr;
+  // neither conductor Fresnel has an evalAvg (Fresnel.ih:307-319): Favg = 0,
   // so the albedo and the Kulla-Conty compensation term are zero
   b.albedo = v3(0.f);
   if (m.roughness == 0.0f) {

@@ -103,6 +103,10 @@ struct DMaterial
   float eta[3], k[3], roughness; // metal
   int32_t inside, outside; // glass: indices into the media table
   float radiance[3], transparency; // luminous
+  // metal with a spectral 'ior': eta and k at kSpectrumSamples wavelengths,
+  // kS directly after etaS (MetalShared.h)
+  int32_t spectral;
+  float etaS[8], kS[8];
 };
 
 struct DMedium

@@ -447,6 +447,9 @@ def to_ospray(scene: Scene, renderer_params: Optional[dict] = None) -> OspScene:
     mat_h = []
     for m in scene.materials:
         p = {k: v for k, v in m.items() if k != "type"}
+        if "ior" in p:  # metal: (wavelength, eta, k) rows as a vec3f array
+            p["ior"] = A.shared_data(np.ascontiguousarray(p["ior"], np.float32).reshape(-1, 3))
+            keep.append(p["ior"])
         mat_h.append(A.new("material", m["type"], p))
     model_h = []
     for m in scene.models:

@@ -285,6 +285,28 @@ def test_obj_specular_bit_exact(hip_device, max_path_length):
     assert col[..., :3].mean() > 0.01
 
 
+# a copper-like (wavelength nm, eta, k) table, ordered by wavelength
+_COPPER_IOR = np.array([[400.0, 1.18, 2.21], [450.0, 1.17, 2.40], [500.0, 1.13, 2.57], [550.0, 0.72, 2.62],
+                        [600.0, 0.27, 3.24], [650.0, 0.21, 3.67], [700.0, 0.21, 4.05]], np.float32)
+
+
+@pytest.mark.parametrize("max_path_length", [2, 5])
+def test_spectral_metal_bit_exact(hip_device, max_path_length):
+    # metal with an 'ior' table (Metal.cpp:34-57): smooth and rough, beside
+    # RGB metals and Lambertian spheres
+    sc = scenes.c0(24, 16, spp=2, max_path_length=max_path_length)
+    for i in range(0, 64, 3):
+        sc.materials[i] = {"type": "metal", "roughness": (0.0, 0.25, 0.6)[(i // 3) % 3],
+                           "ior": _COPPER_IOR}
+    for i in range(1, 64, 6):
+        sc.materials[i] = {"type": "metal", "roughness": 0.3}
+    s, fb, ofb, _, oren = _render_both(sc, frames=2)
+    col = A.map_color(fb, sc.width, sc.height, K.OSP_FB_RGBA32F)
+    assert np.array_equal(col, ofb.channel(K.OSP_FB_COLOR))
+    _assert_channels_equal(fb, ofb, sc, ALL_CH)
+    assert col[..., :3].mean() > 0.01
+
+
 def _set_device_params(**kv):
     L = K.lib()
     dev = L.ospGetCurrentDevice()

@@ -4,7 +4,9 @@ conductor Fresnel term at normal and grazing incidence (bsdfs/Fresnel.ih:
 through the visible-normal sampler and through the eval pdf
 (GGXDistribution.ih, MicrofacetConductor.ih:27-110), sample/eval
 consistency and the smooth-limit white furnace of the microfacet
-conductor, and RobustDielectric's split (RobustDielectric.ih:31-160):
+conductor, the spectral conductor's resampling of its ior table and its
+sum over the spectrum (Metal.cpp:34-57, Fresnel.ih:83-90), and
+RobustDielectric's split (RobustDielectric.ih:31-160):
 mirror reflection with the probability of the Fresnel equations in angle
 form, Snell's law for the refracted direction, total internal reflection
 and pass-through at equal indices."""
@@ -35,6 +37,54 @@ def _material(kind, **kv):
     m.intensity = 1.0
     m.ns = 10.0
     return m
+
+
+def _metal_ior(table, roughness=0.0):
+    m = _material("metal", roughness=roughness)
+    t = np.ascontiguousarray(table, np.float32).reshape(-1, 3)
+    m.ior = t.ctypes.data
+    m.numIor = len(t)
+    m._table = t  # keep the array alive with the structure
+    return m
+
+
+# spectrum_sRGB (math/spectrum.ispc:11-38), data of the reference
+_SRGB = np.array([
+    [0.0598548, -0.0234574, -0.220138, -0.238902, 0.316327, 0.738315, 0.323302, 0.0446981],
+    [-0.0567346, -0.0160361, 0.223861, 0.531185, 0.337221, 0.0149718, -0.0296053, -0.00486239],
+    [0.420693, 0.616597, 0.0796766, -0.0496266, -0.0473149, -0.0167536, -0.00295686, -0.000314818]])
+_WL = 430.0 + 35.0 * np.arange(8)
+
+
+def test_spectral_conductor_flat_spectrum_is_grey():
+    # wavelength-independent eta and k: every sample has the same
+    # reflectance F, and F times each row sum of the sRGB weights (all
+    # within 2e-6 of 1: a flat spectrum is white) clamps to F or 1
+    eta, k = 0.3, 2.9
+    m = _metal_ior([[380.0, eta, k], [780.0, eta, k]])
+    out = _bsdf(m, np.array([[0.0, 0.0, 1.0]]), (0.0, 0.0, 1.0), (0.5, 0.5), 0.5)
+    F = ((eta - 1) ** 2 + k ** 2) / ((eta + 1) ** 2 + k ** 2)
+    np.testing.assert_allclose(out[0, 8:11], np.minimum(1.0, F * _SRGB.sum(1)), rtol=3e-6)
+    assert np.all(np.abs(_SRGB.sum(1) - 1.0) < 2e-6)
+
+
+def test_spectral_conductor_resampling():
+    # eta linear in the wavelength between two entries at 400 and 700 nm,
+    # k = 0: linear resampling reproduces the line exactly, and the Fresnel
+    # term at normal incidence is ((eta - 1) / (eta + 1))^2 per sample
+    m = _metal_ior([[400.0, 1.2, 0.0], [700.0, 2.0, 0.0]])
+    eta = 1.2 + 0.8 * (_WL - 400.0) / 300.0
+    F = ((eta - 1) / (eta + 1)) ** 2
+    out = _bsdf(m, np.array([[0.0, 0.0, 1.0]]), (0.0, 0.0, 1.0), (0.5, 0.5), 0.5)
+    np.testing.assert_allclose(out[0, 8:11], np.clip(_SRGB @ F, 0.0, 1.0), atol=2e-7)
+    # entries only beyond 500 nm and only before 600 nm: the nearest entry
+    # holds outside the table; one entry: a flat spectrum
+    for table, eta in (([[500.0, 1.5, 0.0], [800.0, 1.5, 0.0]], np.full(8, 1.5)),
+                       ([[300.0, 1.1, 0.0], [600.0, 1.4, 0.0]], np.minimum(1.1 + 0.3 * (_WL - 300.0) / 300.0, 1.4)),
+                       ([[550.0, 1.7, 0.0]], np.full(8, 1.7))):
+        F = ((eta - 1) / (eta + 1)) ** 2
+        out = _bsdf(_metal_ior(table), np.array([[0.0, 0.0, 1.0]]), (0.0, 0.0, 1.0), (0.5, 0.5), 0.5)
+        np.testing.assert_allclose(out[0, 8:11], np.clip(_SRGB @ F, 0.0, 1.0), atol=2e-7)
 
 
 def _bsdf(m, wo, wi, s, ss):
