@@ -47,11 +47,12 @@ OSPRAY_INTERFACE int ospHipRenderBatch(int n, const OSPFrameBuffer *framebuffers
 /* Autotuner of a batch: times the eager batch (best of `reps`) for each
  * candidate resident traversal grid (8, 12, 16 blocks per CU), shading
  * grid (4, 8, 16) and path-state budget (2 GiB, 16 GiB: the samples per
- * pass, i.e. the wavefront queue sizes), keeps the fastest as the device's
- * setting.  Returns its batch milliseconds; best3 = traceBlocksPerCU,
- * shadeBlocksPerCU, pathBudgetMB. */
+ * pass, i.e. the wavefront queue sizes), then, at the fastest of those,
+ * for 1, 2, 4 and 8 lanes (batchStreams), and keeps the fastest as the
+ * device's setting.  Returns its batch milliseconds; best4 (4 ints) =
+ * traceBlocksPerCU, shadeBlocksPerCU, pathBudgetMB, batchStreams. */
 OSPRAY_INTERFACE float ospHipTuneBatch(int n, const OSPFrameBuffer *framebuffers, const OSPRenderer *renderers,
-    const OSPCamera *cameras, OSPWorld world, int reps, int *best3);
+    const OSPCamera *cameras, OSPWorld world, int reps, int *best4);
 
 /* The radiance cache of a "pathtracer" renderer with "nrc" set: the loss of
  * its last training step (relative L2), the number of Adam steps taken so

@@ -1514,11 +1514,31 @@ class HipDevice : public Device
     shadeBlocksPerCU = bs;
     pathBudgetBytes = bb;
     sizeGrids();
+    // second stage at the chosen grids and budget: the lanes the views
+    // spread over (1 when a batch cannot run concurrently, see batchLanes)
+    const int ls0 = batchStreams;
+    int bl = ls0;
+    bestMs = 1e30f;
+    for (int ls : {1, 2, 4, 8}) {
+      batchStreams = ls;
+      renderBatch(v, world, 0, ms.data()); // warm-up (lane allocations)
+      float t = 1e30f;
+      for (int k = 0; k < std::max(1, reps); k++) {
+        renderBatch(v, world, 0, ms.data());
+        t = std::min(t, ms.back());
+      }
+      if (t < bestMs) {
+        bestMs = t;
+        bl = ls;
+      }
+    }
+    batchStreams = bl;
     releaseGraph();
     if (best) {
       best[0] = bt;
       best[1] = bs;
       best[2] = (int)(bb >> 20);
+      best[3] = bl;
     }
     return bestMs;
   }
@@ -2582,14 +2602,14 @@ OSPRAY_INTERFACE int ospHipRenderBatch(int n, const OSPFrameBuffer *fbs, const O
 }
 
 OSPRAY_INTERFACE float ospHipTuneBatch(int n, const OSPFrameBuffer *fbs, const OSPRenderer *renderers,
-    const OSPCamera *cameras, OSPWorld world, int reps, int *best3)
+    const OSPCamera *cameras, OSPWorld world, int reps, int *best4)
 {
   HipDevice *d = current();
   std::vector<HipDevice::BatchView> v;
   if (!d || !world || !batchViews(n, fbs, renderers, cameras, v))
     return -1.f;
   try {
-    return d->tuneBatch(v, (ManagedObject *)world, reps, best3);
+    return d->tuneBatch(v, (ManagedObject *)world, reps, best4);
   } catch (const std::exception &e) {
     extensionError(d, e.what());
     return -2.f;
