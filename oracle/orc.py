@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "liborc.so")
+# ORC_LIB names another build of the oracle (for instance one with a planted
+# error, to show that the pinning tests reject it)
+LIBPATH = os.environ.get("ORC_LIB", os.path.join(HERE, "liborc.so"))
 
 ORC_GEOM = {"triangles": 0, "quads": 1, "spheres": 2}
 ORC_MAT = {"obj": 0, "metal": 1, "glass": 2, "luminous": 3}
@@ -132,6 +134,12 @@ def lib():
         L.orc_hg_sample.argtypes = [C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_int64]
         L.orc_hg_eval.argtypes = [C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_variance_buffers.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_void_p,
+                                           C.c_void_p]
+        L.orc_distribution1d.restype = C.c_float
+        L.orc_distribution1d.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.orc_light_samples.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4 + [C.c_int64, C.c_void_p]
+        L.orc_select_light.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -307,6 +315,26 @@ class OracleScene:
         return tuv, ids, (int(st[0]), int(st[1]))
 
 
+    def light_samples(self, light: int, P, Ng, Ns, s2):
+        """lightSample of light `light` (index into [geometry lights, world
+        lights]) at P: columns dir (3), dist, weight (3), pdf."""
+        s2 = np.ascontiguousarray(s2, np.float32).reshape(-1, 2)
+        vecs = [np.ascontiguousarray(v, np.float32).reshape(3) for v in (P, Ng, Ns)]
+        out = np.empty((s2.shape[0], 8), np.float32)
+        lib().orc_light_samples(self.h, light, vecs[0].ctypes.data, vecs[1].ctypes.data, vecs[2].ctypes.data,
+                                s2.ctypes.data, s2.shape[0], out.ctypes.data)
+        return out
+
+    def select_light(self, ss, geometry_lights: bool = True):
+        """The path tracer's light selection for each value of ss: (index
+        into the light list, selection probability)."""
+        ss = np.ascontiguousarray(ss, np.float32).reshape(-1)
+        idx = np.empty(ss.size, np.int32)
+        prob = np.empty(ss.size, np.float32)
+        lib().orc_select_light(self.h, int(geometry_lights), ss.ctypes.data, ss.size, idx.ctypes.data,
+                               prob.ctypes.data)
+        return idx, prob
+
     # ---- volume hooks ----
     def volume_sample(self, vol: int, pts: np.ndarray) -> np.ndarray:
         p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
@@ -425,3 +453,45 @@ def render_scene(scene, frames: int = 1, fmt: int = 3, channels: int = 1 | 4, nt
     for _ in range(frames):
         fb.render(os_, cam, ren, nthreads, rows)
     return fb, os_, ren
+
+
+def variance_buffers(color4: np.ndarray, variance4: np.ndarray, threshold: float):
+    """Variance_kernel + getAvgError on given (h, w, 4) accumulated colour
+    and variance buffers: (per-task errors (ceil(h/8), ceil(w/8)), frame
+    estimate)."""
+    c = np.ascontiguousarray(color4, np.float32)
+    v = np.ascontiguousarray(variance4, np.float32)
+    h, w = c.shape[:2]
+    tasks = np.empty(((h + 7) // 8, (w + 7) // 8), np.float32)
+    avg = C.c_float()
+    lib().orc_variance_buffers(w, h, c.ctypes.data, v.ctypes.data, threshold, tasks.ctypes.data, C.byref(avg))
+    return tasks, float(avg.value)
+
+
+def distribution1d(weights, s):
+    """Distribution1D_create(weights) then _sample(s): (sum, idx, prob, pdf,
+    frac)."""
+    w = np.ascontiguousarray(weights, np.float32).reshape(-1)
+    s = np.ascontiguousarray(s, np.float32).reshape(-1)
+    idx = np.empty(s.size, np.int32)
+    out = np.empty((s.size, 3), np.float32)
+    total = lib().orc_distribution1d(w.ctypes.data, w.size, s.ctypes.data, s.size, idx.ctypes.data, out.ctypes.data)
+    return float(total), idx, out[:, 0], out[:, 1], out[:, 2]
+
+
+def pixel_filter(renderer: dict, s2):
+    """Pixel-filter offsets of the renderer's filter for sample pairs s2."""
+    s2 = np.ascontiguousarray(s2, np.float32).reshape(-1, 2)
+    out = np.empty_like(s2)
+    r = make_renderer(renderer)
+    lib().orc_pixel_filter(C.byref(r), s2.ctypes.data, out.ctypes.data, s2.shape[0])
+    return out
+
+
+def camera_rays(camera: dict, screen2, near_clip: float = 1e-6):
+    """Camera rays (org (3), dir (3)) for screen positions in [0,1]^2."""
+    sc = np.ascontiguousarray(screen2, np.float32).reshape(-1, 2)
+    out = np.empty((sc.shape[0], 6), np.float32)
+    cam = make_camera(camera, near_clip)
+    lib().orc_camera_rays(C.byref(cam), sc.ctypes.data, out.ctypes.data, sc.shape[0])
+    return out

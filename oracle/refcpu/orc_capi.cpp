@@ -352,4 +352,68 @@ void orc_hg_eval(float g, const float *wo3, const float *wi3, float *outPdf, int
   for (int64_t i = 0; i < n; i++)
     outPdf[i] = hgEvalHook(g, v3(wo3[3 * i], wo3[3 * i + 1], wo3[3 * i + 2]), v3(wi3[3 * i], wi3[3 * i + 1], wi3[3 * i + 2]));
 }
+
+// Variance_kernel + getAvgError on given accumulated colour and variance
+// buffers (RGBA float, w*h each): per-task errors into outTasks
+// (ceil(w/8)*ceil(h/8) floats), the frame estimate into *outAvg
+void orc_variance_buffers(int w, int h, const float *color4, const float *variance4, float threshold, float *outTasks,
+    float *outAvg)
+{
+  FrameBuffer fb;
+  initFrameBuffer(fb, w, h, 0, 1u | 4u | 8u);
+  const size_t n = (size_t)w * h;
+  for (size_t i = 0; i < n; i++) {
+    fb.color[i] = {color4[4 * i], color4[4 * i + 1], color4[4 * i + 2], color4[4 * i + 3]};
+    fb.variance[i] = {variance4[4 * i], variance4[4 * i + 1], variance4[4 * i + 2], variance4[4 * i + 3]};
+  }
+  varianceHook(fb);
+  std::memcpy(outTasks, fb.taskVariance.data(), fb.taskVariance.size() * sizeof(float));
+  RendererData r{};
+  r.errorThreshold = threshold;
+  *outAvg = getVariance(fb, r);
+}
+
+// Distribution1D_create over `weights` (n values), then Distribution1D_sample
+// for each of the m values in s: idx, and (prob, pdf, frac) per sample
+float orc_distribution1d(const float *weights, int n, const float *s, int64_t m, int32_t *outIdx, float *out3)
+{
+  std::vector<float> cdf(weights, weights + n);
+  const float sum = Distribution1D_create(n, cdf.data());
+  for (int64_t i = 0; i < m; i++) {
+    const Sample1D r = Distribution1D_sample(n, cdf.data(), 0, s[i]);
+    outIdx[i] = r.idx;
+    out3[3 * i] = r.prob;
+    out3[3 * i + 1] = r.pdf;
+    out3[3 * i + 2] = r.frac;
+  }
+  return sum;
+}
+
+// lightSample of light `light` of the scene's light list ([geometry lights,
+// world lights]) at point P with normals Ng, Ns for n sample pairs s2:
+// out8 = dir (3), dist, weight (3), pdf per sample
+void orc_light_samples(void *scene, int light, const float *P, const float *Ng, const float *Ns, const float *s2,
+    int64_t n, float *out8)
+{
+  const Scene &sc = ((OrcSceneHandle *)scene)->scene;
+  for (int64_t i = 0; i < n; i++)
+    lightSampleHook(sc, light, v3(P[0], P[1], P[2]), v3(Ng[0], Ng[1], Ng[2]), v3(Ns[0], Ns[1], Ns[2]),
+        v2(s2[2 * i], s2[2 * i + 1]), out8 + 8 * i);
+}
+
+// the path tracer's light selection (NextEventEstimation.ispc:60-63) for
+// each value ss: the light's index in the scene's light list and its
+// selection probability; geometryLights selects the CDF with or without
+// the emissive geometry (PathTracer.ispc:95-110)
+void orc_select_light(void *scene, int geometryLights, const float *ss, int64_t n, int32_t *outIdx, float *outProb)
+{
+  const Scene &sc = ((OrcSceneHandle *)scene)->scene;
+  const std::vector<float> &cdf = geometryLights ? sc.lightsCDF : sc.worldLightsCDF;
+  const int first = geometryLights ? 0 : (int)sc.numGeoLights;
+  for (int64_t i = 0; i < n; i++) {
+    const Sample1D r = Distribution1D_sample((int)cdf.size(), cdf.data(), 0, ss[i]);
+    outIdx[i] = first + r.idx;
+    outProb[i] = r.prob;
+  }
+}
 } // extern "C"

@@ -1397,6 +1397,19 @@ static LightEval lightEval(const LightData &l, const DG &dg, vec3f dir, float mi
   return res;
 }
 
+void lightSampleHook(const Scene &s, int light, vec3f P, vec3f Ng, vec3f Ns, vec2f smp, float out[8])
+{
+  DG dg{};
+  dg.P = P;
+  dg.Ng = Ng;
+  dg.Ns = Ns;
+  dg.material = -1;
+  const LightSample ls = lightSample(s, s.lights[(size_t)light], dg, smp);
+  const float v[8] = {ls.dir.x, ls.dir.y, ls.dir.z, ls.dist, ls.weight.x, ls.weight.y, ls.weight.z, ls.pdf};
+  for (int k = 0; k < 8; k++)
+    out[k] = v[k];
+}
+
 // ---- path tracer -----------------------------------------------------------------
 
 enum VertexType
@@ -2075,6 +2088,8 @@ static void varianceKernel(FrameBuffer &fb)
     }
   fb.varianceValid = true;
 }
+
+void varianceHook(FrameBuffer &fb) { varianceKernel(fb); }
 
 // LiveVarianceFrameOp::getAvgError (fb/frame_ops/Variance.cpp:32-46), with the
 // sum taken in FP64

@@ -170,5 +170,12 @@ void specularHook(vec3f ks, float ns, vec3f wo, vec3f wi, vec2f s, float out[11]
 // pdf, weight, type, and the BSDF's scattering type
 void bsdfHook(const OrcMaterial &m, vec3f wo, vec3f wi, vec2f s, float ss, float out[13]);
 float hgEvalHook(float g, vec3f wo, vec3f wi);
+// Variance_kernel over the frame buffer's colour and variance buffers as
+// they stand (fills taskVariance, marks the estimate valid)
+void varianceHook(FrameBuffer &fb);
+// lightSample of world light `light` (index into the scene's light list)
+// at a surface point P with normals Ng, Ns: out = dir (3), dist, weight (3),
+// pdf
+void lightSampleHook(const Scene &s, int light, vec3f P, vec3f Ng, vec3f Ns, vec2f smp, float out[8]);
 
 } // namespace orc

@@ -1028,7 +1028,8 @@ class HipDevice : public Device
       HIP_CHECK(hipStreamSynchronize(stream));
       sizeGrids();
     }
-    const int budgetMB = getInt("pathBudgetMB", 16384);
+    // defaults to the current budget, so a recommit keeps the tuned value
+    const int budgetMB = getInt("pathBudgetMB", (int)(pathBudgetBytes >> 20));
     pathBudgetBytes = (size_t)std::max(64, budgetMB) << 20;
   }
 
@@ -1474,6 +1475,15 @@ class HipDevice : public Device
       }
   }
 
+  // The batch ends when its last lane joins the device stream: the latest
+  // of the views' end events (each measured from the batch start).
+  static float batchMs(const std::vector<float> &ms)
+  {
+    float t = 0.f;
+    for (float x : ms)
+      t = std::max(t, x < 0.f ? 1e30f : x);
+    return t;
+  }
   // Autotuner of the batch: times the eager batch for each candidate of
   // the resident traversal grid, the shading grid and the path-state
   // budget (which sets the samples per pass, i.e. the wavefront-queue
@@ -1501,7 +1511,7 @@ class HipDevice : public Device
           float t = 1e30f;
           for (int k = 0; k < std::max(1, reps); k++) {
             renderBatch(v, world, 0, ms.data());
-            t = std::min(t, ms.back());
+            t = std::min(t, batchMs(ms));
           }
           if (t < bestMs) {
             bestMs = t;
@@ -1525,7 +1535,7 @@ class HipDevice : public Device
       float t = 1e30f;
       for (int k = 0; k < std::max(1, reps); k++) {
         renderBatch(v, world, 0, ms.data());
-        t = std::min(t, ms.back());
+        t = std::min(t, batchMs(ms));
       }
       if (t < bestMs) {
         bestMs = t;

@@ -780,11 +780,15 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
         stage[wpos + __popcll(mask & ((1ull << laneId()) - 1ull))] = r;
     }
     __syncthreads();
-    if (stageCount > (uint32_t)(kStage - blockDim.x)) { // block-uniform
+    // every wave reads the count before any wave can append again in the
+    // next iteration, so the flush branch below is block-uniform
+    const uint32_t cnt = stageCount;
+    __syncthreads();
+    if (cnt > (uint32_t)(kStage - blockDim.x)) {
       if (threadIdx.x == 0)
-        stageBase = (uint32_t)atomicAdd(&P.counters[kCntShadow], (int)stageCount);
+        stageBase = (uint32_t)atomicAdd(&P.counters[kCntShadow], (int)cnt);
       __syncthreads();
-      for (uint32_t k = threadIdx.x; k < stageCount; k += blockDim.x)
+      for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x)
         P.shadowQueue[stageBase + k] = stage[k];
       __syncthreads();
       if (threadIdx.x == 0)

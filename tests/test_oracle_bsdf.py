@@ -77,9 +77,10 @@ def test_spectral_conductor_resampling():
     F = ((eta - 1) / (eta + 1)) ** 2
     out = _bsdf(m, np.array([[0.0, 0.0, 1.0]]), (0.0, 0.0, 1.0), (0.5, 0.5), 0.5)
     np.testing.assert_allclose(out[0, 8:11], np.clip(_SRGB @ F, 0.0, 1.0), atol=2e-7)
-    # entries only beyond 500 nm and only before 600 nm: the nearest entry
-    # holds outside the table; one entry: a flat spectrum
-    for table, eta in (([[500.0, 1.5, 0.0], [800.0, 1.5, 0.0]], np.full(8, 1.5)),
+    # entries only beyond 500 nm (a sloped table, so holding the first entry
+    # differs from extrapolating it) and only before 600 nm: the nearest
+    # entry holds outside the table; one entry: a flat spectrum
+    for table, eta in (([[500.0, 1.2, 0.0], [800.0, 2.0, 0.0]], 1.2 + 0.8 * np.maximum(_WL - 500.0, 0.0) / 300.0),
                        ([[300.0, 1.1, 0.0], [600.0, 1.4, 0.0]], np.minimum(1.1 + 0.3 * (_WL - 300.0) / 300.0, 1.4)),
                        ([[550.0, 1.7, 0.0]], np.full(8, 1.7))):
         F = ((eta - 1) / (eta + 1)) ** 2
