@@ -10,15 +10,25 @@ API, the scene already committed and resident in HBM.  Prints one JSON line.
   value       whole-job path samples/s (1920*1080*4 per step / wall time)
   roofline    the dominant kernel (closest-hit traversal of the extension or
               the shadow rays) against MI355X's 8 TB/s: algorithmic bytes per
-              launch = 128 B per BVH node visit (4-wide node) + 48 B per
-              primitive test +
+              launch = 64 B per BVH node visit + 48 B per primitive test +
               56 B per ray (32 B ray in, 24 B hit out), with node and
               primitive counts from the device's traversal counters on one
               extra frame, over the kernel's mean launch duration from HIP
               events recorded on the device stream during the timed steps
+              (frac_alg); the measured DRAM bytes per launch (FETCH_SIZE +
+              WRITE_SIZE from the committed rocprofv3 summary) over the same
+              duration (frac_dram); frac = the lower of the two; `valu` =
+              the kernel's VALU issue rate and active-lane fraction from the
+              committed SQ-counter summary
   cpu_baseline the CPU oracle (a C++ restatement of the reference path
-              tracer) on a bounded sample of the same frame's rows, all host
-              threads up to 16
+              tracer, built here with -O3 -march=native) on a bounded sample
+              of the same frame's rows, on every CPU of the job's share, with
+              the host's nproc, model name and a probe for an installed
+              OSPRay/Embree
+  c3          plain vs radiance-cache time to equal RMSE on the same scene:
+              1 spp frames x 256, the cache after bounce 2, 2^16 training
+              paths per frame, against a 4096-spp plain render (--no-c3
+              skips it)
 
 --workload c4 measures the mixed surface + volume scene instead (the C0
 spheres inside a 512^3 fBm density volume, 1280x720, 2 spp, maxPathLength
@@ -40,34 +50,90 @@ sys.path.insert(0, ROOT)
 
 METRIC = "path samples/s at 1080p; plain vs neural-cache time-to-equal-RMSE"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, PRIM_BYTES, RAY_BYTES = 128, 48, 56
+# algorithmic bytes of one traversal: 64 B per node visit and 48 B per
+# primitive test (the declared record sizes of the metric's definition),
+# plus 56 B per ray (32 B ray in, 24 B hit out).  The device's 4-wide node
+# is one 128-byte line; NODE_RECORD_BYTES gives that figure beside it.
+NODE_BYTES, PRIM_BYTES, RAY_BYTES = 64, 48, 56
+NODE_RECORD_BYTES = 128
 SAMPLE_BYTES, CELL_BYTES, VOL_RAY_BYTES = 32, 8, 80
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_pathtracer_1080p.md")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_traversal_pathtracer_1080p.md")
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def host_threads() -> int:
+def host_cpus() -> dict:
+    """The host's CPUs: nproc, the CPUs this process may run on, the CPU
+    share the job is given (OMP_NUM_THREADS when set, which the one-GPU job
+    environment sets to its CPU allotment) and the lscpu model name."""
     try:
-        n = len(os.sched_getaffinity(0))
+        allowed = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        allowed = os.cpu_count() or 1
+    share = allowed
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count() or 1, "allowed": allowed, "threads": max(1, share), "model": model}
+
+
+def reference_probe() -> dict:
+    """Looks for an installed OSPRay 3.x / Embree 4 / rkcommon / OpenVKL on
+    this host (the reference CPU module would be timed against it)."""
+    import glob
+    import subprocess
+
+    found = []
+    try:
+        r = subprocess.run(["ldconfig", "-p"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                           timeout=20)
+        found += [l.strip() for l in r.stdout.splitlines() if re.search(r"ospray|embree4|rkcommon|openvkl", l)]
+    except (OSError, subprocess.SubprocessError):
+        pass
+    for pat in ("/usr/lib*/libospray_module_cpu*", "/opt/*/lib*/libospray*", "/usr/local/lib*/libospray*"):
+        found += glob.glob(pat)
+    return {"ldconfig_and_paths": found, "reference_installed": bool(found)}
+
+
+def build_native_oracle() -> str:
+    """Builds the oracle with -O3 -march=native for this host (the timing
+    build; same arithmetic as the test build) and returns its path."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native"], check=True)
+    return os.path.join(ROOT, "oracle", "liborc_native.so")
 
 
 def cpu_baseline(sc, target_s: float) -> dict:
     """Oracle samples/s on evenly spaced 8-row bands of the frame (about
-    target_s seconds of CPU work)."""
+    target_s seconds of CPU work), on every CPU of the job's share, from the
+    -O3 -march=native build."""
+    t = time.perf_counter()
+    os.environ["ORC_LIB"] = build_native_oracle()
+    log(f"oracle built for this host (-O3 -march=native) in {time.perf_counter() - t:.1f} s")
     from oracle import orc
 
+    cpus = host_cpus()
+    probe = reference_probe()
+    log(f"host CPUs: {cpus}; installed reference: {probe}")
     t = time.perf_counter()
     osc = orc.OracleScene(sc)
     log(f"oracle BVH built in {time.perf_counter() - t:.1f} s")
     cam, ren = orc.make_camera(sc.camera), orc.make_renderer(sc.renderer)
     fb = orc.OracleFrameBuffer(sc.width, sc.height, 3, 1 | 4)
-    nt = host_threads()
+    nt = cpus["threads"]
     nbands = (sc.height + 7) // 8
     spp = sc.renderer["pixelSamples"]
 
@@ -97,7 +163,9 @@ def cpu_baseline(sc, target_s: float) -> dict:
             "sample": f"{len(picks)} 8-row bands ({frames} whole frames + {len(picks) - frames * nbands} bands spread "
                       f"over the frame) of the {sc.width}x{sc.height} {spp}-spp frame: {total_n} path samples in "
                       f"{total_t:.1f} s; "
-                      f"oracle C++ restatement of the reference path tracer, {nt} threads"}
+                      f"oracle C++ restatement of the reference path tracer (CPU restatement, not Intel "
+                      f"ISPC+Embree), -O3 -march=native, {nt} threads",
+            "host": cpus, "reference_probe": probe}
 
 
 def traffic_per_launch(kernel: str):
@@ -112,6 +180,20 @@ def traffic_per_launch(kernel: str):
             m = pat.match(line.strip())
             if m:
                 return int(m.group(1))
+    return None
+
+
+def valu_profile(kernel: str):
+    """VALU issue and lane activity of `kernel` from the committed rocprofv3
+    SQ-counter summary, or None."""
+    if not os.path.exists(VALU_FILE):
+        return None
+    pat = re.compile(r"^\|\s*" + re.escape(kernel) + r"\s*\|.*\|\s*valu=(\S+)\s*\|\s*$")
+    with open(VALU_FILE) as fh:
+        for line in fh:
+            m = pat.match(line.strip())
+            if m:
+                return dict(kv.split(":", 1) for kv in m.group(1).split(","))
     return None
 
 
@@ -179,22 +261,24 @@ def bench_c2(args) -> int:
     return 0
 
 
-def bench_c3(args) -> int:
+def measure_c3(args, width: int, height: int) -> dict:
     """Plain vs radiance-cache rendering of the 1080p scene, 1 spp per frame,
     each accumulated over --c3-frames frames and compared (RMSE over RGB)
     with a plain render of --c3-ref-spp samples per pixel.  Frame times are
     ospRenderFrame + ospWait wall times (the cache's inference and training
     included); framebuffer maps for the RMSE checkpoints are outside them.
-    value = (plain time to its final RMSE) / (cache time to the same RMSE)."""
+    value = (plain time to its final RMSE) / (cache time to the same RMSE).
+    Expects the hip device to be current (with profileKernels set)."""
+    import math
+
     import numpy as np
 
     from ospray_amd import api as A
     from ospray_amd import capi as K
     from ospray_amd import scenes
 
-    w, h = args.width or 1920, args.height or 1080
+    w, h = width, height
     sc = scenes.c1(width=w, height=h, spp=1)
-    A.init("hip", profileKernels=True, **args.dparams)
     name, _ = A.device_info()
     L = K.lib()
 
@@ -213,12 +297,14 @@ def bench_c3(args) -> int:
     nref = max(1, args.c3_ref_spp // ref_spp_frame)
     for i in range(nref):
         frame(sref, fbr)
-        if i % 32 == 31:
-            log(f"reference {16 * (i + 1)} spp")
+        if i % 64 == 63:
+            log(f"c3 reference {ref_spp_frame * (i + 1)} spp")
     ref = A.map_color(fbr, w, h, K.OSP_FB_RGBA32F)[..., :3].astype(np.float64)
-    log(f"reference: {nref * ref_spp_frame} spp in {time.perf_counter() - t:.1f} s")
+    ref_s = time.perf_counter() - t
+    log(f"c3 reference: {nref * ref_spp_frame} spp in {ref_s:.1f} s")
+    del fbr, sref
 
-    checkpoints = sorted({1, 2, 4, 8, 16, 32, 64, 128, 256, args.c3_frames} & set(range(1, args.c3_frames + 1)))
+    checkpoints = sorted({1, 2, 4, 8, 16, 32, 64, 128, 192, 256, args.c3_frames} & set(range(1, args.c3_frames + 1)))
 
     def run(rp, label):
         s = scenes.to_ospray(sc, rp)
@@ -234,13 +320,13 @@ def bench_c3(args) -> int:
             if i in checkpoints:
                 img = A.map_color(fb, w, h, K.OSP_FB_RGBA32F)[..., :3].astype(np.float64)
                 curve.append((i, elapsed, float(np.sqrt(np.mean((img - ref) ** 2)))))
-                log(f"{label}: {i} frames {elapsed:.3f} s rmse {curve[-1][2]:.5f}")
+        log(f"c3 {label}: {args.c3_frames} frames {elapsed:.3f} s rmse {curve[-1][2]:.5f}")
         ks = {k: round(v[1] / args.c3_frames, 3) for k, v in sorted(A.kernel_stats().items())}
         return curve, s, ks
 
     plain, _, kplain = run({}, "plain")
     nrc, snrc, knrc = run({"nrc": True, "nrcBounce": args.c3_bounce, "nrcTrainPaths": args.c3_train_paths,
-                           "nrcTrainSteps": 4}, "cache")
+                           "nrcTrainSteps": args.c3_train_steps}, "cache")
     loss, steps, queries, records = A.nrc_state(snrc.renderer)
     target = plain[-1][2]
 
@@ -251,32 +337,39 @@ def bench_c3(args) -> int:
                 if prev[2] == float("inf"):
                     return c[1]
                 # log-log interpolation between checkpoints (RMSE ~ t^-1/2)
-                import math
                 f = (math.log(prev[2]) - math.log(rmse)) / max(math.log(prev[2]) - math.log(c[2]), 1e-12)
                 return prev[1] * (c[1] / prev[1]) ** f
             prev = c
         return None
 
     tp, tn = time_to(plain, target), time_to(nrc, target)
+    return {"time_to_equal_rmse": round(tp / tn, 3) if tn else None, "unit": "x (plain time / cache time)",
+            "workload": f"C1 scene {w}x{h}, 1 spp per frame x {args.c3_frames} frames, cache after bounce "
+                        f"{args.c3_bounce}, {args.c3_train_paths} training paths per frame in "
+                        f"{args.c3_train_steps} Adam steps (lr 1e-2); reference {nref * ref_spp_frame} spp plain",
+            "device": name, "target_rmse": target,
+            "cache_bias_floor_rmse": nrc[-1][2],
+            "plain": {"rmse": [c[2] for c in plain], "seconds": [round(c[1], 4) for c in plain],
+                      "frames": [c[0] for c in plain], "ms_per_frame": round(plain[-1][1] / args.c3_frames * 1e3, 3),
+                      "kernels_ms_per_frame": kplain},
+            "cache": {"rmse": [c[2] for c in nrc], "seconds": [round(c[1], 4) for c in nrc],
+                      "frames": [c[0] for c in nrc], "ms_per_frame": round(nrc[-1][1] / args.c3_frames * 1e3, 3),
+                      "kernels_ms_per_frame": knrc, "final_training_loss": loss, "adam_steps": steps,
+                      "queries_last_frame": queries, "training_records_last_frame": records},
+            "time_to_target_s": {"plain": tp, "cache": tn}, "reference_seconds": round(ref_s, 2)}
+
+
+def bench_c3(args) -> int:
+    """--workload c3: the plain vs radiance-cache comparison alone."""
+    from ospray_amd import api as A
+
+    A.init("hip", profileKernels=True, **args.dparams)
+    r = measure_c3(args, args.width or 1920, args.height or 1080)
     out = {"metric": "plain vs neural-cache time to equal RMSE (1080p scene, 1 spp frames)",
-           "value": round(tp / tn, 3) if tn else None, "unit": "x (plain time / cache time)",
-           "n_gpus": 1, "steps": args.c3_frames, "warmup": 1,
-           "ms_per_step": round(nrc[-1][1] / args.c3_frames * 1e3, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32 path tracer, bf16 MLP (fp32 accumulate, fp32 Adam)",
-           "data": "synthetic",
-           "config": {"workload": f"C1 scene {w}x{h}, 1 spp per frame x {args.c3_frames} frames, cache after bounce "
-                                  f"{args.c3_bounce}, {args.c3_train_paths} training paths per frame in 4 Adam steps "
-                                  f"(lr 1e-2); reference "
-                                  f"{nref * ref_spp_frame} spp plain", "device": name},
-           "target_rmse": target,
-           "plain": {"rmse": [c[2] for c in plain], "seconds": [round(c[1], 4) for c in plain],
-                     "frames": [c[0] for c in plain], "ms_per_frame": round(plain[-1][1] / args.c3_frames * 1e3, 3),
-                     "kernels_ms_per_frame": kplain},
-           "cache": {"rmse": [c[2] for c in nrc], "seconds": [round(c[1], 4) for c in nrc],
-                     "frames": [c[0] for c in nrc], "ms_per_frame": round(nrc[-1][1] / args.c3_frames * 1e3, 3),
-                     "kernels_ms_per_frame": knrc, "final_training_loss": loss, "adam_steps": steps,
-                     "queries_last_frame": queries, "training_records_last_frame": records},
-           "time_to_target_s": {"plain": tp, "cache": tn}}
+           "value": r["time_to_equal_rmse"], "unit": r["unit"], "n_gpus": 1, "steps": args.c3_frames, "warmup": 1,
+           "ms_per_step": r["cache"]["ms_per_frame"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 path tracer, bf16 MLP (fp32 accumulate, fp32 Adam)",
+           "data": "synthetic", "config": {"workload": r["workload"], "device": r["device"]}, **r}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -293,8 +386,12 @@ def main() -> int:
                          "512^3 volume")
     ap.add_argument("--c3-frames", type=int, default=256, help="c3: 1-spp frames per method")
     ap.add_argument("--c3-ref-spp", type=int, default=4096, help="c3: samples per pixel of the plain reference")
-    ap.add_argument("--c3-bounce", type=int, default=1, help="c3: depth of the cache vertex (renderer nrcBounce)")
-    ap.add_argument("--c3-train-paths", type=int, default=8192, help="c3: training paths per frame (nrcTrainPaths)")
+    ap.add_argument("--c3-bounce", type=int, default=2, help="c3: depth of the cache vertex (renderer nrcBounce)")
+    ap.add_argument("--c3-train-paths", type=int, default=65536,
+                    help="c3: training paths per frame (nrcTrainPaths)")
+    ap.add_argument("--c3-train-steps", type=int, default=4, help="c3: Adam steps per frame (nrcTrainSteps)")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="c1: skip the plain vs radiance-cache leg (time to equal RMSE) of the default run")
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)
@@ -400,11 +497,13 @@ def main() -> int:
         if not launches:
             continue
         per_frame_bytes = NODE_BYTES * nodes + PRIM_BYTES * prims + RAY_BYTES * rays
+        record_bytes = NODE_RECORD_BYTES * nodes + PRIM_BYTES * prims + RAY_BYTES * rays
         ms_frame = ms / args.steps
         kern[k] = {"ms_per_frame": ms_frame, "launches_per_frame": launches / args.steps,
                    "bytes_per_frame": per_frame_bytes, "rays_per_frame": rays,
                    "nodes_per_ray": nodes / max(rays, 1), "prims_per_ray": prims / max(rays, 1),
-                   "gbs": per_frame_bytes / (ms_frame * 1e-3) / 1e9}
+                   "gbs": per_frame_bytes / (ms_frame * 1e-3) / 1e9,
+                   "record_gbs": record_bytes / (ms_frame * 1e-3) / 1e9}
     if kstats.get("volume", (0, 0.0))[0]:
         launches, ms = kstats["volume"]
         samples_v, cells_v, tracks_v = vst[:3]
@@ -419,10 +518,24 @@ def main() -> int:
     d = kern[dom]
     lpf = d["launches_per_frame"]
     traffic = traffic_per_launch(f"k_{dom}")
+    launch_ms = d["ms_per_frame"] / lpf
+    frac_alg = d["gbs"] / HBM_PEAK_GBS
+    # measured DRAM bytes per launch (committed counter summary) over the
+    # live launch time; the HBM fraction stated is the lower of the two
+    frac_dram = traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None
     roofline = {"bound": "hbm", "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_{dom}",
+                "frac": round(min(frac_alg, frac_dram) if frac_dram is not None else frac_alg, 4),
+                "traffic": traffic, "kernel": f"k_{dom}",
+                "frac_alg": round(frac_alg, 4),
+                "frac_dram": round(frac_dram, 4) if frac_dram is not None else None,
+                "alg_bytes": f"{NODE_BYTES} B per node visit + {PRIM_BYTES} B per primitive test + {RAY_BYTES} B "
+                             f"per ray",
+                "frac_alg_128B_node_record": round(d["record_gbs"] / HBM_PEAK_GBS, 4),
                 "bytes_per_launch": int(d["bytes_per_frame"] / lpf),
-                "launch_ms": round(d["ms_per_frame"] / lpf, 4)}
+                "launch_ms": round(launch_ms, 4)}
+    valu = valu_profile(f"k_{dom}")
+    if valu:
+        roofline["valu"] = valu
     rays_total = sum(kern[k]["rays_per_frame"] for k in kern)
     copy_ms = A.copy_bandwidth(1 << 30, 10)
     roofline["copy_kernel_gbs"] = round(2 * (1 << 30) / (copy_ms * 1e-3) / 1e9, 1)
@@ -453,6 +566,9 @@ def main() -> int:
                              "rays_per_frame": v["rays_per_frame"], "gbs": round(v["gbs"], 1)}
                          for k, v in kern.items() if k != "volume"},
            "roofline": roofline, "cpu_baseline": cpu}
+    if args.workload == "c1" and not args.no_c3:
+        out["c3"] = measure_c3(args, 1920, 1080)
+        log(f"c3: time to equal RMSE {out['c3']['time_to_equal_rmse']}x")
     if "volume" in kern:
         v = kern["volume"]
         out["volume"] = {"free_path": {"samples": vst[0], "cells": vst[1], "tracks": vst[2],

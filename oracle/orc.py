@@ -304,14 +304,18 @@ class OracleScene:
             _lib.orc_scene_destroy(self.h)
             self.h = None
 
-    def intersect(self, rays8: np.ndarray, fp64: bool = False):
+    def intersect(self, rays8: np.ndarray, fp64: bool = False, float_order: bool = False):
+        """Closest hits: the extension-ray traversal (near-ties re-resolved
+        in FP64); fp64: the FP64 brute force; float_order: the traversal
+        with (float t, key) order only."""
         L = lib()
         r = np.ascontiguousarray(rays8, np.float32)
         n = r.shape[0]
         tuv = np.empty((n, 3), np.float32)
         ids = np.empty((n, 3), np.uint32)
         st = (C.c_uint64 * 2)()
-        L.orc_intersect(self.h, r.ctypes.data, n, int(fp64), tuv.ctypes.data, ids.ctypes.data, st)
+        L.orc_intersect(self.h, r.ctypes.data, n, 1 if fp64 else (2 if float_order else 0), tuv.ctypes.data,
+                        ids.ctypes.data, st)
         return tuv, ids, (int(st[0]), int(st[1]))
 
 

@@ -227,8 +227,10 @@ void orc_camera_rays(const OrcCamera *cam, const float *screen2, float *out6, in
 }
 
 // closest hits for rays given as (org3, dir3, t0, t1); out: t,u,v (floats)
-// and inst, model, prim (uint32, 0xffffffff on miss).  fp64 != 0 uses the
-// double-precision brute force instead of the BVH.
+// and inst, model, prim (uint32, 0xffffffff on miss).  fp64: 0 the BVH
+// traversal of extension rays (near-ties re-resolved in FP64), 1 the
+// double-precision brute force, 2 the BVH traversal in float order only
+// (the shadow rays' closest hits).
 void orc_intersect(void *scene, const float *rays8, int64_t n, int fp64, float *outTUV, uint32_t *outIDs, uint64_t *stats2)
 {
   const World &w = ((OrcSceneHandle *)scene)->scene.world;
@@ -236,7 +238,7 @@ void orc_intersect(void *scene, const float *rays8, int64_t n, int fp64, float *
   for (int64_t i = 0; i < n; i++) {
     const float *r = rays8 + 8 * i;
     Ray ray{v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7]};
-    const Hit h = fp64 ? intersectBruteForceFP64(w, ray) : w.intersect(ray, &nodes, &prims);
+    const Hit h = fp64 == 1 ? intersectBruteForceFP64(w, ray) : w.intersect(ray, &nodes, &prims, fp64 == 0);
     outTUV[3 * i] = h.hit ? h.t : kInf;
     outTUV[3 * i + 1] = h.u;
     outTUV[3 * i + 2] = h.v;

@@ -262,7 +262,46 @@ static inline float boxEntry(const BoxRay &br, const Box &b, float t0, float tbe
   return tn;
 }
 
-Hit World::intersect(const Ray &rayIn, uint64_t *nodeVisits, uint64_t *primTests) const
+// float hit of BLAS item `it` of instance `inst` for the object ray r
+// (t in (r.t0, tmax]): t, u, v, Ng and, for a sphere, which root
+static bool itemHit(const OrcScene &desc, const BLASItem &it, const OrcGeometry &g, const Ray &r, const TriPrecomp &pc,
+    float tmax, float &t, float &u, float &v, vec3f &Ng, int &exitRoot)
+{
+  u = v = 0.f;
+  exitRoot = 0;
+  if (g.type == ORC_GEOM_SPHERES) {
+    const float *sp4 = g.spheres + 4 * (int64_t)it.prim;
+    const bool h = intersectSphere(r, v3(sp4[0], sp4[1], sp4[2]), sp4[3], tmax, t, Ng);
+    if (h) {
+      // intersectSphere takes the entry root when it is in range: recompute
+      // the entry root with the same arithmetic to tell which one it was
+      const vec3f d = r.dir;
+      const float rd2 = 1.0f / dot(d, d);
+      const vec3f CO = v3(sp4[0], sp4[1], sp4[2]) - r.org;
+      const float projCO = dot(CO, d) * rd2;
+      const vec3f perp = CO - projCO * d;
+      const float l2 = dot(perp, perp);
+      const float r2 = sp4[3] * sp4[3];
+      const float td = std::sqrt((r2 - l2) * rd2);
+      const float tE = projCO - td;
+      exitRoot = (tE > r.t0 && tE <= tmax) ? 0 : 1;
+    }
+    return h;
+  }
+  vec3f a, b, c;
+  itemTriangle(desc, g, it, a, b, c);
+  const bool h = intersectTriangle(r, pc, a, b, c, tmax, t, u, v);
+  if (h) {
+    Ng = cross(b - a, c - a);
+    if (it.half) {
+      u = 1.0f - u;
+      v = 1.0f - v;
+    }
+  }
+  return h;
+}
+
+Hit World::intersect(const Ray &rayIn, uint64_t *nodeVisits, uint64_t *primTests, bool resolveTies) const
 {
   Hit best;
   best.hit = false;
@@ -273,6 +312,10 @@ Hit World::intersect(const Ray &rayIn, uint64_t *nodeVisits, uint64_t *primTests
   best.Ng = v3(0.f);
   if (tlas.nodes.empty() || desc.numInstances == 0)
     return best;
+  bool tie = false; // a near-tie of the current closest hit has been seen
+  // with resolveTies, boxes and primitives up to a near-tie beyond the
+  // closest hit stay open
+  auto tLimit = [&]() { return resolveTies && best.hit ? minf(best.t + kTieEps * best.t, rayIn.t) : best.t; };
   const BoxRay wbr = makeBoxRay(rayIn);
   uint32_t tstack[64];
   int tsp = 0;
@@ -281,7 +324,7 @@ Hit World::intersect(const Ray &rayIn, uint64_t *nodeVisits, uint64_t *primTests
     const BVHNode &n = tlas.nodes[tstack[--tsp]];
     if (nodeVisits)
       (*nodeVisits)++;
-    if (boxEntry(wbr, n.box, rayIn.t0, best.t) == kInf)
+    if (boxEntry(wbr, n.box, rayIn.t0, tLimit()) == kInf)
       continue;
     if (n.left >= 0) {
       tstack[tsp++] = (uint32_t)n.right;
@@ -306,7 +349,7 @@ Hit World::intersect(const Ray &rayIn, uint64_t *nodeVisits, uint64_t *primTests
         const BVHNode &bn = ga.bvh.nodes[stack[--sp]];
         if (nodeVisits)
           (*nodeVisits)++;
-        if (boxEntry(br, bn.box, r.t0, best.t) == kInf)
+        if (boxEntry(br, bn.box, r.t0, tLimit()) == kInf)
           continue;
         if (bn.left >= 0) {
           stack[sp++] = (uint32_t)bn.right;
@@ -319,27 +362,15 @@ Hit World::intersect(const Ray &rayIn, uint64_t *nodeVisits, uint64_t *primTests
           const OrcGeometry &g = desc.geometries[m.geometry];
           if (primTests)
             (*primTests)++;
-          float t, u = 0.f, v = 0.f;
+          float t, u, v;
           vec3f Ng;
-          bool h;
-          if (g.type == ORC_GEOM_SPHERES) {
-            const float *sp4 = g.spheres + 4 * (int64_t)it.prim;
-            h = intersectSphere(r, v3(sp4[0], sp4[1], sp4[2]), sp4[3], best.t, t, Ng);
-          } else {
-            vec3f a, b, c;
-            itemTriangle(desc, g, it, a, b, c);
-            h = intersectTriangle(r, pc, a, b, c, best.t, t, u, v);
-            if (h) {
-              Ng = cross(b - a, c - a);
-              if (it.half) {
-                u = 1.0f - u;
-                v = 1.0f - v;
-              }
-            }
-          }
-          if (!h)
+          int exitRoot;
+          if (!itemHit(desc, it, g, r, pc, tLimit(), t, u, v, Ng, exitRoot))
             continue;
+          const bool nt = resolveTies && best.hit && nearTie(t, best.t);
           if (better(t, inst, it.model, it.prim, it.half, best)) {
+            // a hit below the window of the previous closest one clears the flag
+            tie = nt;
             best.hit = true;
             best.t = t;
             best.u = u;
@@ -349,12 +380,100 @@ Hit World::intersect(const Ray &rayIn, uint64_t *nodeVisits, uint64_t *primTests
             best.prim = it.prim;
             best.half = it.half;
             best.Ng = Ng;
+          } else if (nt) {
+            tie = true;
           }
         }
       }
     }
   }
-  return best;
+  if (!tie)
+    return best;
+  // FP64 re-resolution over the window around the closest hit: the
+  // smallest (FP64 t, key) among every hit whose float t is a near-tie
+  const float tF = best.t;
+  const float tHi = minf(tF + kTieEps * tF, rayIn.t);
+  Hit w = best;
+  double wD = 0.0;
+  bool have = false;
+  tsp = 0;
+  tstack[tsp++] = 0;
+  while (tsp) {
+    const BVHNode &n = tlas.nodes[tstack[--tsp]];
+    if (nodeVisits)
+      (*nodeVisits)++;
+    if (boxEntry(wbr, n.box, rayIn.t0, tHi) == kInf)
+      continue;
+    if (n.left >= 0) {
+      tstack[tsp++] = (uint32_t)n.right;
+      tstack[tsp++] = (uint32_t)n.left;
+      continue;
+    }
+    for (uint32_t k = 0; k < n.count; k++) {
+      const uint32_t inst = tlas.items[n.first + k];
+      const GroupAccel &ga = groups[desc.instances[inst].group];
+      const affine3f &rx = rcpXfm[inst];
+      Ray r;
+      r.org = xfmPoint(rx, rayIn.org);
+      r.dir = xfmVector(rx, rayIn.dir);
+      r.t0 = rayIn.t0;
+      r.t = rayIn.t;
+      const BoxRay br = makeBoxRay(r);
+      const TriPrecomp pc = triPrecompute(r.dir);
+      double od[3], dd[3];
+      fp64ObjectRay(rx, rayIn.org, rayIn.dir, od, dd);
+      uint32_t stack[128];
+      int sp = 0;
+      stack[sp++] = 0;
+      while (sp) {
+        const BVHNode &bn = ga.bvh.nodes[stack[--sp]];
+        if (nodeVisits)
+          (*nodeVisits)++;
+        if (boxEntry(br, bn.box, r.t0, tHi) == kInf)
+          continue;
+        if (bn.left >= 0) {
+          stack[sp++] = (uint32_t)bn.right;
+          stack[sp++] = (uint32_t)bn.left;
+          continue;
+        }
+        for (uint32_t j = 0; j < bn.count; j++) {
+          const BLASItem &it = ga.items[ga.bvh.items[bn.first + j]];
+          const OrcModel &m = desc.models[desc.groupModels[desc.groups[desc.instances[inst].group].firstModel + it.model]];
+          const OrcGeometry &g = desc.geometries[m.geometry];
+          if (primTests)
+            (*primTests)++;
+          float t, u, v;
+          vec3f Ng;
+          int exitRoot;
+          if (!itemHit(desc, it, g, r, pc, tHi, t, u, v, Ng, exitRoot) || !nearTie(t, tF))
+            continue;
+          double tD;
+          if (g.type == ORC_GEOM_SPHERES) {
+            const float *sp4 = g.spheres + 4 * (int64_t)it.prim;
+            tD = fp64SphereT(od, dd, v3(sp4[0], sp4[1], sp4[2]), sp4[3], exitRoot);
+          } else {
+            vec3f a, b, c;
+            itemTriangle(desc, g, it, a, b, c);
+            tD = fp64TriangleT(od, dd, a, b, c);
+          }
+          if (!have || tD < wD || (tD == wD && keyLess(inst, it.model, it.prim, it.half, w))) {
+            have = true;
+            wD = tD;
+            w.hit = true;
+            w.t = t;
+            w.u = u;
+            w.v = v;
+            w.inst = inst;
+            w.model = it.model;
+            w.prim = it.prim;
+            w.half = it.half;
+            w.Ng = Ng;
+          }
+        }
+      }
+    }
+  }
+  return have ? w : best;
 }
 
 // double-precision closest hit by brute force (no BVH)

@@ -115,14 +115,12 @@ inline bool intersectTriangle(const Ray &ray,
   const float By = comp(B, pc.ky) - pc.Sy * Bz_;
   const float Cx = comp(C, pc.kx) - pc.Sx * Cz_;
   const float Cy = comp(C, pc.ky) - pc.Sy * Cz_;
-  float U = Cx * By - Cy * Bx;
-  float V = Ax * Cy - Ay * Cx;
-  float W = Bx * Ay - By * Ax;
-  if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
-  }
+  // in float only: an edge function depends on the edge's two vertices
+  // and is exactly antisymmetric between the two triangles sharing it, so
+  // no ray passes between them
+  const float U = Cx * By - Cy * Bx;
+  const float V = Ax * Cy - Ay * Cx;
+  const float W = Bx * Ay - By * Ax;
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
     return false;
   const float det = U + V + W;
@@ -207,6 +205,49 @@ struct GroupAccel
   Box bounds;
 };
 
+// Near-ties of extension rays: hits whose float distances lie within
+// kTieEps (relative) of the closest one are re-resolved by their FP64
+// distances (World::intersect with resolveTies).
+static constexpr float kTieEps = 1.0e-5f;
+inline bool nearTie(float t, float tBest) { return std::fabs(t - tBest) <= kTieEps * tBest; }
+
+// the object ray of a world ray in double: the float transform applied in
+// double precision (o = l * worg + p, d = l * wdir, columns in order)
+inline void fp64ObjectRay(const affine3f &rx, vec3f worg, vec3f wdir, double o[3], double d[3])
+{
+  const float c[12] = {rx.l.vx.x, rx.l.vx.y, rx.l.vx.z, rx.l.vy.x, rx.l.vy.y, rx.l.vy.z,
+      rx.l.vz.x, rx.l.vz.y, rx.l.vz.z, rx.p.x, rx.p.y, rx.p.z};
+  const double wo[3] = {worg.x, worg.y, worg.z}, wd[3] = {wdir.x, wdir.y, wdir.z};
+  for (int r = 0; r < 3; r++) {
+    o[r] = (double)c[r] * wo[0] + (double)c[3 + r] * wo[1] + (double)c[6 + r] * wo[2] + (double)c[9 + r];
+    d[r] = (double)c[r] * wd[0] + (double)c[3 + r] * wd[1] + (double)c[6 + r] * wd[2];
+  }
+}
+
+// FP64 distance to the plane of triangle (a, b, c) along a double ray
+inline double fp64TriangleT(const double o[3], const double d[3], vec3f a, vec3f b, vec3f c)
+{
+  const double e1[3] = {(double)b.x - a.x, (double)b.y - a.y, (double)b.z - a.z};
+  const double e2[3] = {(double)c.x - a.x, (double)c.y - a.y, (double)c.z - a.z};
+  const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  const double ao[3] = {(double)a.x - o[0], (double)a.y - o[1], (double)a.z - o[2]};
+  const double num = ao[0] * n[0] + ao[1] * n[1] + ao[2] * n[2];
+  const double den = d[0] * n[0] + d[1] * n[1] + d[2] * n[2];
+  return num / den;
+}
+
+// FP64 entry (exitRoot 0) or exit (1) distance of a sphere
+inline double fp64SphereT(const double o[3], const double d[3], vec3f center, float radius, int exitRoot)
+{
+  const double oc[3] = {o[0] - center.x, o[1] - center.y, o[2] - center.z};
+  const double A = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const double B = oc[0] * d[0] + oc[1] * d[1] + oc[2] * d[2];
+  const double C = oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2] - (double)radius * (double)radius;
+  const double disc = B * B - A * C;
+  const double sq = disc > 0.0 ? std::sqrt(disc) : 0.0;
+  return (exitRoot ? (-B + sq) : (-B - sq)) / A;
+}
+
 struct World
 {
   OrcScene desc; // shallow copy (arrays owned by the caller)
@@ -216,8 +257,12 @@ struct World
   std::vector<Box> instBounds;
 
   void build(const OrcScene &scene);
-  // closest hit; ray in world space; statistic counters optional
-  Hit intersect(const Ray &ray, uint64_t *nodeVisits = nullptr, uint64_t *primTests = nullptr) const;
+  // closest hit; ray in world space; statistic counters optional.
+  // resolveTies (extension rays): when another hit's float t is a near-tie
+  // of the closest one's, the hits within that window are re-resolved by
+  // their FP64 distances (smallest FP64 t, then key)
+  Hit intersect(const Ray &ray, uint64_t *nodeVisits = nullptr, uint64_t *primTests = nullptr,
+      bool resolveTies = false) const;
 };
 
 inline vec3f ld3(const float *p, int64_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }

@@ -1468,7 +1468,10 @@ static inline float misHeuristic(float pdf1, float pdf2) // PathTracerUtil.ih:30
   return pdf1 > 1e17f ? 1.0f : p;
 }
 
+// closest hit of a shadow-ray segment (TransparentShadow.ispc:36-84)
 static Hit trace(const Scene &s, const Ray &ray) { return s.world.intersect(ray); }
+// closest hit of a camera or extension ray, near-ties re-resolved in FP64
+static Hit traceExtension(const Scene &s, const Ray &ray) { return s.world.intersect(ray, nullptr, nullptr, true); }
 
 // TransparentShadow.ispc:18-85
 // On return shadowRay spans [t0 of the last segment traced, original t]:
@@ -1798,7 +1801,7 @@ static void samplePath(const PathCtx &c, PathState &ps, Ray ray, ScreenSample &s
     pv.numLightSamples = 0;
     pv.volInst = -1;
     const float r0 = ray.t0, r1 = ray.t; // the clipping pass's ray interval
-    const Hit hit = trace(*c.scene, ray);
+    const Hit hit = traceExtension(*c.scene, ray);
     if (hit.hit) {
       pv.type = V_SURFACE;
       ray.t = hit.t;

@@ -398,6 +398,14 @@ def world_info(world: Handle) -> tuple:
     return tuple(int(x) for x in out)
 
 
+def world_build(world: Handle) -> tuple:
+    """(BLAS build milliseconds of the last commit, BLASes built on the
+    device, traversal stack entries the BVH needs)."""
+    ms, ndev, need = C.c_double(0.0), C.c_int(0), C.c_int(0)
+    K.ext().ospHipGetWorldBuild(world.h, C.byref(ms), C.byref(ndev), C.byref(need))
+    return float(ms.value), int(ndev.value), int(need.value)
+
+
 def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
     """Mean ms of one device-to-device copy of nbytes by the plain copy kernel."""
     ms = C.c_float(0.0)

@@ -38,7 +38,7 @@ HIPFLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contra
 
 CORE_SRC = ["core/managed.cpp", "core/api.cpp", "core/util.cpp"]
 HIP_CXX_SRC = ["hip/bvh_build.cpp"]
-HIP_SRC = ["hip/pt_kernels.hip", "hip/hip_device.hip", "hip/nrc.hip"]
+HIP_SRC = ["hip/pt_kernels.hip", "hip/hip_device.hip", "hip/nrc.hip", "hip/bvh_device.hip"]
 
 
 def _run(cmd):

@@ -140,6 +140,7 @@ _EXT_SIGS = {
     "ospHipTraceRays": (C.c_int, [_VP, _VP, C.c_int, _VP, _VP, C.POINTER(_U64)]),
     "ospHipGetDeviceInfo": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
     "ospHipGetWorldInfo": (C.c_int, [_VP, C.POINTER(_I64)]),
+    "ospHipGetWorldBuild": (C.c_int, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ospHipCopyBandwidth": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(_F)]),
 }
 

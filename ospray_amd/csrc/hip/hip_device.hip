@@ -25,6 +25,7 @@
 #include "../core/core.h"
 #include "bvh_build.h"
 #include "ospray/ospray_hip.h"
+#include "ospray_hip/hip_bvh.h"
 #include "ospray_hip/hip_nrc.h"
 #include "ospray_hip/hip_pt.h"
 #include "pt_sampler.h"
@@ -730,14 +731,20 @@ struct Light : public ManagedObject
   }
 };
 
-// BLAS of one list of geometries (shared by groups with the same list)
+// BLAS of one list of geometries (shared by groups with the same list):
+// built on the device (nodes and records in dNodes/dPrims) or, as the
+// fallback, on the host (nodes/prims)
 struct Blas
 {
   std::vector<hpt::DNode4> nodes;
   std::vector<hpt::DPrim> prims;
+  DevBuf dNodes, dPrims;
+  bool onDevice = false;
+  size_t numNodes = 0, numPrims = 0;
   hpt::BuildBox bounds;
   int maxDepth = 0;
   int stackNeed = 0;
+  double buildMs = 0.0;
 };
 
 struct World : public ManagedObject
@@ -753,6 +760,8 @@ struct World : public ManagedObject
   bool built = false;
   int64_t totalNodes = 0, totalPrims = 0;
   int maxDepth = 0, stackNeed = 0;
+  double blasBuildMs = 0.0; // device BLAS builds of the last commit (0 for cached or host builds)
+  int deviceBlas = 0; // BLASes of this world built on the device
   float bounds[6] = {0, 0, 0, 0, 0, 0};
   World(HipDevice *d) : dev(d) { managedType = OSP_WORLD; }
   std::string toString() const override { return "ospray::World"; }
@@ -991,6 +1000,16 @@ class HipDevice : public Device
   int residentTraceBlocks = 0, shadeBlocks = 0;
   size_t pathBudgetBytes = (size_t)16 << 30;
   bool profile = false, travStats = false;
+  // BLAS build: on the device (default) or on the host (device parameter
+  // "hostBvh"); pinned words for the device build's readbacks
+  bool hostBvh = false;
+  uint64_t *bvhWords = nullptr;
+  uint64_t *bvhHostWords()
+  {
+    if (!bvhWords)
+      HIP_CHECK(hipHostMalloc((void **)&bvhWords, 4 * sizeof(uint64_t), hipHostMallocDefault));
+    return bvhWords;
+  }
   bool capturing = false; // enqueuing a batch of views into a HIP graph
   std::vector<KernelTimer> timers;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pendingEvents;
@@ -1005,6 +1024,8 @@ class HipDevice : public Device
       (void)hipEventDestroy(e);
     if (batchStart)
       (void)hipEventDestroy(batchStart);
+    if (bvhWords)
+      (void)hipHostFree(bvhWords);
     for (auto e : laneEvents)
       (void)hipEventDestroy(e);
     for (auto e : passEvents)
@@ -1024,6 +1045,7 @@ class HipDevice : public Device
     shadeBlocksPerCU = std::max(1, std::min(64, getInt("shadeBlocksPerCU", shadeBlocksPerCU)));
     batchStreams = std::max(1, std::min(8, getInt("batchStreams", batchStreams)));
     frameLanes = std::max(1, std::min(8, getInt("frameLanes", frameLanes)));
+    hostBvh = getBool("hostBvh", hostBvh);
     if (stream) {
       HIP_CHECK(hipStreamSynchronize(stream));
       sizeGrids();
@@ -1653,8 +1675,51 @@ static hpt::BuildBox primBox(const Geometry &g, size_t p, int half)
   return b;
 }
 
-static std::shared_ptr<Blas> buildBlas(const std::vector<Geometry *> &geoms)
+// Device build (hip_bvh.h) of the BLAS of `geoms`; null when the list is
+// too small for it or the tree needs more traversal stack than the device
+// build may use (the host build takes over).
+constexpr int kDeviceBlasStackLimit = 96;
+static std::shared_ptr<Blas> buildBlasDevice(HipDevice &D, const std::vector<Geometry *> &geoms)
 {
+  std::vector<HptBvhGeometry> hg;
+  for (auto *g : geoms)
+    hg.push_back({g->dVertex.as<const float>(), g->dIndex.as<const uint32_t>(), g->dSpheres.as<const float>(),
+        g->geomType, (uint32_t)g->numPrims});
+  const uint64_t n = hpt_bvh_items(hg.data(), (int)hg.size());
+  if (n < 3 || n >= (1ull << 28) || geoms.size() > 4096)
+    return nullptr;
+  D.gpu();
+  const auto t0 = std::chrono::steady_clock::now();
+  auto B = std::make_shared<Blas>();
+  DevBuf ws, nodes;
+  ws.ensure(hpt_bvh_workspace_bytes(n));
+  nodes.ensure(n * sizeof(hpt::DNode4));
+  B->dPrims.ensure(n * sizeof(hpt::DPrim));
+  HptBvhResult r;
+  HIP_CHECK(hpt_bvh_build(hg.data(), (int)hg.size(), ws.ptr, ws.bytes, nodes.ptr, B->dPrims.ptr, D.bvhHostWords(),
+      &r, D.stream));
+  if (r.stackNeed > kDeviceBlasStackLimit)
+    return nullptr;
+  B->dNodes.ensure((size_t)r.numNodes * sizeof(hpt::DNode4));
+  HIP_CHECK(hipMemcpyAsync(B->dNodes.ptr, nodes.ptr, (size_t)r.numNodes * sizeof(hpt::DNode4), hipMemcpyDeviceToDevice,
+      D.stream));
+  HIP_CHECK(hipStreamSynchronize(D.stream));
+  B->onDevice = true;
+  B->numNodes = r.numNodes;
+  B->numPrims = n;
+  for (int k = 0; k < 3; k++) {
+    B->bounds.lo[k] = r.lo[k];
+    B->bounds.hi[k] = r.hi[k];
+  }
+  B->maxDepth = r.levels;
+  B->stackNeed = r.stackNeed;
+  B->buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return B;
+}
+
+static std::shared_ptr<Blas> buildBlasHost(const std::vector<Geometry *> &geoms)
+{
+  const auto t0 = std::chrono::steady_clock::now();
   struct Item
   {
     uint32_t model, prim, half;
@@ -1676,6 +1741,8 @@ static std::shared_ptr<Blas> buildBlas(const std::vector<Geometry *> &geoms)
   hpt::BuildResult R = hpt::buildBVH(boxes, 2, std::min(nthreads, 16));
   auto B = std::make_shared<Blas>();
   B->nodes = std::move(R.nodes);
+  B->numNodes = B->nodes.size();
+  B->numPrims = R.order.size();
   B->bounds = R.bounds;
   B->maxDepth = R.maxDepth;
   B->stackNeed = R.stackNeed;
@@ -1715,6 +1782,7 @@ static std::shared_ptr<Blas> buildBlas(const std::vector<Geometry *> &geoms)
   }
   if (geoms.size() >= (1u << 28))
     throw std::runtime_error("too many geometric models in one group");
+  B->buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return B;
 }
 
@@ -1761,8 +1829,10 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
       key.push_back({m->geom.p, m->geom->version});
     }
     auto it = D.blasCache.find(key);
-    if (it == D.blasCache.end())
-      it = D.blasCache.emplace(key, buildBlas(geoms)).first;
+    if (it == D.blasCache.end()) {
+      std::shared_ptr<Blas> b = D.hostBvh ? nullptr : buildBlasDevice(D, geoms);
+      it = D.blasCache.emplace(key, b ? b : buildBlasHost(geoms)).first;
+    }
     groupBlas[gi] = it->second;
     if (!blasSlot.count(it->second.get())) {
       blasSlot[it->second.get()] = (int)uniqueBlas.size();
@@ -1833,8 +1903,8 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   for (size_t k = 0; k < uniqueBlas.size(); k++) {
     blasNodeOff[k] = (int)nNodes;
     blasPrimOff[k] = (int)nPrims;
-    nNodes += uniqueBlas[k]->nodes.size();
-    nPrims += uniqueBlas[k]->prims.size();
+    nNodes += uniqueBlas[k]->numNodes;
+    nPrims += uniqueBlas[k]->numPrims;
     maxDepth = std::max(maxDepth, uniqueBlas[k]->maxDepth);
     blasStackNeed = std::max(blasStackNeed, uniqueBlas[k]->stackNeed);
   }
@@ -1893,8 +1963,6 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   for (size_t k = 0; k < T.order.size(); k++)
     tlasOrder[k] = tlasInst[T.order[k]];
   const int tlasOff = (int)nNodes;
-  std::vector<hpt::DNode4> nodes(nNodes + T.nodes.size());
-  std::vector<hpt::DPrim> prims(nPrims);
   auto remap = [](hpt::DNode4 n, int nodeOff, int primOff, const std::vector<uint32_t> *order) {
     for (int c = 0; c < 4; c++) {
       // an unused slot (box at +inf) is never entered: leave it alone
@@ -1913,14 +1981,41 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
     }
     return n;
   };
+  // the world's node array: every BLAS (relocated on the device for device
+  // builds, on the host otherwise), then the TLAS; primitive records of
+  // every BLAS in the same order
+  const size_t totalNodeCount = nNodes + T.nodes.size();
+  D.gpu();
+  HIP_CHECK(hipStreamSynchronize(D.stream)); // frames in flight read the old arrays
+  dNodes.ensure(std::max<size_t>(sizeof(hpt::DNode4) * totalNodeCount, 16));
+  dPrims.ensure(std::max<size_t>(sizeof(hpt::DPrim) * nPrims, 16));
+  hpt::DNode4 *gNodes = dNodes.as<hpt::DNode4>();
+  hpt::DPrim *gPrims = dPrims.as<hpt::DPrim>();
   for (size_t k = 0; k < uniqueBlas.size(); k++) {
     const Blas &B = *uniqueBlas[k];
-    for (size_t i = 0; i < B.nodes.size(); i++)
-      nodes[blasNodeOff[k] + i] = remap(B.nodes[i], blasNodeOff[k], blasPrimOff[k], nullptr);
-    std::copy(B.prims.begin(), B.prims.end(), prims.begin() + blasPrimOff[k]);
+    if (B.onDevice) {
+      HIP_CHECK(hpt_bvh_relocate(gNodes + blasNodeOff[k], B.dNodes.ptr, B.numNodes, blasNodeOff[k],
+          (uint32_t)blasPrimOff[k], D.stream));
+      HIP_CHECK(hipMemcpyAsync(gPrims + blasPrimOff[k], B.dPrims.ptr, sizeof(hpt::DPrim) * B.numPrims,
+          hipMemcpyDeviceToDevice, D.stream));
+    } else {
+      std::vector<hpt::DNode4> moved(B.nodes.size());
+      for (size_t i = 0; i < B.nodes.size(); i++)
+        moved[i] = remap(B.nodes[i], blasNodeOff[k], blasPrimOff[k], nullptr);
+      HIP_CHECK(hipMemcpy(gNodes + blasNodeOff[k], moved.data(), sizeof(hpt::DNode4) * moved.size(),
+          hipMemcpyHostToDevice));
+      if (!B.prims.empty())
+        HIP_CHECK(hipMemcpy(gPrims + blasPrimOff[k], B.prims.data(), sizeof(hpt::DPrim) * B.prims.size(),
+            hipMemcpyHostToDevice));
+    }
   }
-  for (size_t i = 0; i < T.nodes.size(); i++)
-    nodes[tlasOff + i] = remap(T.nodes[i], tlasOff, 0, &tlasOrder);
+  {
+    std::vector<hpt::DNode4> tl(T.nodes.size());
+    for (size_t i = 0; i < T.nodes.size(); i++)
+      tl[i] = remap(T.nodes[i], tlasOff, 0, &tlasOrder);
+    HIP_CHECK(hipMemcpy(gNodes + tlasOff, tl.data(), sizeof(hpt::DNode4) * tl.size(), hipMemcpyHostToDevice));
+  }
+  HIP_CHECK(hipStreamSynchronize(D.stream));
 
   // lights (PathTracerData.cpp:32-82): [geometry lights, world lights],
   // uniform selection CDFs with and without the geometry lights.  Geometry
@@ -1992,8 +2087,6 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   if (instances.size() >= hpt::kVolumeHit || volInsts.size() >= hpt::kVolumeHit)
     throw std::runtime_error("too many instances for the hip device");
 
-  dNodes.upload(nodes);
-  dPrims.upload(prims);
   dInstances.upload(dinst);
   dModels.upload(models);
   dMaterials.upload(dmats);
@@ -2020,8 +2113,14 @@ void World::commit() // World.cpp:70-188 plus the Group BLAS builds (Group.cpp:9
   scene.volumes = dVolumes.as<const hpt::DVolume>();
   scene.volInsts = dVolInsts.as<const hpt::DVolInst>();
   scene.numVolInsts = (int32_t)volInsts.size();
-  totalNodes = (int64_t)nodes.size();
-  totalPrims = (int64_t)prims.size();
+  totalNodes = (int64_t)totalNodeCount;
+  totalPrims = (int64_t)nPrims;
+  blasBuildMs = 0.0;
+  deviceBlas = 0;
+  for (auto &b : uniqueBlas) {
+    blasBuildMs += b->buildMs;
+    deviceBlas += b->onDevice ? 1 : 0;
+  }
   for (int k = 0; k < 3; k++) {
     bounds[k] = T.bounds.lo[k];
     bounds[3 + k] = T.bounds.hi[k];
@@ -2750,6 +2849,21 @@ OSPRAY_INTERFACE int ospHipCopyBandwidth(size_t bytes, int reps, float *msPerCop
   if (dst)
     (void)hipFree(dst);
   return rc;
+}
+
+OSPRAY_INTERFACE int ospHipGetWorldBuild(OSPWorld world, double *blasBuildMs, int *deviceBlas, int *stackNeed)
+{
+  HipDevice *d = current();
+  if (!d || !world)
+    return -1;
+  World *w = (World *)world;
+  if (blasBuildMs)
+    *blasBuildMs = w->blasBuildMs;
+  if (deviceBlas)
+    *deviceBlas = w->deviceBlas;
+  if (stackNeed)
+    *stackNeed = w->stackNeed;
+  return 0;
 }
 
 OSPRAY_INTERFACE int ospHipGetWorldInfo(OSPWorld world, int64_t out4[4])

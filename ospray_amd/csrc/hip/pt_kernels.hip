@@ -306,7 +306,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
       const float4 d = ld4(P.dir + 4 * (size_t)p);
       Hit h;
       bool sawT;
-      traceRay<MODE_CLOSEST, STATS>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds, ovf, sawT, st);
+      traceRay<MODE_CLOSEST, STATS, false, true>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds, ovf, sawT,
+          st);
       st4(P.hitA + 4 * (size_t)p, h.hit ? h.t : kInf, h.u, h.v, __uint_as_float(h.hit ? h.inst : kInvalidID));
       st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
       P.hitPrim[p] = h.prim | (h.half << 31);
@@ -1247,7 +1248,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DScene S, const floa
     const float *r = rays8 + 8 * (size_t)i;
     Hit h;
     bool sawT;
-    traceRay<MODE_CLOSEST, true>(S, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7], h, lds, ovf, sawT, st);
+    traceRay<MODE_CLOSEST, true, false, true>(S, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7], h, lds, ovf,
+        sawT, st);
     outTUV[3 * (size_t)i] = h.hit ? h.t : kInf;
     outTUV[3 * (size_t)i + 1] = h.u;
     outTUV[3 * (size_t)i + 2] = h.v;

@@ -11,12 +11,24 @@
 //   * the traversal stack lives in LDS (LDS_DEPTH entries per lane, laid out
 //     [depth][lane] so lanes hit distinct banks) and spills to a per-thread
 //     global overflow row beyond that;
-//   * triangles: watertight test (Woop, Benthin, Wald 2013) with an FP64
-//     fallback for zero edge functions; quads as two triangles (v0,v1,v3) and
+//   * triangles: watertight test (Woop, Benthin, Wald 2013) in float: an
+//     edge function depends on the edge's two vertices only and is exactly
+//     antisymmetric between the two triangles sharing the edge, so a ray
+//     never passes between them; quads as two triangles (v0,v1,v3) and
 //     (v2,v3,v1); spheres: Intersect.ih:29-80;
 //   * closest hit = smallest t in (t0, tfar]; exact ties resolved by the
 //     smallest (instance, model, primitive, half), so results do not depend
 //     on BVH layout;
+//   * near-ties of extension rays (TIES): when another hit's float t lies
+//     within kTieEps (relative) of the closest one, a second traversal over
+//     that window re-resolves the hit in FP64: every hit in the window gets
+//     its distance recomputed in double precision from the world ray
+//     (object ray formed in double from the instance's inverse transform)
+//     and the float vertices, and the smallest (FP64 t, key) wins.  The
+//     flag that triggers it is independent of traversal order: a hit is
+//     compared with the current closest on arrival, hits up to kTieEps
+//     beyond it are still tested, and the flag clears only when a hit
+//     closer than the window's lower end arrives;
 //   * node boxes only cull, conservatively (far plane scaled by 1+4e-7).
 #pragma once
 
@@ -104,14 +116,9 @@ HPT_HD bool intersectTriangle(vec3f org, float t0, const TriPre &pc, vec3f v0, v
   const float By = comp(B, pc.ky) - pc.Sy * Bz_;
   const float Cx = comp(C, pc.kx) - pc.Sx * Cz_;
   const float Cy = comp(C, pc.ky) - pc.Sy * Cz_;
-  float U = Cx * By - Cy * Bx;
-  float V = Ax * Cy - Ay * Cx;
-  float W = Bx * Ay - By * Ax;
-  if (U == 0.0f || V == 0.0f || W == 0.0f) {
-    U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-    V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-    W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
-  }
+  const float U = Cx * By - Cy * Bx;
+  const float V = Ax * Cy - Ay * Cx;
+  const float W = Bx * Ay - By * Ax;
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
     return false;
   const float det = U + V + W;
@@ -182,6 +189,49 @@ struct TraceStats
 {
   uint32_t nodes, prims;
 };
+
+// relative window of float distances within which two extension-ray hits
+// count as a near-tie (64 ulp at 1.0 is 7.6e-6; the float t of the
+// watertight test is good to a few ulp away from grazing incidence)
+constexpr float kTieEps = 1.0e-5f;
+
+HPT_HD bool nearTie(float t, float tBest) { return fabsf(t - tBest) <= kTieEps * tBest; }
+
+// FP64 distance along the world ray (worg + t wdir) to the plane of
+// triangle (a, b, c) of instance rcp (inverse transform, columns vx, vy,
+// vz, p): the object ray is formed in double from the float world ray and
+// the float transform, the plane from the float vertices
+HPT_HD void fp64ObjectRay(const float *rcp, vec3f worg, vec3f wdir, double o[3], double d[3])
+{
+  const double wo[3] = {worg.x, worg.y, worg.z}, wd[3] = {wdir.x, wdir.y, wdir.z};
+  for (int r = 0; r < 3; r++) {
+    o[r] = (double)rcp[r] * wo[0] + (double)rcp[3 + r] * wo[1] + (double)rcp[6 + r] * wo[2] + (double)rcp[9 + r];
+    d[r] = (double)rcp[r] * wd[0] + (double)rcp[3 + r] * wd[1] + (double)rcp[6 + r] * wd[2];
+  }
+}
+
+HPT_HD double fp64TriangleT(const double o[3], const double d[3], vec3f a, vec3f b, vec3f c)
+{
+  const double e1[3] = {(double)b.x - a.x, (double)b.y - a.y, (double)b.z - a.z};
+  const double e2[3] = {(double)c.x - a.x, (double)c.y - a.y, (double)c.z - a.z};
+  const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  const double ao[3] = {(double)a.x - o[0], (double)a.y - o[1], (double)a.z - o[2]};
+  const double num = ao[0] * n[0] + ao[1] * n[1] + ao[2] * n[2];
+  const double den = d[0] * n[0] + d[1] * n[1] + d[2] * n[2];
+  return num / den;
+}
+
+// FP64 entry (exitRoot 0) or exit (1) distance of a sphere
+HPT_HD double fp64SphereT(const double o[3], const double d[3], vec3f center, float radius, int exitRoot)
+{
+  const double oc[3] = {o[0] - center.x, o[1] - center.y, o[2] - center.z};
+  const double A = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const double B = oc[0] * d[0] + oc[1] * d[1] + oc[2] * d[2];
+  const double C = oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2] - (double)radius * (double)radius;
+  const double disc = B * B - A * C;
+  const double sq = disc > 0.0 ? sqrt(disc) : 0.0;
+  return (exitRoot ? (-B + sq) : (-B - sq)) / A;
+}
 
 // one 4-wide node: child boxes as per-axis float4s (lane c = child c)
 struct Node4
@@ -264,10 +314,145 @@ enum TraceMode
   MODE_OCCLUSION = 1
 };
 
-template <int MODE, bool STATS, bool RECORD = false>
+// Second traversal of a ray whose closest hit `best` has a near-tie: every
+// hit whose float t is a near-tie of best.t gets its FP64 distance, and the
+// smallest (FP64 t, key) becomes `best` (with its float hit record).  A
+// plain stack traversal: the result is a minimum over a set, so the visit
+// order does not matter.
+template <bool STATS>
+__device__ void resolveNearTie(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
+    uint32_t *lds, uint32_t *ovf, TraceStats &st)
+{
+  const int lane = threadIdx.x;
+  const float tF = best.t;
+  const float tHi = fminf(tF + kTieEps * tF, tfar);
+  const vec3f winv = v3(safeInv(wdir.x), safeInv(wdir.y), safeInv(wdir.z));
+  Hit w = best;
+  double wD = 0.0;
+  bool have = false;
+  int sp = 0;
+  auto push = [&](int v) {
+    if (sp < kLdsDepth)
+      lds[sp * kTraceBlock + lane] = (uint32_t)v;
+    else
+      ovf[sp - kLdsDepth] = (uint32_t)v;
+    sp++;
+  };
+  auto pop = [&]() -> int {
+    sp--;
+    if (sp < kLdsDepth)
+      return (int)lds[sp * kTraceBlock + lane];
+    return (int)ovf[sp - kLdsDepth];
+  };
+  // children of node `cur` whose boxes the ray enters in (t0, tHi]
+  auto expand = [&](int cur, vec3f o, vec3f inv) -> bool {
+    const Node4 nd = loadNode(S.nodes, cur);
+    if (STATS)
+      st.nodes++;
+    const float tc[4] = {slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tHi),
+        slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tHi),
+        slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tHi),
+        slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, t0, tHi)};
+    const int ch[4] = {nd.child.x, nd.child.y, nd.child.z, nd.child.w};
+    for (int c = 0; c < 4; c++)
+      if (tc[c] != kInf) {
+        if (sp >= kLdsDepth + kOverflowDepth) // cannot happen: the world commit checks the stack need
+          return false;
+        push(ch[c]);
+      }
+    return true;
+  };
+  push(S.tlasRoot);
+  while (sp > 0) {
+    const int cur = pop();
+    if (cur >= 0) {
+      if (!expand(cur, worg, winv))
+        return;
+      continue;
+    }
+    const uint32_t inst = ~(uint32_t)cur >> 3; // world leaf: one instance
+    const DInstance &I = S.instances[inst];
+    const linear3f rl = {{I.rcp[0], I.rcp[1], I.rcp[2]}, {I.rcp[3], I.rcp[4], I.rcp[5]}, {I.rcp[6], I.rcp[7], I.rcp[8]}};
+    const vec3f o = mul(rl, worg) + v3(I.rcp[9], I.rcp[10], I.rcp[11]);
+    const vec3f d = mul(rl, wdir);
+    const vec3f inv = v3(safeInv(d.x), safeInv(d.y), safeInv(d.z));
+    const TriPre pc = triPrecompute(d);
+    double od[3], dd[3];
+    fp64ObjectRay(I.rcp, worg, wdir, od, dd);
+    const int base = sp;
+    push(I.blasRoot);
+    while (sp > base) {
+      const int c = pop();
+      if (c >= 0) {
+        if (!expand(c, o, inv))
+          return;
+        continue;
+      }
+      const uint32_t enc = ~(uint32_t)c;
+      const uint32_t first = enc >> 3, count = (enc & 7u) + 1u;
+      for (uint32_t k = 0; k < count; k++) {
+        const DPrim &P = S.prims[first + k];
+        const float4 pa = reinterpret_cast<const float4 *>(&P)[0];
+        const float4 pb = reinterpret_cast<const float4 *>(&P)[1];
+        const float4 pcv = reinterpret_cast<const float4 *>(&P)[2];
+        const uint32_t primID = __float_as_uint(pa.w);
+        const uint32_t mk = __float_as_uint(pb.w);
+        const uint32_t model = mk & 0x0fffffffu, kind = mk >> 28;
+        if (STATS)
+          st.prims++;
+        float t, u = 0.f, v = 0.f;
+        vec3f Ng;
+        uint32_t half = 0;
+        double tD;
+        if (kind == PRIM_SPHERE) {
+          float tE, tX;
+          vec3f NE, NX;
+          const int roots = sphereRoots(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, tHi, tE, NE, tX, NX);
+          if (!roots)
+            continue;
+          const int exitRoot = (roots & 1) ? 0 : 1;
+          t = exitRoot ? tX : tE;
+          Ng = exitRoot ? NX : NE;
+          if (!nearTie(t, tF))
+            continue;
+          tD = fp64SphereT(od, dd, v3(pa.x, pa.y, pa.z), pb.x, exitRoot);
+        } else {
+          const vec3f A = v3(pa.x, pa.y, pa.z), B = v3(pb.x, pb.y, pb.z), C = v3(pcv.x, pcv.y, pcv.z);
+          if (!intersectTriangle(o, t0, pc, A, B, C, tHi, t, u, v) || !nearTie(t, tF))
+            continue;
+          Ng = cross(B - A, C - A);
+          if (kind == PRIM_QUAD1) {
+            half = 1;
+            u = 1.0f - u;
+            v = 1.0f - v;
+          }
+          tD = fp64TriangleT(od, dd, A, B, C);
+        }
+        if (!have || tD < wD || (tD == wD && keyLess(inst, model, primID, half, w))) {
+          have = true;
+          wD = tD;
+          w.hit = true;
+          w.t = t;
+          w.u = u;
+          w.v = v;
+          w.Ng = Ng;
+          w.inst = inst;
+          w.model = model;
+          w.prim = primID;
+          w.half = half;
+        }
+      }
+    }
+  }
+  if (have)
+    best = w;
+}
+
+template <int MODE, bool STATS, bool RECORD = false, bool TIES = false>
 __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
     uint32_t *lds, uint32_t *ovf, bool &sawTransparent, TraceStats &st)
 {
+  bool tie = false; // a near-tie of the current closest hit has been seen (TIES)
   const int lane = threadIdx.x;
   best.hit = false;
   best.t = tfar;
@@ -311,7 +496,8 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
         const Node4 nd = loadNode(S.nodes, cur);
         if (STATS)
           st.nodes++;
-        const float tb = best.t;
+        // with TIES, boxes up to a near-tie beyond the closest hit stay open
+        const float tb = TIES && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
         float t0c = slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tb);
         float t1c = slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tb);
         float t2c = slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tb);
@@ -410,11 +596,12 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
       vec3f Ng;
       bool h;
       uint32_t half = 0;
+      const float tmaxP = TIES && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
       if (kind == PRIM_SPHERE) {
-        h = intersectSphere(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, best.t, t, Ng);
+        h = intersectSphere(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, tmaxP, t, Ng);
       } else {
         const vec3f A = v3(pa.x, pa.y, pa.z), B = v3(pb.x, pb.y, pb.z), C = v3(pcv.x, pcv.y, pcv.z);
-        h = intersectTriangle(o, t0, pc, A, B, C, best.t, t, u, v);
+        h = intersectTriangle(o, t0, pc, A, B, C, tmaxP, t, u, v);
         if (h) {
           Ng = cross(B - A, C - A);
           if (kind == PRIM_QUAD1) {
@@ -434,7 +621,10 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
           recordHit(ovf, t, (uint32_t)inst, first + k, 0);
         continue;
       }
+      const bool nt = TIES && best.hit && nearTie(t, best.t);
       if (better(t, (uint32_t)inst, model, primID, half, best)) {
+        if (TIES)
+          tie = nt; // a hit below the window of the previous closest one clears the flag
         best.hit = true;
         best.t = t;
         best.u = u;
@@ -444,11 +634,15 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
         best.model = model;
         best.prim = primID;
         best.half = half;
+      } else if (nt) {
+        tie = true;
       }
     }
     if (cur == kDone)
       break;
   }
+  if (TIES && tie)
+    resolveNearTie<STATS>(S, worg, wdir, t0, tfar, best, lds, ovf, st);
   return best.hit;
 }
 

@@ -508,3 +508,28 @@ def to_ospray(scene: Scene, renderer_params: Optional[dict] = None) -> OspScene:
     ren = A.new("renderer", "pathtracer", rp)
     handles = keep + geo_h + mat_h + model_h + vmodel_h + group_h + inst_h + light_h
     return OspScene(world, cam, ren, handles)
+
+def near_tie_layers(width: int = 48, height: int = 48, instanced: bool = False) -> Scene:
+    """Stacked near-coplanar surfaces for the near-tie re-resolution of
+    closest hits: four 20x20 quads at z = 4 + k * 2^-21 (k = 0..3, one ulp
+    of 4.0 apart), the farthest first in model order (so a float exact tie,
+    which goes to the smaller key, picks the wrong layer), and in front of
+    them two concentric spheres of radii 0.5 and 0.5 * (1 + 2^-22) at
+    (0, 0, 3), the larger one first.  instanced: the group under a scale of
+    0.5 and a small translation instead of the identity.  Camera at the
+    origin looking along +z; Lambertian surfaces under an ambient light."""
+    geoms = []
+    for k in (3, 2, 1, 0):
+        z = np.float32(4.0) + np.float32(k) * np.float32(2.0 ** -21)
+        v = np.array([[-10, -10, z], [10, -10, z], [10, 10, z], [-10, 10, z]], np.float32)
+        geoms.append(Geometry("quads", vertex=v, index=np.array([[0, 1, 2, 3]], np.uint32)))
+    r0 = np.float32(0.5)
+    r1 = np.float32(0.5) * (np.float32(1.0) + np.float32(2.0 ** -22))
+    geoms.append(Geometry("spheres", spheres=np.array([[0.0, 0.0, 3.0, r1]], np.float32)))
+    geoms.append(Geometry("spheres", spheres=np.array([[0.0, 0.0, 3.0, r0]], np.float32)))
+    models = [Model(i, i % 2) for i in range(len(geoms))]
+    insts = [Instance(0, xfm=(0.5, 0, 0, 0, 0.5, 0, 0, 0, 0.5, 0.1, -0.05, 0.0))] if instanced else [Instance(0)]
+    mats = [{"type": "obj", "kd": (0.8, 0.3, 0.2)}, {"type": "obj", "kd": (0.2, 0.4, 0.8)}]
+    cam = {"position": (0.0, 0.0, 0.0), "direction": (0.0, 0.0, 1.0), "up": (0.0, 1.0, 0.0), "fovy": 50.0}
+    return Scene("near_tie_layers", width, height, geoms, models, [list(range(len(geoms)))], insts, mats,
+                 [{"type": "ambient", "intensity": 1.0}], cam, {"pixelSamples": 2, "maxPathLength": 3})
