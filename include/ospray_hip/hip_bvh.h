@@ -8,11 +8,11 @@
  * kernels' layout (hpt::DNode4 4-wide nodes, hpt::DPrim records in leaf
  * order; ospray_amd/csrc/hip/pt_types.h).
  *
- * Method: primitive boxes -> 30-bit Morton codes of the box centres in the
- * centre bounds -> stable LSD radix sort -> PLOC agglomerative clustering
- * (Meister and Bittner 2018: every cluster picks the neighbour within a
- * window of the sorted order whose merged box has the smallest surface
- * area; mutual pairs merge) -> binary tree -> 4-wide nodes, each opening
+ * Method: primitive boxes -> 30-bit Morton codes of the box centres on a
+ * cubic grid over the centre bounds -> stable LSD radix sort -> PLOC
+ * agglomerative clustering (Meister and Bittner 2018: every cluster picks
+ * the neighbour within a window of the sorted order whose merged box has
+ * the smallest surface area; mutual pairs merge) -> binary tree -> 4-wide nodes, each opening
  * its child of largest surface area until it holds four, subtrees of at
  * most two primitives becoming leaves.
  *
@@ -64,11 +64,13 @@ HPT_API uint64_t hpt_bvh_items(const HptBvhGeometry *geoms, int numGeoms);
 /* Bytes of workspace hpt_bvh_build needs for numItems items. */
 HPT_API size_t hpt_bvh_workspace_bytes(uint64_t numItems);
 /* geoms: host array of numGeoms entries (copied into the workspace).
+ * radius: the clustering window, 1..32 clusters on either side (16 is the
+ * usual choice; larger windows build better trees more slowly).
  * Requires 3 <= items < 2^28.  outNodes: capacity numItems nodes;
  * outPrims: numItems records.  Node child references are local to
  * outNodes, leaf `first` indices local to outPrims. */
-HPT_API hipError_t hpt_bvh_build(const HptBvhGeometry *geoms, int numGeoms, void *workspace, size_t workspaceBytes,
-    void *outNodes, void *outPrims, uint64_t *hostWords, HptBvhResult *result, hipStream_t s);
+HPT_API hipError_t hpt_bvh_build(const HptBvhGeometry *geoms, int numGeoms, int radius, void *workspace,
+    size_t workspaceBytes, void *outNodes, void *outPrims, uint64_t *hostWords, HptBvhResult *result, hipStream_t s);
 /* dst[i] = src[i] for n nodes, inner child references + nodeOffset, leaf
  * first indices + primOffset (placing a BLAS in the world's arrays). */
 HPT_API hipError_t hpt_bvh_relocate(void *dst, const void *src, size_t n, int32_t nodeOffset, uint32_t primOffset,

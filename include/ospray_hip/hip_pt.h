@@ -86,8 +86,11 @@ HPT_API hipError_t hpt_finish_pass(const HptFrame *F, const HptPaths *P, hipStre
 HPT_API hipError_t hpt_fb_accumulate(const HptFrame *F, hipStream_t s);
 /* Variance_kernel (fb/frame_ops/Variance.ispc:11-59). */
 HPT_API hipError_t hpt_variance(const HptFrame *F, hipStream_t s);
-/* getAvgError (fb/frame_ops/Variance.cpp:32-46), FP64 sum, one float out. */
-HPT_API hipError_t hpt_variance_reduce(const float *taskVariance, int n, float threshold, float *out, hipStream_t s);
+/* getAvgError (fb/frame_ops/Variance.cpp:32-46), FP64 sum, one float out.
+ * frameState (optional): the device-resident frame index of a captured
+ * batch frame; the reduction then runs only when it is odd. */
+HPT_API hipError_t hpt_variance_reduce(const float *taskVariance, int n, float threshold, float *out,
+    const uint32_t *frameState, hipStream_t s);
 /* ColorConversion_kernel (fb/frame_ops/ColorConversion.ispc:11-36);
  * format is an OSPFrameBufferFormat. */
 HPT_API hipError_t hpt_color_convert(const float *color, int npix, int format, void *out, hipStream_t s);

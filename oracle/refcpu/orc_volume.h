@@ -218,44 +218,56 @@ inline void forEachInterval(const VolumeData &v, vec3f o, vec3f d, float t0, flo
   }
 }
 
-// tentative collisions per interval before delta tracking gives up (the
-// interval then counts as passed): only reached when densityScale times the
-// interval length is ~1e4 or more
+// tentative collisions per track before delta tracking gives up (the track
+// then counts as passed): only reached when densityScale times the track
+// length is ~1e4 or more
 static constexpr int kMaxDeltaSteps = 1 << 16;
 
 // delta_tracking (VolumeSampler.ispc:13-92) over [t0, t1] (inside the box):
 // the distance of the first real collision (inf if none), its extinction and
 // albedo
+// The optical depth to the next tentative collision, tau ~ Exp(1), is drawn
+// once per track and after each tentative collision, and carried across the
+// intervals: an interval of majorant mu and length l uses up mu * l of it.
+// The reference draws afresh in every interval; the exponential is
+// memoryless, so the collision distances have the same distribution.
 inline float deltaTracking(const VolumeData &v, vec3f o, vec3f d, float t0, float t1, RandomSampler &rs, float &mu_t,
     vec3f &albedo)
 {
   float result = INFINITY;
+  if (!(t0 < t1))
+    return result;
+  float tau = -flog(1.f - RandomSampler_getFloat(rs));
+  int steps = 0;
   forEachInterval(v, o, d, t0, t1, [&](float ta, float tb, float lo, float hi) {
     float t = ta;
     const float maxOpacity = tfMaxOpacity(v, lo, hi);
     const float mu_max = v.densityScale * maxOpacity;
     if (mu_max <= 0.f)
       return true;
-    for (int k = 0; k < kMaxDeltaSteps; k++) {
-      float xi = RandomSampler_getFloat(rs);
-      const float dt = -flog(1.f - xi) / mu_max;
-      t += dt;
-      if (t > tb)
-        break;
-      xi = RandomSampler_getFloat(rs);
+    for (;;) {
+      const float seg = mu_max * (tb - t);
+      if (tau > seg) { // the track leaves the interval
+        tau -= seg;
+        return true;
+      }
+      t += tau / mu_max; // tentative collision
+      const float xi = RandomSampler_getFloat(rs);
       const vec3f p = o + t * d;
       const float smp = sampleVolume(v, p);
-      if (std::isnan(smp))
-        continue;
-      const vec4f color = tfGet(v, smp);
-      mu_t = v.densityScale * color.w;
-      if (xi < mu_t / mu_max) {
-        albedo = v3(color.x, color.y, color.z);
-        result = t;
-        return false;
+      if (!std::isnan(smp)) {
+        const vec4f color = tfGet(v, smp);
+        mu_t = v.densityScale * color.w;
+        if (xi < mu_t / mu_max) {
+          albedo = v3(color.x, color.y, color.z);
+          result = t;
+          return false;
+        }
       }
+      if (++steps >= kMaxDeltaSteps)
+        return false;
+      tau = -flog(1.f - RandomSampler_getFloat(rs));
     }
-    return true;
   });
   return result;
 }

@@ -7,12 +7,13 @@
 // step and per collapse level):
 //   items    per item (triangle, quad half, sphere): box, and the centre
 //            bounds reduced over all items
-//   morton   30-bit Morton code of the box centre in the centre bounds
+//   morton   30-bit Morton code of the box centre on a cubic grid over the
+//            centre bounds
 //   sort     stable LSD radix sort, 8-bit digits: per-block digit
 //            histograms, one exclusive scan, then a scatter whose ranks
 //            come from wave ballots (stable within a wave, waves and rounds
 //            in order)
-//   cluster  PLOC: every cluster takes the neighbour within +-kRadius of
+//   cluster  PLOC: every cluster takes the neighbour within +-radius of
 //            the sorted order whose union box has the least surface area
 //            (ties to the smaller index, so the globally closest link is
 //            always mutual and every step merges); mutual pairs become new
@@ -37,7 +38,7 @@ using hpt::DNode4;
 using hpt::DPrim;
 
 constexpr int kBlock = 256;
-constexpr int kRadius = 16; // PLOC search window
+constexpr int kMaxRadius = 32; // PLOC search window: up to kMaxRadius clusters on either side
 constexpr uint32_t kMaxLeaf = 2; // items per leaf (as the host build)
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kBlock * kSortItems;
@@ -176,9 +177,12 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4 *lo, const float
     return;
   const float4 a = lo[i], b = hi[i];
   const float p[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+  // one cubic grid over the centre bounds (the largest extent on every
+  // axis): a thin axis then takes few code bits instead of splitting
+  // spatial neighbours at coarse levels
+  const float ext = fmaxf(bounds6[3] - bounds6[0], fmaxf(bounds6[4] - bounds6[1], bounds6[5] - bounds6[2]));
   uint32_t q[3];
   for (int k = 0; k < 3; k++) {
-    const float ext = bounds6[3 + k] - bounds6[k];
     const float t = ext > 0.f ? (p[k] - bounds6[k]) / ext : 0.f;
     const float s = fminf(fmaxf(t * 1024.f, 0.f), 1023.f);
     q[k] = (uint32_t)s;
@@ -372,12 +376,12 @@ __global__ __launch_bounds__(kBlock) void k_leaves(const uint32_t *vals, uint32_
 }
 
 __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *cluster, uint32_t c, const float4 *lo,
-    const float4 *hi, uint32_t *nn)
+    const float4 *hi, uint32_t *nn, int radius)
 {
-  __shared__ float4 sLo[kBlock + 2 * kRadius], sHi[kBlock + 2 * kRadius];
+  __shared__ float4 sLo[kBlock + 2 * kMaxRadius], sHi[kBlock + 2 * kMaxRadius];
   const int64_t b0 = (int64_t)blockIdx.x * kBlock;
-  for (int t = threadIdx.x; t < kBlock + 2 * kRadius; t += kBlock) {
-    const int64_t j = b0 - kRadius + t;
+  for (int t = threadIdx.x; t < kBlock + 2 * kMaxRadius; t += kBlock) {
+    const int64_t j = b0 - kMaxRadius + t;
     if (j >= 0 && j < (int64_t)c) {
       const uint32_t node = cluster[j];
       sLo[t] = lo[node];
@@ -388,13 +392,13 @@ __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *cluster, uin
   const int64_t i = b0 + threadIdx.x;
   if (i >= (int64_t)c)
     return;
-  const float4 a0 = sLo[threadIdx.x + kRadius], a1 = sHi[threadIdx.x + kRadius];
+  const float4 a0 = sLo[threadIdx.x + kMaxRadius], a1 = sHi[threadIdx.x + kMaxRadius];
   unsigned long long best = ~0ull;
-  for (int o = -kRadius; o <= kRadius; o++) {
+  for (int o = -radius; o <= radius; o++) {
     const int64_t j = i + o;
     if (o == 0 || j < 0 || j >= (int64_t)c)
       continue;
-    const float area = unionHalfArea(a0, a1, sLo[threadIdx.x + kRadius + o], sHi[threadIdx.x + kRadius + o]);
+    const float area = unionHalfArea(a0, a1, sLo[threadIdx.x + kMaxRadius + o], sHi[threadIdx.x + kMaxRadius + o]);
     const unsigned long long key = ((unsigned long long)__float_as_uint(area) << 32) | (unsigned long long)j;
     best = key < best ? key : best;
   }
@@ -651,12 +655,12 @@ HPT_API uint64_t hpt_bvh_items(const HptBvhGeometry *geoms, int numGeoms)
 
 HPT_API size_t hpt_bvh_workspace_bytes(uint64_t numItems) { return layout(numItems < 1 ? 1 : numItems).end; }
 
-HPT_API hipError_t hpt_bvh_build(const HptBvhGeometry *geoms, int numGeoms, void *workspace, size_t workspaceBytes,
-    void *outNodes, void *outPrims, uint64_t *hostWords, HptBvhResult *result, hipStream_t s)
+HPT_API hipError_t hpt_bvh_build(const HptBvhGeometry *geoms, int numGeoms, int radius, void *workspace,
+    size_t workspaceBytes, void *outNodes, void *outPrims, uint64_t *hostWords, HptBvhResult *result, hipStream_t s)
 {
   const uint64_t items = hpt_bvh_items(geoms, numGeoms);
-  if (items < 3 || items >= (1ull << 28) || numGeoms < 1 || numGeoms > 4096 || !workspace || !outNodes ||
-      !outPrims || !hostWords || !result)
+  if (items < 3 || items >= (1ull << 28) || numGeoms < 1 || numGeoms > 4096 || radius < 1 || radius > kMaxRadius ||
+      !workspace || !outNodes || !outPrims || !hostWords || !result)
     return hipErrorInvalidValue;
   const uint32_t n = (uint32_t)items;
   const Layout L = layout(n);
@@ -715,7 +719,7 @@ HPT_API hipError_t hpt_bvh_build(const HptBvhGeometry *geoms, int numGeoms, void
   int steps = 0;
   while (c > 1) {
     const uint32_t cb = cdiv(c, kBlock);
-    k_ploc_nn<<<cb, kBlock, 0, s>>>(cl, c, lo, hi, nn);
+    k_ploc_nn<<<cb, kBlock, 0, s>>>(cl, c, lo, hi, nn, radius);
     HPT_TRY(hipGetLastError());
     k_ploc_flags<<<cb, kBlock, 0, s>>>(nn, c, flags);
     HPT_TRY(hipGetLastError());

@@ -991,6 +991,12 @@ class HipDevice : public Device
   int batchStreams = 4; // lanes a batch of views spreads over
   int frameLanes = 1; // lanes the passes of one frame alternate over (1: one stream)
   bool inBatch = false; // enqueuing the views of a batch
+  int batchView = -1; // index of the view being enqueued in a batch
+  // captured batches: one frame-index word per view (written before each
+  // graph launch) and the variance-frame coin table of mt19937(43)
+  DevBuf batchState, varianceCoins;
+  uint32_t *batchStateHost = nullptr;
+  size_t batchStateViews = 0;
   hipEvent_t passEvents[2] = {nullptr, nullptr};
   DevBuf stats;
   // radiance cache per-pass buffers: vertex records, queues, raw inputs,
@@ -1003,6 +1009,7 @@ class HipDevice : public Device
   // BLAS build: on the device (default) or on the host (device parameter
   // "hostBvh"); pinned words for the device build's readbacks
   bool hostBvh = false;
+  int bvhRadius = 16; // the device build's clustering window ("bvhRadius", 1..32)
   uint64_t *bvhWords = nullptr;
   uint64_t *bvhHostWords()
   {
@@ -1026,6 +1033,8 @@ class HipDevice : public Device
       (void)hipEventDestroy(batchStart);
     if (bvhWords)
       (void)hipHostFree(bvhWords);
+    if (batchStateHost)
+      (void)hipHostFree(batchStateHost);
     for (auto e : laneEvents)
       (void)hipEventDestroy(e);
     for (auto e : passEvents)
@@ -1046,6 +1055,7 @@ class HipDevice : public Device
     batchStreams = std::max(1, std::min(8, getInt("batchStreams", batchStreams)));
     frameLanes = std::max(1, std::min(8, getInt("frameLanes", frameLanes)));
     hostBvh = getBool("hostBvh", hostBvh);
+    bvhRadius = std::max(1, std::min(32, getInt("bvhRadius", bvhRadius)));
     if (stream) {
       HIP_CHECK(hipStreamSynchronize(stream));
       sizeGrids();
@@ -1413,16 +1423,19 @@ class HipDevice : public Device
     try {
       for (size_t i = 0; i < v.size(); i++) {
         lane = lanes[i % K].get();
+        batchView = (int)i;
         futures.push_back(renderFrame(v[i].fb, v[i].renderer, v[i].camera, world));
         HIP_CHECK(hipEventRecord(viewEvents[i], lane->stream));
       }
     } catch (...) {
       lane = lanes[0].get();
       inBatch = false;
+      batchView = -1;
       throw;
     }
     lane = lanes[0].get();
     inBatch = false;
+    batchView = -1;
     for (int k = 1; k < K; k++) { // join
       HIP_CHECK(hipEventRecord(laneEvents[k], lanes[k]->stream));
       HIP_CHECK(hipStreamWaitEvent(stream, laneEvents[k], 0));
@@ -1456,13 +1469,16 @@ class HipDevice : public Device
             key.push_back(o);
             key.push_back((void *)(uintptr_t)o->commitCounter);
           }
+        bool relaunch = true;
         if (key != graphKey || !graphExec) {
           releaseGraph();
+          relaunch = false;
           // an eager pass first: it sizes every buffer, so nothing is
           // allocated (or synchronised) while capturing
           enqueueViews(v, world, futures);
           HIP_CHECK(hipStreamSynchronize(stream));
           releaseFutures();
+          ensureBatchState(v.size());
           capturing = true;
           HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
           try {
@@ -1481,9 +1497,29 @@ class HipDevice : public Device
           HIP_CHECK(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
           graphKey = key;
         }
+        // the views' frame indices: the captured frames' on the first
+        // launch; each relaunch begins the next frame of every view as
+        // ospRenderFrame would (FrameBuffer::beginFrame, FrameBuffer.cpp:
+        // 115-137), the device reading the index written here
+        for (size_t i = 0; i < v.size(); i++) {
+          FrameBuffer *fb = (FrameBuffer *)v[i].fb;
+          if (relaunch && (fb->channels & OSP_FB_ACCUM))
+            beginFrameHost(fb);
+          batchStateHost[i] = (uint32_t)std::max(0, fb->frameID);
+        }
+        HIP_CHECK(hipMemcpyAsync(batchState.ptr, batchStateHost, v.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+            stream));
         HIP_CHECK(hipGraphLaunch(graphExec, stream));
       }
       HIP_CHECK(hipStreamSynchronize(stream));
+      if (mode != 0) // the error estimates the captured frames left on the device
+        for (auto &x : v) {
+          FrameBuffer *fb = (FrameBuffer *)x.fb;
+          if (fb->variance.ptr && fb->frameID >= 1) {
+            HIP_CHECK(hipMemcpy(fb->hostVariance, fb->varianceOut.ptr, sizeof(float), hipMemcpyDeviceToHost));
+            fb->varianceValid = true;
+          }
+        }
     } catch (...) {
       capturing = false;
       releaseFutures();
@@ -1497,6 +1533,41 @@ class HipDevice : public Device
       }
   }
 
+  // device words of a captured batch's frame indices (one per view) and the
+  // coin table of the variance frames: bit k is the k-th draw of
+  // mt19937(43) (FrameBuffer.h:134-137), which a framebuffer takes on its
+  // frame 2k.  The graph refers to batchState: growing it drops the graph.
+  void ensureBatchState(size_t views)
+  {
+    if (views > batchStateViews) {
+      releaseGraph();
+      batchState.ensure(views * sizeof(uint32_t));
+      if (batchStateHost)
+        HIP_CHECK(hipHostFree(batchStateHost));
+      batchStateHost = nullptr;
+      HIP_CHECK(hipHostMalloc((void **)&batchStateHost, views * sizeof(uint32_t), hipHostMallocDefault));
+      batchStateViews = views;
+    }
+    if (!varianceCoins.ptr) {
+      std::vector<uint32_t> bits(hpt::kVarianceCoinBits / 32, 0u);
+      std::mt19937 g(43);
+      for (uint32_t k = 0; k < hpt::kVarianceCoinBits; k++)
+        if (g() & 1u)
+          bits[k >> 5] |= 1u << (k & 31u);
+      varianceCoins.upload(bits);
+    }
+  }
+  // FrameBuffer::beginFrame's host half (FrameBuffer.cpp:115-137)
+  static void beginFrameHost(FrameBuffer *fb)
+  {
+    fb->frameID = (fb->channels & OSP_FB_ACCUM) ? fb->frameID + 1 : 0;
+    if (fb->variance.ptr) {
+      const bool evenFrame = (fb->frameID & 1) == 0;
+      if (evenFrame)
+        fb->pickOdd = (fb->mtGen() & 1u) != 0;
+      fb->accumulateVariance = evenFrame != fb->pickOdd;
+    }
+  }
   // The batch ends when its last lane joins the device stream: the latest
   // of the views' end events (each measured from the batch start).
   static float batchMs(const std::vector<float> &ms)
@@ -1696,8 +1767,8 @@ static std::shared_ptr<Blas> buildBlasDevice(HipDevice &D, const std::vector<Geo
   nodes.ensure(n * sizeof(hpt::DNode4));
   B->dPrims.ensure(n * sizeof(hpt::DPrim));
   HptBvhResult r;
-  HIP_CHECK(hpt_bvh_build(hg.data(), (int)hg.size(), ws.ptr, ws.bytes, nodes.ptr, B->dPrims.ptr, D.bvhHostWords(),
-      &r, D.stream));
+  HIP_CHECK(hpt_bvh_build(hg.data(), (int)hg.size(), D.bvhRadius, ws.ptr, ws.bytes, nodes.ptr, B->dPrims.ptr,
+      D.bvhHostWords(), &r, D.stream));
   if (r.stackNeed > kDeviceBlasStackLimit)
     return nullptr;
   B->dNodes.ensure((size_t)r.numNodes * sizeof(hpt::DNode4));
@@ -2338,13 +2409,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
 
   // FrameBuffer::beginFrame (FrameBuffer.cpp:115-137)
   const bool doAccum = (fb->channels & OSP_FB_ACCUM) != 0;
-  fb->frameID = doAccum ? fb->frameID + 1 : 0;
-  if (fb->variance.ptr) {
-    const bool evenFrame = (fb->frameID & 1) == 0;
-    if (evenFrame)
-      fb->pickOdd = (fb->mtGen() & 1u) != 0;
-    fb->accumulateVariance = evenFrame != fb->pickOdd;
-  }
+  beginFrameHost(fb);
   fb->errorThreshold = r->errorThreshold;
   const bool finishedAccum = fb->targetFrames && fb->frameID >= fb->targetFrames;
 
@@ -2416,6 +2481,18 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   // minimumAdaptiveFrames = max(2, 16 / spp) (fb/FrameBuffer.h:183-186)
   if (r->errorThreshold > 0.f && fb->variance.ptr && fb->frameID >= std::max(2, 16 / std::max(1, r->spp)))
     F.adaptiveThreshold = r->errorThreshold;
+  // a captured frame on an accumulating framebuffer reads its index from
+  // the device, so each launch of the graph renders the next frame
+  if (capturing && doAccum) {
+    if (fb->targetFrames > 0)
+      throw std::runtime_error("batched views: a captured batch cannot stop at targetFrames");
+    if (batchView < 0 || (size_t)batchView >= batchStateViews)
+      throw std::runtime_error("batched views: no frame state for this view");
+    F.frameState = batchState.as<const uint32_t>() + batchView;
+    F.varianceCoins = varianceCoins.as<const uint32_t>();
+    F.errorThreshold = fb->variance.ptr ? r->errorThreshold : 0.f;
+    F.minAdaptiveFrames = std::max(2, 16 / std::max(1, r->spp));
+  }
   F.maxRec = (int)std::max<uint32_t>(1u, F.numFirstLightSamples);
   F.hasVolumes = sceneF.numVolInsts > 0 ? 1 : 0;
   F.stats = travStats ? 1 : 0;
@@ -2524,11 +2601,14 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     const HptFrame *Fp = reinterpret_cast<const HptFrame *>(&F);
     launch("fb_accumulate", [&]() { return hpt_fb_accumulate(Fp, stream); });
     // LocalFrameBuffer::postProcess (LocalFB.cpp:459-478)
-    if (fb->variance.ptr && (fb->frameID & 1) == 1) {
+    // (a captured frame runs them on every launch; they check the device-
+    // resident frame index and do their work after odd frames)
+    if (fb->variance.ptr && ((fb->frameID & 1) == 1 || F.frameState)) {
       const int ntask = F.tilesX * F.tilesY;
       launch("variance", [&]() { return hpt_variance(Fp, stream); });
       launch("variance_reduce", [&]() {
-        return hpt_variance_reduce(F.taskVariance, ntask, fb->errorThreshold, fb->varianceOut.as<float>(), stream);
+        return hpt_variance_reduce(F.taskVariance, ntask, fb->errorThreshold, fb->varianceOut.as<float>(),
+            F.frameState, stream);
       });
       if (!capturing) // a captured batch leaves the error estimate on the device
         HIP_CHECK(hipMemcpyAsync(fb->hostVariance, fb->varianceOut.ptr, sizeof(float), hipMemcpyDeviceToHost, stream));

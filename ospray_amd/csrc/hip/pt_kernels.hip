@@ -118,11 +118,34 @@ __device__ __forceinline__ int pixelSlot(const DFrame &F, int ix, int iy)
   return ((iy >> 3) * F.tilesX + (ix >> 3)) * 64 + (iy & 7) * 8 + (ix & 7);
 }
 
-// adaptive accumulation: is the 8x8 task of pixel (ix, iy) rendered this
-// frame (LocalFB.cpp:238-261)?  adaptiveThreshold 0: every task is
-__device__ __forceinline__ bool taskActive(const DFrame &F, int ix, int iy)
+// the frame index (accumID): the host's, or the device-resident one of a
+// captured batch frame
+__device__ __forceinline__ uint32_t frameAccumID(const DFrame &F) { return F.frameState ? *F.frameState : F.accumID; }
+
+// does this frame accumulate into the variance buffer (FrameBuffer.cpp:
+// 126-136): even frames draw a coin, odd frames are the other half
+__device__ __forceinline__ bool frameAccVar(const DFrame &F)
 {
-  return F.adaptiveThreshold == 0.f || F.taskVariance[(iy >> 3) * F.tilesX + (ix >> 3)] > F.adaptiveThreshold;
+  if (!F.frameState)
+    return F.accumulateVariance != 0;
+  const uint32_t id = *F.frameState;
+  const bool pickOdd = ((F.varianceCoins[id >> 6] >> ((id >> 1) & 31u)) & 1u) != 0u;
+  return ((id & 1u) == 0u) != pickOdd;
+}
+
+// the adaptive-accumulation threshold of this frame (0: every task)
+__device__ __forceinline__ float frameAdaptive(const DFrame &F)
+{
+  if (!F.frameState)
+    return F.adaptiveThreshold;
+  return *F.frameState >= (uint32_t)F.minAdaptiveFrames ? F.errorThreshold : 0.f;
+}
+
+// adaptive accumulation: is the 8x8 task of pixel (ix, iy) rendered this
+// frame (LocalFB.cpp:238-261)?  threshold 0: every task is
+__device__ __forceinline__ bool taskActive(const DFrame &F, float threshold, int ix, int iy)
+{
+  return threshold == 0.f || F.taskVariance[(iy >> 3) * F.tilesX + (ix >> 3)] > threshold;
 }
 
 __device__ __forceinline__ uint32_t clz32(uint32_t x) { return x ? (uint32_t)__clz((int)x) : 32u; }
@@ -202,12 +225,14 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
     cdfY = lut + nn;
   }
   const int total = F.passCount * F.numPixPadded;
+  const uint32_t accumID = frameAccumID(F);
+  const float adaptive = frameAdaptive(F);
   for (int i = globalId(); i < total; i += gridStride()) {
     const int sLocal = i / F.numPixPadded;
     const int q = i - sLocal * F.numPixPadded;
     int ix, iy;
     slotPixel(F, q, ix, iy);
-    const bool valid = ix < F.width && iy < F.height && taskActive(F, ix, iy);
+    const bool valid = ix < F.width && iy < F.height && taskActive(F, adaptive, ix, iy);
     if (valid) {
       const uint32_t spp = (uint32_t)F.spp;
       const uint32_t s = (uint32_t)(F.passS0 + sLocal);
@@ -220,7 +245,7 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
       } else {
         seed ^= (uint32_t)F.width * (uint32_t)iy + (uint32_t)ix;
       }
-      const uint32_t sampleID = F.accumID * spp + s;
+      const uint32_t sampleID = accumID * spp + s;
       // LDSampler_init (random.ih:283-294); the index masks are frame-uniform
       LDSampler lds;
       lds.revIndex = reverseBits(sampleOffs + sampleID);
@@ -257,7 +282,7 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
           st4(a + k, 0.f, 0.f, 0.f, 0.f);
       }
     }
-    if (F.adaptiveThreshold == 0.f) {
+    if (adaptive == 0.f) {
       // every pixel of the image is queued: the compacted position follows
       // from the slot layout (whole 8-row tile rows above, full-width tiles
       // to the left, then row-major inside the tile)
@@ -273,7 +298,7 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
         P.queue[0][slot] = (uint32_t)i;
     }
   }
-  if (F.adaptiveThreshold == 0.f && globalId() == 0)
+  if (adaptive == 0.f && globalId() == 0)
     P.counters[0] = F.passCount * F.width * F.height;
 }
 
@@ -1023,9 +1048,10 @@ __global__ __launch_bounds__(256) void k_resolve(DScene S, DFrame F, DPaths P, i
 __global__ __launch_bounds__(256) void k_finish_pass(DFrame F, DPaths P)
 {
   const int npix = F.width * F.height;
+  const float adaptive = frameAdaptive(F);
   for (int pix = globalId(); pix < npix; pix += gridStride()) {
     const int ix = pix % F.width, iy = pix / F.width;
-    if (!taskActive(F, ix, iy))
+    if (!taskActive(F, adaptive, ix, iy))
       continue;
     const int q = pixelSlot(F, ix, iy);
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1087,11 +1113,13 @@ __global__ __launch_bounds__(256) void k_fb_accumulate(DFrame F)
 {
   const int npix = F.width * F.height;
   const float rspp = 1.0f / (float)F.spp;
-  const int accumID = (int)F.accumID;
+  const int accumID = (int)frameAccumID(F);
   const float accScale = 1.0f / (float)(accumID + 1);
   const float accVarScale = 1.0f / (float)(accumID / 2 + 1);
+  const bool accVar = frameAccVar(F);
+  const float adaptive = frameAdaptive(F);
   for (int pix = globalId(); pix < npix; pix += gridStride()) {
-    if (!taskActive(F, pix % F.width, pix / F.width))
+    if (!taskActive(F, adaptive, pix % F.width, pix / F.width))
       continue;
     const float4 a = ld4(F.accRGBA + 4 * (size_t)pix);
     const float r = a.x * rspp, g = a.y * rspp, b = a.z * rspp, al = a.w * rspp;
@@ -1105,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_fb_accumulate(DFrame F)
             lerpAcc(old.w, al, accScale));
       }
     }
-    if (F.fbVariance && F.accumulateVariance) {
+    if (F.fbVariance && accVar) {
       float *c = F.fbVariance + 4 * (size_t)pix;
       if (accVarScale > 0.8f)
         st4(c, r, g, b, al);
@@ -1145,9 +1173,13 @@ __global__ __launch_bounds__(256) void k_fb_accumulate(DFrame F)
   }
 }
 
-// Variance_kernel (fb/frame_ops/Variance.ispc:11-59), one thread per task
+// Variance_kernel (fb/frame_ops/Variance.ispc:11-59), one thread per task;
+// run after odd frames (LocalFB.cpp:459-466): a captured batch frame
+// checks its device-resident index
 __global__ __launch_bounds__(256) void k_variance(DFrame F)
 {
+  if (F.frameState && (*F.frameState & 1u) == 0u)
+    return;
   const int tx = (F.width + 7) / 8, ty = (F.height + 7) / 8;
   const int ntask = tx * ty;
   for (int t = globalId(); t < ntask; t += gridStride()) {
@@ -1181,8 +1213,11 @@ __global__ __launch_bounds__(256) void k_variance(DFrame F)
 
 // LiveVarianceFrameOp::getAvgError (Variance.cpp:32-46) with an FP64 sum;
 // single block
-__global__ __launch_bounds__(256) void k_variance_reduce(const float *taskVar, int n, float threshold, float *out)
+__global__ __launch_bounds__(256) void k_variance_reduce(const float *taskVar, int n, float threshold, float *out,
+    const uint32_t *frameState)
 {
+  if (frameState && (*frameState & 1u) == 0u) // even frame of a captured batch: nothing to reduce
+    return;
   __shared__ double sSum[256];
   __shared__ float sMax[256];
   __shared__ int sCnt[256];
@@ -1449,9 +1484,10 @@ hipError_t hpt_variance(const HptFrame *F, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t hpt_variance_reduce(const float *taskVariance, int n, float threshold, float *out, hipStream_t s)
+hipError_t hpt_variance_reduce(const float *taskVariance, int n, float threshold, float *out,
+    const uint32_t *frameState, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_variance_reduce, dim3(1), dim3(256), 0, s, taskVariance, n, threshold, out);
+  hipLaunchKernelGGL(k_variance_reduce, dim3(1), dim3(256), 0, s, taskVariance, n, threshold, out, frameState);
   return hipGetLastError();
 }
 

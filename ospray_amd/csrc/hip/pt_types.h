@@ -329,6 +329,19 @@ struct DFrame
   // nrcTrainOffset), which runs on and becomes a training record
   int32_t nrcBounce;
   uint32_t nrcTrainStride, nrcTrainOffset;
+  // Frames of a captured batch of views on accumulating framebuffers read
+  // their frame index from device memory (*frameState, written before each
+  // launch of the graph), and derive from it what eager frames get from the
+  // host: the variance-frame choice (bit k of varianceCoins is the k-th
+  // draw of the framebuffer's mt19937(43), taken on frame 2k), adaptive
+  // accumulation (errorThreshold from frame minAdaptiveFrames on) and the
+  // odd-frame variance estimate.  Null in eager frames.
+  const uint32_t *frameState;
+  const uint32_t *varianceCoins;
+  float errorThreshold;
+  int32_t minAdaptiveFrames;
 };
+
+constexpr uint32_t kVarianceCoinBits = 1u << 20; // frames up to 2^21 in captured batches
 
 } // namespace hpt

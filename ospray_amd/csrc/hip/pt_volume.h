@@ -156,8 +156,8 @@ HPT_HD bool volumeBox(const DVolume &v, vec3f o, vec3f d, float t0, float t1, fl
   return lo < hi;
 }
 
-// tentative collisions per interval before delta tracking gives up (the
-// interval then counts as passed)
+// tentative collisions per track before delta tracking gives up (the
+// track then counts as passed)
 constexpr int kMaxDeltaSteps = 1 << 16;
 
 // work counters of delta tracking: voxel samples (tentative collisions),
@@ -169,7 +169,13 @@ struct VolCount
 
 // delta_tracking (VolumeSampler.ispc:13-92) over [t0, t1] (inside the box),
 // walking the macrocells with a 3D DDA: the distance of the first real
-// collision (inf if none) and its albedo
+// collision (inf if none) and its albedo.  The optical depth to the next
+// tentative collision, tau ~ Exp(1), is drawn once per track and after each
+// tentative collision, and carried across macrocell intervals: an interval
+// of majorant mu and length l uses up mu * l of it.  The exponential is
+// memoryless, so the collision distances have the distribution of a fresh
+// draw per interval, with one logarithm per collision instead of one per
+// interval.
 HPT_HD float deltaTracking(
     const DVolume &v, vec3f o, vec3f d, float t0, float t1, RandomSampler &rs, vec3f &albedo, VolCount &cnt)
 {
@@ -191,6 +197,8 @@ HPT_HD float deltaTracking(
     tnext[a] = step[a] == 0 ? kInf : (v.origin[a] + (float)(c + (step[a] > 0 ? 1 : 0)) * msz[a] - oo[a]) * ii[a];
   }
   float t = t0;
+  float tau = -flog(1.f - randomFloat(rs));
+  int steps = 0;
   for (int guard = v.mdims[0] + v.mdims[1] + v.mdims[2]; t < t1 && guard > 0; guard--) {
     const int a = tnext[0] < tnext[1] ? (tnext[0] < tnext[2] ? 0 : 2) : (tnext[1] < tnext[2] ? 1 : 2);
     const float tb = minf(tnext[a], t1);
@@ -201,24 +209,28 @@ HPT_HD float deltaTracking(
       const float mu_max = v.majorant[mi];
       if (mu_max > 0.f) {
         float tt = t;
-        for (int k = 0; k < kMaxDeltaSteps; k++) {
-          float xi = randomFloat(rs);
-          const float dt = -flog(1.f - xi) / mu_max;
-          tt += dt;
-          if (tt > tb)
+        for (;;) {
+          const float seg = mu_max * (tb - tt);
+          if (tau > seg) { // the track leaves the interval
+            tau -= seg;
             break;
-          xi = randomFloat(rs);
+          }
+          tt += tau / mu_max; // tentative collision
+          const float xi = randomFloat(rs);
           cnt.samples++;
           const vec3f p = o + tt * d;
           const float smp = volumeSample(v, p);
-          if (smp != smp)
-            continue;
-          const float u = tfUnit(v, smp);
-          const float mu_t = v.densityScale * tfOpacity(v, u);
-          if (xi < mu_t / mu_max) {
-            albedo = tfColor(v, u);
-            return tt;
+          if (smp == smp) {
+            const float u = tfUnit(v, smp);
+            const float mu_t = v.densityScale * tfOpacity(v, u);
+            if (xi < mu_t / mu_max) {
+              albedo = tfColor(v, u);
+              return tt;
+            }
           }
+          if (++steps >= kMaxDeltaSteps)
+            return kInf;
+          tau = -flog(1.f - randomFloat(rs));
         }
       }
       t = tb;

@@ -34,8 +34,9 @@ def test_core_exports_every_declared_function():
 
 def test_module_exports_every_declared_function():
     names = (_declared("ospray/ospray_hip.h", "OSPRAY_INTERFACE") + _declared("ospray_hip/hip_pt.h", "HPT_API")
-             + _declared("ospray_hip/hip_nrc.h", "HPT_API"))
+             + _declared("ospray_hip/hip_nrc.h", "HPT_API") + _declared("ospray_hip/hip_bvh.h", "HPT_API"))
     assert "hpt_extend" in names and "ospHipTraceRays" in names and "hpt_nrc_train" in names
+    assert "hpt_bvh_build" in names and "ospHipGetWorldBuild" in names
     exported = _exports(os.path.join(K.LIBDIR, "libospray_module_hip.so"))
     missing = [n for n in names + ["ospray_module_init_hip"] if n not in exported]
     assert not missing, missing

@@ -94,3 +94,81 @@ def test_autotuner_keeps_images(hip_device):
     for k, v in (("traceBlocksPerCU", 12), ("shadeBlocksPerCU", 8), ("pathBudgetMB", 16384), ("batchStreams", 4)):
         A.set_device_param(dev, k, v)
     L.ospDeviceCommit(dev)
+
+
+def test_graph_relaunches_accumulate(hip_device):
+    """Accumulating framebuffers (with a variance buffer, one renderer with
+    adaptive accumulation): the first captured call renders two frames (the
+    sizing pass and the graph's first launch), each relaunch the next frame
+    of every view, as ospRenderFrame would; images, depth and ids and
+    ospGetVariance (which follows the variance buffer and, with the
+    threshold, the adaptive task choice) equal those of as many eager
+    frames."""
+    sc = scenes.c0(32, 32)
+    s = scenes.to_ospray(sc)
+    ch = K.OSP_FB_COLOR | K.OSP_FB_ACCUM | K.OSP_FB_VARIANCE | K.OSP_FB_DEPTH | K.OSP_FB_ID_OBJECT
+    specs = [(40, 24, 1, (0.0, 4.0, -20.0), 0.0), (64, 64, 8, (15.0, 6.0, -10.0), 0.05),
+             (24, 40, 4, (-12.0, 10.0, 12.0), 0.0)]
+
+    def views():
+        fbs, rens, cams = [], [], []
+        for (w, h, spp, pos, thr) in specs:
+            cams.append(A.new("camera", "perspective", {"position": pos, "direction": tuple(-x for x in pos),
+                                                         "up": (0.0, 1.0, 0.0), "fovy": 45.0, "aspect": w / h}))
+            rens.append(A.new("renderer", "pathtracer", {"pixelSamples": spp, "maxPathLength": 4,
+                                                         "varianceThreshold": thr}))
+            fbs.append(A.framebuffer(w, h, K.OSP_FB_RGBA32F, ch))
+        return fbs, rens, cams
+
+    fbs, rens, cams = views()
+    efbs, erens, ecams = views()
+    L = K.lib()
+    frames = 0
+    for call in range(4):
+        A.render_batch(fbs, rens, cams, s.world, graph=True)
+        n = 2 if call == 0 else 1
+        for _ in range(n):
+            for fb, r, c in zip(efbs, erens, ecams):
+                A.render(fb, r, c, s.world)
+        frames += n
+        for fb, efb, (w, h, *_rest) in zip(fbs, efbs, specs):
+            assert np.array_equal(A.map_color(fb, w, h, K.OSP_FB_RGBA32F), A.map_color(efb, w, h, K.OSP_FB_RGBA32F))
+            for c in (K.OSP_FB_DEPTH, K.OSP_FB_ID_OBJECT):
+                assert np.array_equal(A.map_channel(fb, c, w, h), A.map_channel(efb, c, w, h)), (call, c)
+            assert L.ospGetVariance(fb.h) == L.ospGetVariance(efb.h), (call, frames)
+    assert frames == 5
+    # the images do change from launch to launch (accumulation advances)
+    before = A.map_color(fbs[0], 40, 24, K.OSP_FB_RGBA32F)
+    A.render_batch(fbs, rens, cams, s.world, graph=True)
+    assert not np.array_equal(before, A.map_color(fbs[0], 40, 24, K.OSP_FB_RGBA32F))
+
+
+def test_c2_views_against_oracle_bands(hip_device):
+    """The 64-view C2 batch of the full 1080p scene (seeded positions on a
+    radius-30 sphere, resolutions {256^2, 512^2, 768x432, 1024^2}, spp
+    {1, 2, 4, 8}, maxPathLength 8) as one captured graph on accumulating
+    framebuffers, launched twice: every view's two middle rows equal the
+    oracle's render of those rows over the same two frames, bit for bit."""
+    from oracle import orc
+
+    sc = scenes.c1()
+    views = scenes.c2_views()
+    s = scenes.to_ospray(sc)
+    fbs, rens, cams = [], [], []
+    for v in views:
+        cams.append(A.new("camera", "perspective", dict(v["camera"])))
+        rens.append(A.new("renderer", "pathtracer", {"pixelSamples": v["spp"], "maxPathLength": 8}))
+        fbs.append(A.framebuffer(v["width"], v["height"], K.OSP_FB_RGBA32F, K.OSP_FB_COLOR | K.OSP_FB_ACCUM))
+    A.render_batch(fbs, rens, cams, s.world, graph=True)  # sizing pass + first launch: 2 frames
+    osc = orc.OracleScene(sc)
+    for v, fb in zip(views, fbs):
+        w, h = v["width"], v["height"]
+        img = A.map_color(fb, w, h, K.OSP_FB_RGBA32F)
+        assert np.all(np.isfinite(img))
+        y0 = h // 2
+        ofb = orc.OracleFrameBuffer(w, h, 3, K.OSP_FB_COLOR | K.OSP_FB_ACCUM)
+        cam = orc.make_camera(v["camera"])
+        ren = orc.make_renderer({"pixelSamples": v["spp"], "maxPathLength": 8})
+        for _ in range(2):
+            ofb.render(osc, cam, ren, 16, (y0, y0 + 2))
+        assert np.array_equal(img[y0:y0 + 2], ofb.channel(K.OSP_FB_COLOR)[y0:y0 + 2]), v

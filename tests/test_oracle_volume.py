@@ -189,6 +189,40 @@ def test_linear_medium_null_collisions_and_instance_scale():
             assert abs((1 - hit[i]) - p) <= _binom_tol(p, n), (scale, i, hit[i], p)
 
 
+def test_transmittance_across_many_macrocells():
+    """Tracks through 6 x 2 x 2 macrocells (16^3 cells each): the exponential
+    draw of the distance to the next tentative collision carries over from
+    one macrocell interval to the next, so a homogeneous medium still passes
+    exp(-mu L) and a ramp whose majorant changes from cell to cell passes
+    exp(-integral of mu)."""
+    n = 20000
+    nx, ny, nz = 97, 33, 33  # spacing 0.125: box [0,12] x [0,4] x [0,4]
+    mu = 0.3
+    hom = _const_volume(opacity=mu, dims=(nx, ny, nz), spacing=(0.125, 0.125, 0.125))
+    d = np.array([12.0, 4.0, 0.0]) / math.hypot(12.0, 4.0)
+    rays = np.array([[-1, 2.1, 1.9, 1, 0, 0, 0, np.inf],  # along x: 6 macrocells, length 12
+                     [*(np.array([0.0, 0.0, 1.3]) - d), *d, 0, np.inf]], np.float32)  # x-y diagonal
+    lengths = [12.0, math.hypot(12.0, 4.0)]
+    T, hit, _ = orc.OracleScene(_vol_scene(hom)).volume_estimates(rays, n, seed=3)
+    for i, L in enumerate(lengths):
+        p = math.exp(-mu * L)
+        assert abs(T[i] - p) <= _binom_tol(p, n), (i, T[i], p)
+        assert abs((1 - hit[i]) - p) <= _binom_tol(p, n), (i, hit[i], p)
+    # value x / 12, opacity = value, densityScale 0.5: mu(x) = x / 24
+    x = np.linspace(0.0, 1.0, nx, dtype=np.float32)
+    data = np.broadcast_to(x[None, None, :], (nz, ny, nx)).copy()
+    ramp = S.Volume(data, np.zeros((2, 3), np.float32), np.array([0.0, 1.0], np.float32),
+                    spacing=(0.125, 0.125, 0.125), density_scale=0.5)
+    rays = np.array([[-1, 2.1, 1.9, 1, 0, 0, 0, np.inf],  # the whole ramp: tau = 3
+                     [13, 0.7, 3.3, -1, 0, 0, 0, np.inf],  # the other way
+                     [-1, 1.0, 1.0, 1, 0, 0, 0, 7.0]], np.float32)  # x in [0, 6]: tau = 0.75
+    tau = [3.0, 3.0, 0.75]
+    T, hit, _ = orc.OracleScene(_vol_scene(ramp)).volume_estimates(rays, n, seed=5)
+    for i in range(3):
+        p = math.exp(-tau[i])
+        assert abs(T[i] - p) <= _binom_tol(p, n), (i, T[i], p)
+
+
 # ---- Henyey-Greenstein -------------------------------------------------------------------------
 
 def _grid_samples(k=256):
