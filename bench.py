@@ -229,8 +229,8 @@ def bench_c2(args) -> int:
     log(f"autotuner: {time.perf_counter() - t:.1f} s, best batch {best_ms:.2f} ms with traceBlocksPerCU {best[0]}, "
         f"shadeBlocksPerCU {best[1]}, pathBudgetMB {best[2]}, batchStreams {best[3]}")
     t = time.perf_counter()
-    A.render_batch(fbs, rens, cams, s.world, graph=True)  # eager sizing pass + capture + first launch
-    log(f"graph captured and launched in {time.perf_counter() - t:.2f} s")
+    A.render_batch(fbs, rens, cams, s.world, graph=True)  # one eager frame, then the capture
+    log(f"eager frame and graph capture in {time.perf_counter() - t:.2f} s")
     for _ in range(args.warmup):
         A.render_batch(fbs, rens, cams, s.world, graph=True)
     t0 = time.perf_counter()

@@ -33,14 +33,16 @@ OSPRAY_INTERFACE int ospHipResetKernelStats(void);
  * (out9[6..8]). */
 OSPRAY_INTERFACE int ospHipGetTraversalStats(uint64_t out9[9]);
 
-/* Renders n views (framebuffer, renderer, camera) of one world in one
- * enqueue and waits for them.  mode 0: eager launches; mode 1: one HIP
- * graph, captured on the first call for this batch (after an eager pass
- * that sizes every buffer) and relaunched while the batch's objects are
- * unchanged; relaunches replay the captured frames (same sample indices),
- * and leave ospGetVariance's estimate on the device.  viewMs (optional,
- * n floats): GPU milliseconds from the batch start to the end of each
- * view.  Renderers with "nrc" cannot be captured.  Returns 0 on success. */
+/* Renders one frame of each of n views (framebuffer, renderer, camera) of
+ * one world in one enqueue and waits for them.  mode 0: eager launches;
+ * mode 1: one HIP graph.  The first mode-1 call for a batch renders its
+ * frame eagerly (which sizes every buffer) and captures the graph; later
+ * calls relaunch it while the batch's objects are unchanged.  Each launch
+ * begins the next frame of every view as ospRenderFrame would (the frame
+ * index is read from device memory), so accumulating framebuffers
+ * accumulate.  viewMs (optional, n floats): GPU milliseconds from the
+ * batch start to the end of each view.  Renderers with "nrc" cannot be
+ * captured.  Returns 0 on success. */
 OSPRAY_INTERFACE int ospHipRenderBatch(int n, const OSPFrameBuffer *framebuffers, const OSPRenderer *renderers,
     const OSPCamera *cameras, OSPWorld world, int mode, float *viewMs);
 

@@ -938,6 +938,7 @@ struct FrameBuffer : public ManagedObject
   float errorThreshold = 0.f;
   DevBuf color, variance, depth, normal, albedo, firstNormal, position, primID, objID, instID, taskVar;
   DevBuf accRGBA, accAux, accZ, accIDs, converted, varianceOut;
+  DevBuf cancelLatch; // set on the device when a frame of this framebuffer is cancelled
   std::map<int, std::vector<unsigned char>> staging;
   float *hostVariance = nullptr; // pinned
   FrameBuffer(HipDevice *d, int w_, int h_, OSPFrameBufferFormat f, uint32_t ch);
@@ -962,6 +963,11 @@ struct Future : public ManagedObject
 {
   HipDevice *dev;
   hipEvent_t start = nullptr, rendered = nullptr, finished = nullptr;
+  // one event after each pass's accumulation (ospGetProgress)
+  std::vector<hipEvent_t> passDone;
+  // the frame's generation: ospCancel writes it into the device's
+  // cancellation ring (0: a captured frame, not cancellable)
+  uint32_t gen = 0;
   Ref<ManagedObject> fb, renderer, camera, world;
   bool empty = false;
   Future(HipDevice *d) : dev(d) { managedType = OSP_FUTURE; }
@@ -1018,6 +1024,12 @@ class HipDevice : public Device
     return bvhWords;
   }
   bool capturing = false; // enqueuing a batch of views into a HIP graph
+  // ospCancel: pinned host words, one per frame generation modulo
+  // kCancelRing; cancelling frame g writes g into word g % kCancelRing,
+  // which the frame's per-bounce counter resets read
+  static constexpr uint32_t kCancelRing = 256;
+  uint32_t *cancelRing = nullptr, *cancelRingDev = nullptr;
+  uint32_t frameGen = 0;
   std::vector<KernelTimer> timers;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pendingEvents;
   std::map<std::vector<std::pair<const Geometry *, uint64_t>>, std::shared_ptr<Blas>> blasCache;
@@ -1035,6 +1047,11 @@ class HipDevice : public Device
       (void)hipHostFree(bvhWords);
     if (batchStateHost)
       (void)hipHostFree(batchStateHost);
+    if (cancelRing) {
+      if (stream)
+        (void)hipStreamSynchronize(stream);
+      (void)hipHostFree(cancelRing);
+    }
     for (auto e : laneEvents)
       (void)hipEventDestroy(e);
     for (auto e : passEvents)
@@ -1080,6 +1097,10 @@ class HipDevice : public Device
       gpuName = prop.gcnArchName;
       stats.ensure(128);
       HIP_CHECK(hipMemset(stats.ptr, 0, 128));
+      HIP_CHECK(hipHostMalloc((void **)&cancelRing, kCancelRing * sizeof(uint32_t),
+          hipHostMallocMapped | hipHostMallocCoherent));
+      std::memset(cancelRing, 0, kCancelRing * sizeof(uint32_t));
+      HIP_CHECK(hipHostGetDevicePointer((void **)&cancelRingDev, cancelRing, 0));
       ensureLane(0);
       lane = lanes[0].get();
       sizeGrids();
@@ -1345,11 +1366,33 @@ class HipDevice : public Device
     if (e >= OSP_FRAME_FINISHED)
       collectTimers();
   }
-  void cancel(ManagedObject *) override
+  // FrameBuffer::cancelFrame (FrameBuffer.cpp:180-187): the frame's
+  // remaining passes and bounces do no work (the reference's render tasks
+  // return early, PathTracer.ispc:231) and a cancelled frame leaves the
+  // framebuffer's image as it was; the frame still advances the frame index
+  // and runs its frame operations, as in the reference.
+  void cancel(ManagedObject *fo) override
   {
-    // queued GPU work is not interrupted; the frame completes normally
+    Future *f = (Future *)fo;
+    if (f->empty || !f->gen || !cancelRing)
+      return;
+    __atomic_store_n(&cancelRing[f->gen % kCancelRing], f->gen, __ATOMIC_SEQ_CST);
   }
-  float getProgress(ManagedObject *fo) override { return isReady(fo, OSP_TASK_FINISHED) ? 1.f : 0.f; }
+  // FrameBuffer::getCurrentProgress (FrameBuffer.cpp:169-178): the fraction
+  // of the frame's passes whose samples have been accumulated (every pass
+  // covers every pixel), 1 once the frame has finished.
+  float getProgress(ManagedObject *fo) override
+  {
+    Future *f = (Future *)fo;
+    if (isReady(fo, OSP_TASK_FINISHED))
+      return 1.f;
+    if (f->passDone.empty())
+      return 0.f;
+    size_t done = 0;
+    for (auto e : f->passDone)
+      done += hipEventQuery(e) == hipSuccess ? 1 : 0;
+    return (float)done / (float)f->passDone.size();
+  }
   float getTaskDuration(ManagedObject *fo) override
   {
     Future *f = (Future *)fo;
@@ -1452,6 +1495,7 @@ class HipDevice : public Device
     if (!batchStart)
       HIP_CHECK(hipEventCreate(&batchStart));
     std::vector<ManagedObject *> futures;
+    bool timed = false;
     auto releaseFutures = [&]() {
       for (auto *f : futures)
         f->refDec();
@@ -1469,16 +1513,22 @@ class HipDevice : public Device
             key.push_back(o);
             key.push_back((void *)(uintptr_t)o->commitCounter);
           }
-        bool relaunch = true;
         if (key != graphKey || !graphExec) {
           releaseGraph();
-          relaunch = false;
-          // an eager pass first: it sizes every buffer, so nothing is
-          // allocated (or synchronised) while capturing
+          // this call's frame is rendered eagerly: it sizes every buffer,
+          // so nothing is allocated (or synchronised) while capturing
           enqueueViews(v, world, futures);
           HIP_CHECK(hipStreamSynchronize(stream));
           releaseFutures();
+          viewTimes(v.size(), outMs);
+          timed = true;
           ensureBatchState(v.size());
+          // the captured renderFrame calls begin a frame on the host; the
+          // framebuffers' host frame state is restored after the capture,
+          // so that every call renders exactly one frame of each view
+          std::map<FrameBuffer *, FrameHostState> saved;
+          for (auto &x : v)
+            saved.emplace((FrameBuffer *)x.fb, FrameHostState((FrameBuffer *)x.fb));
           capturing = true;
           HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
           try {
@@ -1489,27 +1539,31 @@ class HipDevice : public Device
             if (g)
               (void)hipGraphDestroy(g);
             capturing = false;
+            for (auto &kv : saved)
+              kv.second.restore(kv.first);
             throw;
           }
           HIP_CHECK(hipStreamEndCapture(stream, &graph));
           capturing = false;
+          for (auto &kv : saved)
+            kv.second.restore(kv.first);
           releaseFutures();
           HIP_CHECK(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
           graphKey = key;
+        } else {
+          // each launch begins the next frame of every view as
+          // ospRenderFrame would (FrameBuffer::beginFrame, FrameBuffer.cpp:
+          // 115-137), the device reading the index written here
+          for (size_t i = 0; i < v.size(); i++) {
+            FrameBuffer *fb = (FrameBuffer *)v[i].fb;
+            if (fb->channels & OSP_FB_ACCUM)
+              beginFrameHost(fb);
+            batchStateHost[i] = (uint32_t)std::max(0, fb->frameID);
+          }
+          HIP_CHECK(hipMemcpyAsync(batchState.ptr, batchStateHost, v.size() * sizeof(uint32_t),
+              hipMemcpyHostToDevice, stream));
+          HIP_CHECK(hipGraphLaunch(graphExec, stream));
         }
-        // the views' frame indices: the captured frames' on the first
-        // launch; each relaunch begins the next frame of every view as
-        // ospRenderFrame would (FrameBuffer::beginFrame, FrameBuffer.cpp:
-        // 115-137), the device reading the index written here
-        for (size_t i = 0; i < v.size(); i++) {
-          FrameBuffer *fb = (FrameBuffer *)v[i].fb;
-          if (relaunch && (fb->channels & OSP_FB_ACCUM))
-            beginFrameHost(fb);
-          batchStateHost[i] = (uint32_t)std::max(0, fb->frameID);
-        }
-        HIP_CHECK(hipMemcpyAsync(batchState.ptr, batchStateHost, v.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-            stream));
-        HIP_CHECK(hipGraphLaunch(graphExec, stream));
       }
       HIP_CHECK(hipStreamSynchronize(stream));
       if (mode != 0) // the error estimates the captured frames left on the device
@@ -1526,12 +1580,36 @@ class HipDevice : public Device
       throw;
     }
     releaseFutures();
-    for (size_t i = 0; i < v.size(); i++)
-      if (!outMs || hipEventElapsedTime(&outMs[i], batchStart, viewEvents[i]) != hipSuccess) {
-        if (outMs)
-          outMs[i] = -1.f;
-      }
+    if (!timed)
+      viewTimes(v.size(), outMs);
   }
+  // GPU milliseconds from the batch start to the end of each view
+  void viewTimes(size_t n, float *outMs)
+  {
+    for (size_t i = 0; outMs && i < n; i++)
+      if (hipEventElapsedTime(&outMs[i], batchStart, viewEvents[i]) != hipSuccess)
+        outMs[i] = -1.f;
+  }
+  // the host half of a framebuffer's frame state (what beginFrame changes)
+  struct FrameHostState
+  {
+    int frameID;
+    std::mt19937 mtGen;
+    bool pickOdd, accumulateVariance, varianceValid;
+    explicit FrameHostState(const FrameBuffer *fb)
+        : frameID(fb->frameID), mtGen(fb->mtGen), pickOdd(fb->pickOdd), accumulateVariance(fb->accumulateVariance),
+          varianceValid(fb->varianceValid)
+    {
+    }
+    void restore(FrameBuffer *fb) const
+    {
+      fb->frameID = frameID;
+      fb->mtGen = mtGen;
+      fb->pickOdd = pickOdd;
+      fb->accumulateVariance = accumulateVariance;
+      fb->varianceValid = varianceValid;
+    }
+  };
 
   // device words of a captured batch's frame indices (one per view) and the
   // coin table of the variance frames: bit k is the k-th draw of
@@ -1690,6 +1768,7 @@ FrameBuffer::FrameBuffer(HipDevice *d, int w_, int h_, OSPFrameBufferFormat f, u
   if ((ch & OSP_FB_COLOR) && f != OSP_FB_NONE)
     zero(converted, n * (f == OSP_FB_RGBA32F ? 16 : 4));
   zero(varianceOut, 16);
+  zero(cancelLatch, 16);
   HIP_CHECK(hipHostMalloc((void **)&hostVariance, sizeof(float), hipHostMallocDefault));
   *hostVariance = hpt::kInf;
 }
@@ -1712,6 +1791,8 @@ Future::~Future()
     (void)hipEventDestroy(rendered);
   if (finished)
     (void)hipEventDestroy(finished);
+  for (auto e : passDone)
+    (void)hipEventDestroy(e);
 }
 
 // ---- world build -------------------------------------------------------------------------
@@ -2402,10 +2483,14 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   HIP_CHECK(hipEventCreate(&fut->start));
   HIP_CHECK(hipEventCreate(&fut->rendered));
   HIP_CHECK(hipEventCreate(&fut->finished));
-  if (capturing)
+  if (capturing) {
     fut->empty = true; // a captured frame: its completion is the graph's
-  else
+  } else {
     HIP_CHECK(hipEventRecord(fut->start, stream));
+    if (++frameGen == 0)
+      ++frameGen;
+    fut->gen = frameGen;
+  }
 
   // FrameBuffer::beginFrame (FrameBuffer.cpp:115-137)
   const bool doAccum = (fb->channels & OSP_FB_ACCUM) != 0;
@@ -2540,6 +2625,10 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     const HptScene *S = reinterpret_cast<const HptScene *>(&sceneF);
     if (travStats)
       HIP_CHECK(hpt_fill_u32((uint32_t *)stats.ptr, 32, 0u, stream));
+    uint32_t *const latch = capturing ? nullptr : fb->cancelLatch.as<uint32_t>();
+    if (latch)
+      HIP_CHECK(hpt_fill_u32(latch, 1, 0u, stream));
+    F.cancelLatch = latch;
     if (nl > 1) { // fork: the other lanes start after the frame's earlier work
       HIP_CHECK(hipEventRecord(laneEvents[0], stream));
       for (int k = 1; k < nl; k++)
@@ -2548,6 +2637,9 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
     for (int s0 = 0, pass = 0; s0 < F.spp; s0 += sppPerPass, pass++) {
       lane = pass % nl == 0 ? frameLane : lanes[pass % nl].get();
       const hipStream_t stream = lane->stream;
+      lane->paths.cancelFlag = latch ? cancelRingDev + fut->gen % kCancelRing : nullptr;
+      lane->paths.cancelGen = fut->gen;
+      lane->paths.cancelLatch = latch;
       const HptPaths *P = reinterpret_cast<const HptPaths *>(&lane->paths);
       F.passS0 = s0;
       F.passCount = std::min(sppPerPass, F.spp - s0);
@@ -2590,6 +2682,12 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       launch("finish_pass", [&]() { return hpt_finish_pass(Fp, P, stream); });
       if (nl > 1)
         HIP_CHECK(hipEventRecord(passEvents[pass & 1], stream));
+      if (!capturing) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        fut->passDone.push_back(e);
+        HIP_CHECK(hipEventRecord(e, stream));
+      }
     }
     lane = frameLane;
     for (int k = 1; k < nl; k++) { // join

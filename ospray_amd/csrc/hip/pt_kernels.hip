@@ -185,11 +185,34 @@ __device__ __forceinline__ vec2f pixelFilterSample(const DFrame &F, vec2f s, con
 
 // ---- kernels ----------------------------------------------------------------------
 
-__global__ void k_reset(int32_t *counters, int n, int32_t keep)
+// Resets the queue counters of a pass (keep -1) or of a bounce (keep: the
+// extension queue being read).  A cancelled frame (the host wrote its
+// generation into *cancelFlag, or the latch is already set) latches
+// *cancelLatch and also empties the kept queue, so the frame's remaining
+// kernels find no work.  One wave: lane 0's reads are broadcast.
+__global__ void k_reset(int32_t *counters, int n, int32_t keep, const uint32_t *cancelFlag, uint32_t gen,
+    uint32_t *cancelLatch)
 {
   const int i = globalId();
-  if (i < n && i != keep)
+  uint32_t cancelled = 0u;
+  if (cancelLatch) {
+    if (i == 0) {
+      cancelled = __hip_atomic_load(cancelLatch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (!cancelled && cancelFlag)
+        cancelled = __hip_atomic_load(cancelFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == gen ? 1u : 0u;
+    }
+    cancelled = __shfl(cancelled, 0);
+  }
+  if (i < n && (i != keep || cancelled))
     counters[i] = 0;
+  if (cancelled && i == 0)
+    __hip_atomic_store(cancelLatch, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the frame's cancellation latch (read by kernels that run once per pass)
+__device__ __forceinline__ bool frameCancelled(const uint32_t *latch)
+{
+  return latch && __hip_atomic_load(latch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
 
 __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
@@ -212,6 +235,14 @@ constexpr int kLutLds = 1024; // floats of pixel-filter LUT staged in LDS
 
 __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
 {
+  // a cancelled frame generates no rays (block-uniform: the LUT staging
+  // below has a barrier)
+  __shared__ uint32_t cancelled;
+  if (threadIdx.x == 0)
+    cancelled = frameCancelled(P.cancelLatch) ? 1u : 0u;
+  __syncthreads();
+  if (cancelled)
+    return;
   // the filter LUT's CDFs in LDS: every camera sample runs two binary
   // searches over them
   __shared__ float lut[kLutLds];
@@ -1047,6 +1078,8 @@ __global__ __launch_bounds__(256) void k_resolve(DScene S, DFrame F, DPaths P, i
 // (PathSampler.ispc:409-414, PathTracer.ispc:196-197, ScreenSample.ih:59-77)
 __global__ __launch_bounds__(256) void k_finish_pass(DFrame F, DPaths P)
 {
+  if (frameCancelled(F.cancelLatch))
+    return;
   const int npix = F.width * F.height;
   const float adaptive = frameAdaptive(F);
   for (int pix = globalId(); pix < npix; pix += gridStride()) {
@@ -1111,6 +1144,8 @@ __device__ __forceinline__ float lerpAcc(float old, float nw, float f) { return 
 // fb/LocalFB.ispc:14-78)
 __global__ __launch_bounds__(256) void k_fb_accumulate(DFrame F)
 {
+  if (frameCancelled(F.cancelLatch)) // a cancelled frame leaves the framebuffer as it was
+    return;
   const int npix = F.width * F.height;
   const float rspp = 1.0f / (float)F.spp;
   const int accumID = (int)frameAccumID(F);
@@ -1320,7 +1355,8 @@ hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s)
   // a full reset clears every counter; the per-bounce reset only the queue
   // counters of a bounce, keeping the extension queue being read
   const int n = keepIndex < 0 ? kNumCounters : kNumBounceCounters;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, n, keepIndex < 0 ? -1 : 2 * keepIndex);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, n, keepIndex < 0 ? -1 : 2 * keepIndex,
+      p.cancelFlag, p.cancelGen, p.cancelLatch);
   return hipGetLastError();
 }
 

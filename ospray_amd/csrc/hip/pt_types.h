@@ -255,6 +255,15 @@ struct DPaths
   // voxel samples, macrocell intervals, tracked segments: [9..11] free
   // paths, [12..14] transmittance estimates
   unsigned long long *stats;
+  // cancellation (FrameBuffer::cancelFrame, FrameBuffer.cpp:180-187): the
+  // host writes the frame's generation into *cancelFlag (pinned host
+  // memory); the per-bounce counter reset reads it and latches
+  // *cancelLatch (device memory, one word per framebuffer, cleared when the
+  // frame begins), after which the frame's kernels do no more work.  Null
+  // in captured frames.
+  const uint32_t *cancelFlag;
+  uint32_t cancelGen;
+  uint32_t *cancelLatch;
 };
 
 // queue counters: [2q] entries of extension queue q (q = 0, 1), [2q+1] NEE
@@ -340,6 +349,9 @@ struct DFrame
   const uint32_t *varianceCoins;
   float errorThreshold;
   int32_t minAdaptiveFrames;
+  // set when the frame was cancelled: the frame's accumulation is skipped
+  // (null in captured frames)
+  const uint32_t *cancelLatch;
 };
 
 constexpr uint32_t kVarianceCoinBits = 1u << 20; // frames up to 2^21 in captured batches
