@@ -9,8 +9,10 @@
   * cancelling a frame that has already finished changes nothing: later
     frames equal those of a framebuffer whose frames were never cancelled.
 
-The long frame is C0 at 1024x1024 with 64 spp and a 1 MiB path budget, so
-that every pass holds one sample per pixel (64 passes)."""
+The long frame is C0 at 2048x2048 with 16 spp and a 1 MiB path budget, so
+that every pass holds one sample per pixel (16 passes of 4 M paths: few
+enough launches that the host finishes enqueuing long before the GPU
+finishes the frame)."""
 import time
 
 import numpy as np
@@ -22,8 +24,8 @@ from ospray_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
-W = H = 1024
-SPP = 64
+W = H = 2048
+SPP = 16
 
 
 def _budget(mb):
@@ -87,7 +89,7 @@ def test_cancel_stops_the_frame(long_frame):
     assert _finish(f) == 1.0
     cancelled_s = time.perf_counter() - t
     print(f"full frame {full_s * 1e3:.1f} ms, cancelled frame {cancelled_s * 1e3:.1f} ms")
-    assert cancelled_s < 0.75 * full_s
+    assert cancelled_s < 0.5 * full_s
     assert np.array_equal(A.map_color(fb, W, H, K.OSP_FB_RGBA32F), before)
     assert np.array_equal(A.map_channel(fb, K.OSP_FB_DEPTH, W, H), depth)
     # the next frame is not cancelled: it accumulates
