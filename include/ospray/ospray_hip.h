@@ -76,6 +76,12 @@ OSPRAY_INTERFACE int ospHipGetVolumeStats(uint64_t out6[6]);
 OSPRAY_INTERFACE int ospHipTraceRays(
     OSPWorld world, const float *rays8, int n, float *outTUV, uint32_t *outIDs, uint64_t *stats2);
 
+/* The launch-layer view of a committed world (an HptScene, see
+ * ospray_hip/hip_pt.h) for callers of the hpt_* entry points; valid until
+ * the world is committed again or released.  NULL if the world has not
+ * been committed. */
+OSPRAY_INTERFACE const void *ospHipGetWorldLaunchScene(OSPWorld world);
+
 /* GPU name, compute-unit count and the BVH statistics of a committed world
  * (total nodes, total primitives, deepest BLAS). */
 OSPRAY_INTERFACE int ospHipGetDeviceInfo(char *name, int len, int *numCUs);

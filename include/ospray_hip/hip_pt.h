@@ -100,6 +100,22 @@ HPT_API hipError_t hpt_color_convert(const float *color, int npix, int format, v
 HPT_API hipError_t hpt_trace_rays(const HptScene *S, const float *rays8, int n, float *outTUV, uint32_t *outIDs,
     uint32_t *overflow, int residentBlocks, unsigned long long *stats, hipStream_t s);
 
+/* hpt_trace_rays without a scratch argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The traversal scratch rows come from a fixed device array of
+ * the module (62 MiB per GPU, held until the process ends), so
+ * residentBlocks must be in [1, hpt_trace_noalloc_max_blocks()]; anything
+ * else returns hipErrorInvalidValue before launching.  Allocates nothing
+ * (no hipMalloc, hipFree or allocating library call).  All calls on one GPU
+ * share that GPU's array: calls that overlap in time on one GPU (on
+ * different streams or from different threads) race and must be
+ * serialised by the caller.  Results equal hpt_trace_rays's. */
+HPT_API hipError_t hpt_trace_rays_noalloc(const HptScene *S, const float *rays8, int n, float *outTUV,
+    uint32_t *outIDs, int residentBlocks, unsigned long long *stats, hipStream_t s);
+/* the largest residentBlocks hpt_trace_rays_noalloc accepts */
+HPT_API int hpt_trace_noalloc_max_blocks(void);
+
 /* threads per block of the traversal kernels */
 HPT_API int hpt_trace_block_size(void);
 /* 32-bit words of the scratch row each resident traversal thread needs in

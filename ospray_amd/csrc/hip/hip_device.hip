@@ -2970,6 +2970,17 @@ OSPRAY_INTERFACE int ospHipTraceRays(OSPWorld world, const float *rays8, int n, 
   }
 }
 
+OSPRAY_INTERFACE const void *ospHipGetWorldLaunchScene(OSPWorld world)
+{
+  HipDevice *d = current();
+  if (!d || !world)
+    return nullptr;
+  World *w = (World *)world;
+  if (w->managedType != OSP_WORLD || !w->built)
+    return nullptr;
+  return &w->scene;
+}
+
 OSPRAY_INTERFACE int ospHipGetDeviceInfo(char *name, int len, int *numCUs)
 {
   HipDevice *d = current();

@@ -19,6 +19,8 @@
 // resident grid).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "ospray_hip/hip_pt.h"
 #include "pt_shade.h"
 #include "pt_volume.h"
@@ -1333,6 +1335,43 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DScene S, const floa
   }
 }
 
+// The traversal scratch rows of hpt_trace_rays_noalloc: a file-scope device
+// array, kThreadRow words for each thread of at most kNoAllocTraceBlocks
+// resident blocks (62 MiB).  Every GPU that runs the module holds its own
+// copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that GPU, looked up once per
+// device on the first call there.
+constexpr int kNoAllocTraceBlocks = 1024;
+static __device__ __align__(256) uint32_t scratch[(size_t)kNoAllocTraceBlocks * kTraceBlock * kThreadRow];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+uint32_t *g_scratchPtr[kMaxDevices] = {};
+
+hipError_t noAllocScratch(uint32_t **out)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return e;
+  if (dev < 0 || dev >= kMaxDevices)
+    return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return e;
+    if (!p)
+      return hipErrorInvalidSymbol;
+    g_scratchPtr[dev] = static_cast<uint32_t *>(p);
+  }
+  *out = g_scratchPtr[dev];
+  return hipSuccess;
+}
+} // namespace
+
 // ---- C-ABI launch layer -------------------------------------------------------------
 
 namespace {
@@ -1540,6 +1579,26 @@ hipError_t hpt_trace_rays(const HptScene *S, const float *rays8, int n, float *o
       rays8, n, outTUV, outIDs, overflow, stats);
   return hipGetLastError();
 }
+
+hipError_t hpt_trace_rays_noalloc(const HptScene *S, const float *rays8, int n, float *outTUV, uint32_t *outIDs,
+    int residentBlocks, unsigned long long *stats, hipStream_t s)
+{
+  if (!S || n < 0 || residentBlocks < 1 || residentBlocks > kNoAllocTraceBlocks)
+    return hipErrorInvalidValue;
+  if (n > 0 && (!rays8 || !outTUV || !outIDs))
+    return hipErrorInvalidValue;
+  uint32_t *rows = nullptr;
+  const hipError_t e = noAllocScratch(&rows);
+  if (e != hipSuccess)
+    return e;
+  // closest-hit traversal uses only the stack overflow of a row, which it
+  // writes before reading: nothing in the rows needs clearing
+  hipLaunchKernelGGL(k_trace_rays, dim3(residentBlocks), dim3(kTraceBlock), 0, s,
+      *reinterpret_cast<const DScene *>(S), rays8, n, outTUV, outIDs, rows, stats);
+  return hipGetLastError();
+}
+
+int hpt_trace_noalloc_max_blocks() { return kNoAllocTraceBlocks; }
 
 int hpt_trace_block_size() { return kTraceBlock; }
 int hpt_overflow_depth() { return kThreadRow; }
