@@ -89,8 +89,9 @@ OSPRAY_INTERFACE int ospHipGetWorldInfo(OSPWorld world, int64_t out4[4]);
 
 /* BLAS builds of the world's last commit: milliseconds spent building the
  * BLASes it did not find in the device's cache, how many of its BLASes were
- * built on the device (the device parameter "hostBvh" (bool) selects the
- * host builder), and the traversal stack entries its BVH needs. */
+ * built on the device (the device parameter "hostBvh" (bool, default
+ * true) selects the host binned-SAH builder; false selects the device
+ * Morton + PLOC builder), and the traversal stack entries its BVH needs. */
 OSPRAY_INTERFACE int ospHipGetWorldBuild(OSPWorld world, double *blasBuildMs, int *deviceBlas, int *stackNeed);
 
 /* Mean duration in ms of `reps` device-to-device copies of `bytes` (rounded

@@ -41,6 +41,14 @@ HPT_API hipError_t hpt_copy(void *dst, const void *src, size_t bytes, hipStream_
 HPT_API hipError_t hpt_generate(const HptFrame *F, const HptPaths *P, hipStream_t s);
 /* Closest hit of every ray in extension queue `queue` (World.ih:247-286). */
 HPT_API hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int residentBlocks, int stats, hipStream_t s);
+/* FP64 re-resolution of the extension hits hpt_extend found within a
+ * near-tie window of another hit (the FP64 intersection check on near-tie
+ * rays; no counterpart in the reference, which trusts Embree's float hit):
+ * every hit in the window gets its distance in double precision and the
+ * smallest (FP64 t, key) is kept.  residentBlocks: at most the grid the
+ * traversal scratch rows are sized for.  stats != 0 adds node visits and
+ * primitive tests to P->stats[0..1]. */
+HPT_API hipError_t hpt_ties(const HptScene *S, const HptPaths *P, int residentBlocks, int stats, hipStream_t s);
 /* volumeSampleFreePath (volumes/VolumeSampler.ispc:96-199) of every ray of
  * extension queue `queue` after hpt_extend: delta tracking up to the
  * surface hit; a collision turns the hit record into a volume vertex.

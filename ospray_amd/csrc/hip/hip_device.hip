@@ -831,7 +831,7 @@ struct Renderer : public ManagedObject
   // steps per frame, learning rate, initial-weight seed; the network and
   // its optimiser state persist across frames (online training) until
   // "nrcReset" is committed or the seed changes
-  bool nrc = false;
+  bool nrc = false, nrcSelfTrain = true;
   int nrcBounce = 1, nrcTrainPaths = 8192, nrcTrainSteps = 4;
   float nrcLr = 1e-2f;
   uint32_t nrcSeed = 1;
@@ -879,6 +879,7 @@ struct Renderer : public ManagedObject
     nrcBounce = std::max(1, getInt("nrcBounce", 1));
     nrcTrainPaths = std::max(64, getInt("nrcTrainPaths", 8192));
     nrcTrainSteps = std::max(1, getInt("nrcTrainSteps", 4));
+    nrcSelfTrain = getBool("nrcSelfTrain", true);
     nrcLr = getFloat("nrcLearningRate", 1e-2f);
     const uint32_t seed = (uint32_t)getInt("nrcSeed", 1);
     if (getBool("nrcReset", false) || seed != nrcSeed)
@@ -1011,10 +1012,14 @@ class HipDevice : public Device
   size_t nrcSlots = 0;
   int residentTraceBlocks = 0, shadeBlocks = 0;
   size_t pathBudgetBytes = (size_t)16 << 30;
-  bool profile = false, travStats = false;
-  // BLAS build: on the device (default) or on the host (device parameter
-  // "hostBvh"); pinned words for the device build's readbacks
-  bool hostBvh = false;
+  // profileBounces: the stage timers of the bounce loop are kept per bounce
+  // ("extend@2", ...)
+  bool profile = false, travStats = false, profileBounces = false;
+  // BLAS build: on the host (binned SAH, the default: about 10 % fewer node
+  // visits per ray than the device build on the 4.2M-triangle terrain) or on
+  // the device (Morton sort + PLOC, about 8x faster to commit); device
+  // parameter "hostBvh".  Pinned words for the device build's readbacks.
+  bool hostBvh = true;
   int bvhRadius = 16; // the device build's clustering window ("bvhRadius", 1..32)
   uint64_t *bvhWords = nullptr;
   uint64_t *bvhHostWords()
@@ -1067,6 +1072,7 @@ class HipDevice : public Device
     deviceID = getInt("deviceID", 0);
     profile = getBool("profileKernels", false);
     travStats = getBool("traversalStats", false);
+    profileBounces = getBool("profileBounces", false);
     traceBlocksPerCU = std::max(1, std::min(64, getInt("traceBlocksPerCU", traceBlocksPerCU)));
     shadeBlocksPerCU = std::max(1, std::min(64, getInt("shadeBlocksPerCU", shadeBlocksPerCU)));
     batchStreams = std::max(1, std::min(8, getInt("batchStreams", batchStreams)));
@@ -1175,6 +1181,9 @@ class HipDevice : public Device
     timers.push_back({name, 0, 0.0});
     return (int)timers.size() - 1;
   }
+  // grid of the near-tie pass: near-ties are rare, so a small resident grid
+  // (at most the traversal grid, whose scratch rows it uses)
+  int tieBlocks() const { return std::max(1, std::min(residentTraceBlocks, 2 * numCUs)); }
   template <typename F>
   void launch(const char *name, F &&fn)
   {
@@ -1410,9 +1419,9 @@ class HipDevice : public Device
   // ---- batched views (no reference counterpart) ----
   // Many views (framebuffer, renderer, camera) of one world in one
   // enqueue: eagerly (mode 0) or as one captured HIP graph (mode 1), built
-  // once per batch and relaunched; graph launches replay the captured
-  // frames (same sample indices).  outMs[i] = GPU time from the batch start
-  // to the end of view i.
+  // once per batch and relaunched; every call renders the next frame of
+  // each view.  outMs[i] = GPU time from the batch start to the end of
+  // view i.
   struct BatchView
   {
     ManagedObject *fb, *renderer, *camera;
@@ -2312,7 +2321,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   const size_t perSlot = 9 * 16 + 4 + 16 + 4 + 16 + 6 * 16 + (auxAllocated ? 64 : 0);
   L.pathBuf.ensure(S * perSlot);
   L.recBuf.ensure(R * 80);
-  L.queueBuf.ensure((2 * S + 3 * R) * 4);
+  L.queueBuf.ensure((3 * S + 3 * R) * 4);
   // recorded transparent hits of up to 4M transparency-queue records (the
   // rest are traversed again); 4 words per hit + a count
   const size_t transCap = std::min<size_t>(R, (size_t)1 << 22);
@@ -2357,6 +2366,7 @@ void HipDevice::ensurePaths(size_t slots, int maxRec, bool aux)
   paths.shadowQueue = q + 2 * S;
   paths.transQueue = q + 2 * S + R;
   paths.neeSampleQueue = q + 2 * S + 2 * R;
+  paths.tieQueue = q + 2 * S + 3 * R;
   paths.transHits = (float *)L.transBuf.ptr;
   paths.transCount = (uint32_t *)((char *)L.transBuf.ptr + transHitBytes);
   paths.transCap = (int32_t)transCap;
@@ -2371,7 +2381,9 @@ void HipDevice::ensureNrc(size_t slots)
     releaseGraph();
     HIP_CHECK(hipStreamSynchronize(lane->stream));
     nrcSlots = slots;
-    nrcRec.ensure(slots * 80); // 16 floats record + 4 floats contribution before
+    // 16 floats query record + 16 floats training record + 4 floats
+    // contribution before the training vertex
+    nrcRec.ensure(slots * 144);
     nrcQueues.ensure(slots * 8);
     nrcRaw.ensure(slots * 48);
     nrcFeat.ensure(slots * HPT_NRC_INPUTS * 2);
@@ -2380,7 +2392,8 @@ void HipDevice::ensureNrc(size_t slots)
   }
   hpt::DPaths &paths = lane->paths;
   paths.nrcRec = nrcRec.as<float>();
-  paths.nrcBefore = nrcRec.as<float>() + 16 * nrcSlots;
+  paths.nrcTrainRec = nrcRec.as<float>() + 16 * nrcSlots; // aliased to nrcRec without self-training
+  paths.nrcBefore = nrcRec.as<float>() + 32 * nrcSlots;
   paths.nrcQueryQueue = nrcQueues.as<uint32_t>();
   paths.nrcTrainQueue = nrcQueues.as<uint32_t>() + nrcSlots;
 }
@@ -2610,8 +2623,12 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       lane->paths.aux = aux ? lane->auxBase : nullptr;
       lane->paths.overflow = lane->overflow.as<uint32_t>();
     }
-    if (F.nrcBounce >= 0)
+    if (F.nrcBounce >= 0) {
       ensureNrc(lane->pathSlots);
+      F.nrcSelfTrain = r->nrcSelfTrain ? 1 : 0;
+      if (!F.nrcSelfTrain) // training records in the query records: the path runs to its end
+        lane->paths.nrcTrainRec = lane->paths.nrcRec;
+    }
     for (int k = 0; k < 2; k++)
       if (!passEvents[k])
         HIP_CHECK(hipEventCreateWithFlags(&passEvents[k], hipEventDisableTiming));
@@ -2654,19 +2671,32 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       const HptFrame *FpNo = reinterpret_cast<const HptFrame *>(&Fno);
       launch("reset", [&]() { return hpt_reset_counters(P, -1, stream); });
       launch("generate", [&]() { return hpt_generate(Fp, P, stream); });
+      // with self-training no path goes beyond the vertex after the cache's
+      int lastBounce = F.maxDepth;
+      if (F.nrcBounce >= 0 && F.nrcSelfTrain)
+        lastBounce = std::min(F.maxDepth, F.nrcBounce + 1);
       int q = 0;
-      for (int b = 0; b <= F.maxDepth; b++) {
-        launch("reset", [&]() { return hpt_reset_counters(P, q, stream); });
-        launch("extend", [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
+      for (int b = 0; b <= lastBounce; b++) {
+        const std::string at = profile && profileBounces ? "@" + std::to_string(b) : std::string();
+        auto nm = [&](const char *stage) { return at.empty() ? std::string(stage) : stage + at; };
+        launch(nm("reset").c_str(), [&]() { return hpt_reset_counters(P, q, stream); });
+        launch(nm("extend").c_str(),
+            [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
+        launch(nm("ties").c_str(), [&]() { return hpt_ties(S, P, tieBlocks(), travStats ? 1 : 0, stream); });
         if (F.hasVolumes)
-          launch("volume", [&]() { return hpt_volume(S, P, q, shadeBlocks, travStats ? 1 : 0, stream); });
-        launch("shade", [&]() { return hpt_shade(S, b == F.nrcBounce ? Fp : FpNo, P, q, shadeBlocks, stream); });
+          launch(nm("volume").c_str(), [&]() { return hpt_volume(S, P, q, shadeBlocks, travStats ? 1 : 0, stream); });
+        launch(nm("shade").c_str(),
+            [&]() {
+              const bool cacheVertex = b == F.nrcBounce || (F.nrcSelfTrain && b == F.nrcBounce + 1);
+              return hpt_shade(S, cacheVertex ? Fp : FpNo, P, q, shadeBlocks, stream);
+            });
         if (F.numFirstLightSamples > 0) {
-          launch("nee", [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
-          launch("shadow", [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
-          launch("shadow_trans",
+          launch(nm("nee").c_str(), [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
+          launch(nm("shadow").c_str(),
+              [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
+          launch(nm("shadow_trans").c_str(),
               [&]() { return hpt_shadow_trans(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
-          launch("resolve", [&]() { return hpt_resolve(S, Fp, P, q, shadeBlocks, stream); });
+          launch(nm("resolve").c_str(), [&]() { return hpt_resolve(S, Fp, P, q, shadeBlocks, stream); });
         }
         q ^= 1;
       }

@@ -349,7 +349,7 @@ __device__ __forceinline__ int nextBatch(int32_t *counter)
 // closest hit of every queued path (the rtcIntersectV of
 // traceGeometryRayIntervals, World.ih:247-286)
 template <bool STATS>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int qin)
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_extend(DScene S, DPaths P, int qin)
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[2 * qin];
@@ -363,19 +363,59 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, DPaths P, int 
       const float4 o = ld4(P.org + 4 * (size_t)p);
       const float4 d = ld4(P.dir + 4 * (size_t)p);
       Hit h;
-      bool sawT;
-      traceRay<MODE_CLOSEST, STATS, false, true>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds, ovf, sawT,
-          st);
+      bool tie;
+      traceRay<MODE_CLOSEST, STATS, false, TIES_REPORT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds,
+          ovf, tie, st);
       st4(P.hitA + 4 * (size_t)p, h.hit ? h.t : kInf, h.u, h.v, __uint_as_float(h.hit ? h.inst : kInvalidID));
       st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
       P.hitPrim[p] = h.prim | (h.half << 31);
       rays++;
+      if (tie) // rare: k_ties re-resolves the hit in FP64
+        P.tieQueue[atomicAdd(&P.counters[kCntTies], 1)] = p;
     }
   }
   if (STATS) {
     waveAdd(&P.stats[0], st.nodes);
     waveAdd(&P.stats[1], st.prims);
     waveAdd(&P.stats[2], rays);
+  }
+}
+
+// FP64 re-resolution of the extension hits k_extend found near-tied
+// (resolveNearTie): every hit within the window of the float closest hit
+// gets its FP64 distance, the smallest (FP64 t, key) replaces the record.
+template <bool STATS>
+__global__ __launch_bounds__(kTraceBlock) void k_ties(DScene S, DPaths P)
+{
+  __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
+  const int n = P.counters[kCntTies];
+  uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
+  TraceStats st = {0, 0};
+  for (int i = globalId(); i < n; i += gridStride()) {
+    const uint32_t p = P.tieQueue[i];
+    const float4 o = ld4(P.org + 4 * (size_t)p);
+    const float4 d = ld4(P.dir + 4 * (size_t)p);
+    const float4 a = ld4(P.hitA + 4 * (size_t)p);
+    const float4 b = ld4(P.hitB + 4 * (size_t)p);
+    const uint32_t hp = P.hitPrim[p];
+    Hit h;
+    h.hit = true;
+    h.t = a.x;
+    h.u = a.y;
+    h.v = a.z;
+    h.inst = __float_as_uint(a.w);
+    h.Ng = v3(b.x, b.y, b.z);
+    h.model = __float_as_uint(b.w);
+    h.prim = hp & 0x7fffffffu;
+    h.half = hp >> 31;
+    resolveNearTie<STATS>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds, ovf, st);
+    st4(P.hitA + 4 * (size_t)p, h.t, h.u, h.v, __uint_as_float(h.inst));
+    st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
+    P.hitPrim[p] = h.prim | (h.half << 31);
+  }
+  if (STATS) {
+    waveAdd(&P.stats[0], st.nodes);
+    waveAdd(&P.stats[1], st.prims);
   }
 }
 
@@ -576,22 +616,27 @@ __global__ __launch_bounds__(256, VOL ? 2 : 3) void k_shade(DScene S, DFrame F, 
         BSDF b;
         if (!volHit && !getBSDF(S, dg, S.media[medium], b))
           break;
-        if (NRC && depth == (uint32_t)F.nrcBounce) {
+        const bool suffixQuery = NRC && (flags & FLAG_NRC_TRAIN) && depth == (uint32_t)F.nrcBounce + 1u;
+        if (NRC && (depth == (uint32_t)F.nrcBounce || suffixQuery)) {
           // the radiance cache at this vertex (before its NEE): position,
           // direction towards the previous vertex, shading normal, albedo
           // and the throughput.  The path ends here with a cache query;
-          // a training slot runs on and becomes a training record.
+          // a training slot runs on and becomes a training record (with
+          // self-training, its own query one vertex later ends it).
+          const bool query = suffixQuery || (p + F.nrcTrainOffset) % F.nrcTrainStride != 0;
           const vec3f alb = volHit ? hit.Ng : b.albedo;
-          float *rec = P.nrcRec + 16 * (size_t)p;
+          float *rec = (query ? P.nrcRec : P.nrcTrainRec) + 16 * (size_t)p;
           st4(rec, dg.P.x, dg.P.y, dg.P.z, thr.x);
           st4(rec + 4, -rdir.x, -rdir.y, -rdir.z, thr.y);
           st4(rec + 8, dg.Ns.x, dg.Ns.y, dg.Ns.z, thr.z);
           st4(rec + 12, alb.x, alb.y, alb.z, 0.f);
-          if ((p + F.nrcTrainOffset) % F.nrcTrainStride != 0) {
+          if (query) {
             nrcQuery = true;
             break;
           }
           nrcTrain = true;
+          if (F.nrcSelfTrain)
+            flags |= FLAG_NRC_TRAIN;
           st4(P.nrcBefore + 4 * (size_t)p, contribution.x, contribution.y, contribution.z, 0.f);
         }
         ldNextGroup(lds);
@@ -717,10 +762,10 @@ __global__ __launch_bounds__(256, VOL ? 2 : 3) void k_shade(DScene S, DFrame F, 
 
 // radiance cache: the queued vertex records as raw cache inputs
 // (position, direction, normal, albedo)
-__global__ __launch_bounds__(256) void k_nrc_gather(DPaths P, const uint32_t *queue, int n, float *raw12)
+__global__ __launch_bounds__(256) void k_nrc_gather(const float *records, const uint32_t *queue, int n, float *raw12)
 {
   for (int i = globalId(); i < n; i += gridStride()) {
-    const float *rec = P.nrcRec + 16 * (size_t)queue[i];
+    const float *rec = records + 16 * (size_t)queue[i];
     float *o = raw12 + 12 * (size_t)i;
     for (int k = 0; k < 4; k++) {
       o[3 * k] = rec[4 * k];
@@ -748,7 +793,7 @@ __global__ __launch_bounds__(256) void k_nrc_targets(DPaths P, int n, float *tar
 {
   for (int i = globalId(); i < n; i += gridStride()) {
     const uint32_t p = P.nrcTrainQueue[i];
-    const float *rec = P.nrcRec + 16 * (size_t)p;
+    const float *rec = P.nrcTrainRec + 16 * (size_t)p;
     const float *b = P.nrcBefore + 4 * (size_t)p;
     const float *c = P.contrib + 4 * (size_t)p;
     for (int k = 0; k < 3; k++) {
@@ -764,9 +809,74 @@ __global__ __launch_bounds__(256) void k_nrc_targets(DPaths P, int n, float *tar
 // records that can contribute go to the shadow queue.
 constexpr int kStage = 2048;
 
+// Writes the block's staged shadow-ray records to the shadow queue (one
+// atomic for the block), stably partitioned by the light they sample (the
+// light index modulo kLightBuckets): a wave of the occlusion pass then
+// traces rays from neighbouring vertices towards one light, which share
+// node fetches and branch alike.  Thread t owns the kStagePerThread
+// entries from t * kStagePerThread; the partition offsets come from an
+// exclusive scan of the per-(bucket, thread) counts in bucket-major order.
+// Results do not depend on the queue order.  Called by every thread of the
+// (256-thread) block.
+constexpr int kLightBuckets = 8;
+constexpr int kNeeBlock = 256;
+constexpr int kStagePerThread = kStage / kNeeBlock;
+static_assert(kStage % kNeeBlock == 0 && kLightBuckets <= 256, "stage layout");
+
+__device__ __attribute__((noinline)) void flushShadowStage(DPaths &P, const uint32_t *stage, const uint8_t *stageKey,
+    uint32_t cnt, uint32_t *table, uint32_t *partial, uint32_t &stageBase)
+{
+  // column t of table (entries b * 256 + t) is thread t's alone until the
+  // scan: its per-bucket counts
+  const int t = threadIdx.x;
+  const uint32_t e0 = (uint32_t)t * kStagePerThread;
+  for (int b = 0; b < kLightBuckets; b++)
+    table[b * kNeeBlock + t] = 0;
+  for (int k = 0; k < kStagePerThread; k++)
+    if (e0 + k < cnt)
+      table[stageKey[e0 + k] * kNeeBlock + t]++;
+  __syncthreads();
+  // exclusive scan in bucket-major order: thread t sums the table entries
+  // [t * kLightBuckets, (t + 1) * kLightBuckets), the sums are scanned
+  // within each wave by shuffles, then over the waves
+  uint32_t run = 0;
+  for (int k = 0; k < kLightBuckets; k++)
+    run += table[t * kLightBuckets + k];
+  uint32_t v = run;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t u = __shfl_up(v, off);
+    if (laneId() >= off)
+      v += u;
+  }
+  if (laneId() == 63)
+    partial[t >> 6] = v;
+  __syncthreads();
+  uint32_t before = v - run;
+  for (int w = 0; w < (t >> 6); w++)
+    before += partial[w];
+  for (int k = 0; k < kLightBuckets; k++) {
+    const uint32_t x = table[t * kLightBuckets + k];
+    table[t * kLightBuckets + k] = before;
+    before += x;
+  }
+  if (t == 0)
+    stageBase = (uint32_t)atomicAdd(&P.counters[kCntShadow], (int)cnt);
+  __syncthreads();
+  // column t now holds thread t's first position in each bucket
+  for (int k = 0; k < kStagePerThread; k++)
+    if (e0 + k < cnt) {
+      uint32_t &pos = table[stageKey[e0 + k] * kNeeBlock + t];
+      P.shadowQueue[stageBase + pos] = stage[e0 + k];
+      pos++;
+    }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int qin)
 {
   __shared__ uint32_t stage[kStage];
+  __shared__ uint8_t stageKey[kStage];
+  __shared__ uint32_t table[kLightBuckets * kNeeBlock], partial[kNeeBlock / 64];
   __shared__ uint32_t stageCount, stageBase;
   if (threadIdx.x == 0)
     stageCount = 0;
@@ -775,7 +885,7 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
   for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
     const int i = b0 + (int)threadIdx.x;
     bool emit = false;
-    uint32_t r = 0;
+    uint32_t r = 0, light = 0;
     if (i < n) {
       r = P.neeSampleQueue[i];
       const uint32_t p = r / (uint32_t)F.maxRec;
@@ -797,6 +907,7 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
       float ss;
       const vec2f s = ldGet3LightSamples(lds, li, nls > 1, ss);
       const Sample1D sel = distribution1DSample(S.numLights, S.lightsCDF, 0, ss);
+      light = (uint32_t)sel.idx;
       const float selProb = sel.prob * (float)nls;
       LightSample ls = lightSample(S, S.lights[sel.idx], Pv, Ng, Ns, s);
       ls.weight = ls.weight / selProb;
@@ -835,8 +946,11 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
       if (laneId() == 0 && mask)
         wpos = atomicAdd(&stageCount, (uint32_t)__popcll(mask));
       wpos = __shfl(wpos, 0);
-      if (emit)
-        stage[wpos + __popcll(mask & ((1ull << laneId()) - 1ull))] = r;
+      if (emit) {
+        const uint32_t at = wpos + __popcll(mask & ((1ull << laneId()) - 1ull));
+        stage[at] = r;
+        stageKey[at] = (uint8_t)(light % (uint32_t)kLightBuckets);
+      }
     }
     __syncthreads();
     // every wave reads the count before any wave can append again in the
@@ -844,24 +958,15 @@ __global__ __launch_bounds__(256) void k_nee(DScene S, DFrame F, DPaths P, int q
     const uint32_t cnt = stageCount;
     __syncthreads();
     if (cnt > (uint32_t)(kStage - blockDim.x)) {
-      if (threadIdx.x == 0)
-        stageBase = (uint32_t)atomicAdd(&P.counters[kCntShadow], (int)cnt);
-      __syncthreads();
-      for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x)
-        P.shadowQueue[stageBase + k] = stage[k];
-      __syncthreads();
+      flushShadowStage(P, stage, stageKey, cnt, table, partial, stageBase);
       if (threadIdx.x == 0)
         stageCount = 0;
       __syncthreads();
     }
   }
-  if (stageCount > 0) {
-    if (threadIdx.x == 0)
-      stageBase = (uint32_t)atomicAdd(&P.counters[kCntShadow], (int)stageCount);
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < stageCount; k += blockDim.x)
-      P.shadowQueue[stageBase + k] = stage[k];
-  }
+  const uint32_t cnt = stageCount;
+  if (cnt > 0)
+    flushShadowStage(P, stage, stageKey, cnt, table, partial, stageBase);
 }
 
 // transparentShadow (TransparentShadow.ispc:18-85) per NEE record, first
@@ -1320,7 +1425,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DScene S, const floa
     const float *r = rays8 + 8 * (size_t)i;
     Hit h;
     bool sawT;
-    traceRay<MODE_CLOSEST, true, false, true>(S, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7], h, lds, ovf,
+    traceRay<MODE_CLOSEST, true, false, TIES_RESOLVE>(S, v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7], h, lds, ovf,
         sawT, st);
     outTUV[3 * (size_t)i] = h.hit ? h.t : kInf;
     outTUV[3 * (size_t)i + 1] = h.u;
@@ -1433,6 +1538,17 @@ hipError_t hpt_extend(const HptScene *S, const HptPaths *P, int queue, int resid
   return hipGetLastError();
 }
 
+hipError_t hpt_ties(const HptScene *S, const HptPaths *P, int residentBlocks, int stats, hipStream_t s)
+{
+  const DScene &sc = *reinterpret_cast<const DScene *>(S);
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  if (stats)
+    hipLaunchKernelGGL(k_ties<true>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, p);
+  else
+    hipLaunchKernelGGL(k_ties<false>, dim3(residentBlocks), dim3(kTraceBlock), 0, s, sc, p);
+  return hipGetLastError();
+}
+
 hipError_t hpt_shade(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
 {
   const DScene &sc = *reinterpret_cast<const DScene *>(S);
@@ -1460,8 +1576,8 @@ hipError_t hpt_nrc_gather(const HptPaths *P, int which, int n, float *raw12, hip
   const DPaths &p = *reinterpret_cast<const DPaths *>(P);
   if (n <= 0)
     return hipSuccess;
-  hipLaunchKernelGGL(k_nrc_gather, dim3(gridFor(n, 256, 8192)), dim3(256), 0, s, p,
-      which ? p.nrcTrainQueue : p.nrcQueryQueue, n, raw12);
+  hipLaunchKernelGGL(k_nrc_gather, dim3(gridFor(n, 256, 8192)), dim3(256), 0, s,
+      (const float *)(which ? p.nrcTrainRec : p.nrcRec), which ? p.nrcTrainQueue : p.nrcQueryQueue, n, raw12);
   return hipGetLastError();
 }
 
@@ -1485,7 +1601,7 @@ hipError_t hpt_nrc_targets(const HptPaths *P, int n, float *target3, hipStream_t
 
 hipError_t hpt_nee(const HptScene *S, const HptFrame *F, const HptPaths *P, int queue, int blocks, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_nee, dim3(blocks), dim3(256), 0, s, *reinterpret_cast<const DScene *>(S),
+  hipLaunchKernelGGL(k_nee, dim3(blocks), dim3(kNeeBlock), 0, s, *reinterpret_cast<const DScene *>(S),
       *reinterpret_cast<const DFrame *>(F), *reinterpret_cast<const DPaths *>(P), queue);
   return hipGetLastError();
 }

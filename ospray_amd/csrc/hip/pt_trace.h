@@ -448,11 +448,18 @@ __device__ void resolveNearTie(const DScene &S, vec3f worg, vec3f wdir, float t0
     best = w;
 }
 
-template <int MODE, bool STATS, bool RECORD = false, bool TIES = false>
+// near-tie handling of a closest-hit traversal (see the header comment):
+// off; re-resolved in FP64 at the end of the traversal; or only reported
+// (in sawTransparent, unused by closest-hit traversals otherwise) for a
+// later pass to re-resolve with resolveNearTie, which keeps the FP64 code
+// and its registers out of the traversal kernel
+constexpr int TIES_OFF = 0, TIES_RESOLVE = 1, TIES_REPORT = 2;
+
+template <int MODE, bool STATS, bool RECORD = false, int TIES = TIES_OFF>
 __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
     uint32_t *lds, uint32_t *ovf, bool &sawTransparent, TraceStats &st)
 {
-  bool tie = false; // a near-tie of the current closest hit has been seen (TIES)
+  bool tie = false; // a near-tie of the current closest hit has been seen (TIES != TIES_OFF)
   const int lane = threadIdx.x;
   best.hit = false;
   best.t = tfar;
@@ -497,7 +504,7 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
         if (STATS)
           st.nodes++;
         // with TIES, boxes up to a near-tie beyond the closest hit stay open
-        const float tb = TIES && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
+        const float tb = TIES != TIES_OFF && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
         float t0c = slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tb);
         float t1c = slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tb);
         float t2c = slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tb);
@@ -596,7 +603,7 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
       vec3f Ng;
       bool h;
       uint32_t half = 0;
-      const float tmaxP = TIES && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
+      const float tmaxP = TIES != TIES_OFF && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
       if (kind == PRIM_SPHERE) {
         h = intersectSphere(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, tmaxP, t, Ng);
       } else {
@@ -621,9 +628,9 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
           recordHit(ovf, t, (uint32_t)inst, first + k, 0);
         continue;
       }
-      const bool nt = TIES && best.hit && nearTie(t, best.t);
+      const bool nt = TIES != TIES_OFF && best.hit && nearTie(t, best.t);
       if (better(t, (uint32_t)inst, model, primID, half, best)) {
-        if (TIES)
+        if (TIES != TIES_OFF)
           tie = nt; // a hit below the window of the previous closest one clears the flag
         best.hit = true;
         best.t = t;
@@ -641,8 +648,10 @@ __device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, floa
     if (cur == kDone)
       break;
   }
-  if (TIES && tie)
+  if (TIES == TIES_RESOLVE && tie)
     resolveNearTie<STATS>(S, worg, wdir, t0, tfar, best, lds, ovf, st);
+  if (TIES == TIES_REPORT)
+    sawTransparent = tie;
   return best.hit;
 }
 

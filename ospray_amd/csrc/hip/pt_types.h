@@ -232,6 +232,7 @@ struct DPaths
   uint32_t *shadowQueue;
   uint32_t *transQueue; // shadow records that hit only non-opaque surfaces
   uint32_t *neeSampleQueue; // record slots (path * maxRec + sample) to evaluate
+  uint32_t *tieQueue; // path slots whose extension hit is re-resolved in FP64
   // queue counters, see kCnt*: the pair (2q, 2q+1) is one 64-bit word so
   // that k_shade appends to extension queue q and to the NEE sample queue
   // with a single atomic
@@ -246,7 +247,8 @@ struct DPaths
   // radiance cache (when enabled): per path slot the vertex record
   // (P, thr.x | wo, thr.y | Ns, thr.z | albedo, -) and the contribution
   // before the vertex; compact queues of the query and training slots
-  float *nrcRec; // 16 per path
+  float *nrcRec; // 16 per path: the query vertex
+  float *nrcTrainRec; // 16 per path: the training vertex (self-training)
   float *nrcBefore; // 4 per path
   uint32_t *nrcQueryQueue;
   uint32_t *nrcTrainQueue;
@@ -273,10 +275,11 @@ constexpr int kCntFetchExtend = 5; // work fetch of the extension traversal
 constexpr int kCntFetchShadow = 6; // work fetch of the shadow traversal
 constexpr int kCntTrans = 7; // records for the transparency pass
 constexpr int kCntFetchTrans = 8; // work fetch of the transparency pass
-constexpr int kNumBounceCounters = 9; // the counters above are reset every bounce
-constexpr int kCntNrcQuery = 9; // radiance-cache queries of the pass
-constexpr int kCntNrcTrain = 10; // radiance-cache training records of the pass
-constexpr int kNumCounters = 11;
+constexpr int kCntTies = 9; // extension rays whose closest hit has a near-tie
+constexpr int kNumBounceCounters = 10; // the counters above are reset every bounce
+constexpr int kCntNrcQuery = 10; // radiance-cache queries of the pass
+constexpr int kCntNrcTrain = 11; // radiance-cache training records of the pass
+constexpr int kNumCounters = 12;
 
 // path flags in ld[3]
 constexpr uint32_t FLAG_DEPTH_MASK = 0xffu; // bits 0-7
@@ -284,6 +287,7 @@ constexpr uint32_t FLAG_SCAT_SHIFT = 8u; // bits 8-15 scattering events
 constexpr uint32_t FLAG_FIRST_BOUNCE_LIGHT = 1u << 16;
 constexpr uint32_t FLAG_STRAIGHT = 1u << 17;
 constexpr uint32_t FLAG_SPEC_TRANS_PATH = 1u << 18;
+constexpr uint32_t FLAG_NRC_TRAIN = 1u << 19; // a radiance-cache training path past its record vertex
 constexpr uint32_t FLAG_MEDIUM_SHIFT = 20u; // bits 20-31 medium index
 
 struct DFrame
@@ -335,9 +339,13 @@ struct DFrame
   int32_t stats; // count traversal and delta-tracking work into P.stats
   // radiance cache: paths end at the vertex of depth nrcBounce (-1: off)
   // with a cache query, except every nrcTrainStride-th slot (offset
-  // nrcTrainOffset), which runs on and becomes a training record
+  // nrcTrainOffset), which runs on and becomes a training record.  With
+  // nrcSelfTrain a training path ends one vertex later with a cache query
+  // of its own (its target is the radiance gathered up to there plus the
+  // cache's estimate beyond); otherwise it runs to its end.
   int32_t nrcBounce;
   uint32_t nrcTrainStride, nrcTrainOffset;
+  int32_t nrcSelfTrain;
   // Frames of a captured batch of views on accumulating framebuffers read
   // their frame index from device memory (*frameState, written before each
   // launch of the graph), and derive from it what eager frames get from the

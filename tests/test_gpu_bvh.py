@@ -97,7 +97,7 @@ def test_device_bvh_hits_equal_host_and_oracle(hip_device, name):
         sh = scenes.to_ospray(sc)
         _, nhost, _ = A.world_build(sh.world)
     finally:
-        _host_bvh(False)
+        _host_bvh(True)
     assert ndev >= 1 and nhost == 0
     assert 0 < need <= 128
     rays = _random_rays(sc, 20000, 11)
@@ -131,6 +131,6 @@ def test_device_bvh_renders_like_host(hip_device):
                        + [A.map_channel(fb, c, sc.width, sc.height)
                           for c in (K.OSP_FB_DEPTH, K.OSP_FB_ID_PRIMITIVE, K.OSP_FB_ID_OBJECT)])
         finally:
-            _host_bvh(False)
+            _host_bvh(True)
     for a, b in zip(*out):
         assert np.array_equal(a, b)
