@@ -2,6 +2,13 @@
 // files each part follows; function-level citations below.
 #include "orc_render.h"
 
+// ORC_PLANT = k builds the oracle with planted error k (a plausible mistake
+// in one pinned function); tests/test_oracle_planted.py builds each one and
+// checks that the pinning tests reject it.  0: no planted error.
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -64,7 +71,8 @@ inline vec3f uniformSampleSphere(float radius, vec2f s) // sampling.ih:191-197
 inline vec3f uniformSampleCone(float cosAngle, vec2f s) // sampling.ih:92-97
 {
   const float phi = kTwoPi * s.x;
-  const float cosTheta = 1.0f - s.y * (1.0f - cosAngle);
+  const float u = ORC_PLANT == 5 ? s.y * s.y : s.y; // plant 5: cos(theta) not uniform
+  const float cosTheta = 1.0f - u * (1.0f - cosAngle);
   return cartesian(phi, cos2sin(cosTheta), cosTheta);
 }
 inline float rcp_safe(float f) { return rcp(std::fabs(f) < 1e-8f ? std::copysign(1e-8f, f) : f); }
@@ -362,7 +370,7 @@ void setupCamera(CameraData &c, const OrcCamera &d) // PerspectiveCamera.cpp:30-
   const vec3f pos = v3(d.position[0], d.position[1], d.position[2]);
   const vec3f dir = v3(d.direction[0], d.direction[1], d.direction[2]);
   const vec3f up = v3(d.up[0], d.up[1], d.up[2]);
-  const float imgY = 2.f * std::tan(deg2rad(0.5f * d.fovy));
+  const float imgY = 2.f * std::tan(deg2rad((ORC_PLANT == 1 ? 1.0f : 0.5f) * d.fovy)); // plant 1: full angle
   const float imgX = imgY * d.aspect;
   c.org = pos;
   c.direction = dir;
@@ -404,7 +412,7 @@ void setupRenderer(RendererData &r, const OrcRenderer &d) // Renderer.cpp:38-92,
     r.lutSize = n;
     r.lutCdfX.assign(n * n, 0.f);
     r.lutCdfY.assign(n, 0.f);
-    const float sigma = (0.5f * r.filterWidth) / 3.0f;
+    const float sigma = (0.5f * r.filterWidth) / (ORC_PLANT == 2 ? 2.0f : 3.0f); // plant 2: wrong sigma
     const float sigmaSqr = sigma * sigma;
     for (int y = 0; y < n; y++) {
       const float fy = (y + 0.5f) * (1.0f / (float)n);
@@ -1134,7 +1142,7 @@ static vec3f getTransparency(const Scene &s, const DG &dg, const Ray &ray, const
     return v3(m.transparency);
   if (m.type == ORC_MAT_GLASS) { // Glass.ispc:37-52
     const float eta = eq(medium, m.outside) ? m.outside.ior * rcp(m.inside.ior) : m.inside.ior * rcp(m.outside.ior);
-    const float cosThetaO = maxf(-dot(ray.dir, dg.Ns), 0.0f);
+    const float cosThetaO = ORC_PLANT == 6 ? 1.0f : maxf(-dot(ray.dir, dg.Ns), 0.0f); // plant 6: normal incidence
     return v3(1.0f - fresnelDielectric(cosThetaO, eta));
   }
   return v3(0.f);
@@ -1673,7 +1681,7 @@ static vec3f nextEventEstimation(const PathCtx &c, PathState &ps, PathVertex &pv
     float ss;
     const vec2f s = LDSampler_get3LightSamples(ps.ld, i, n > 1, ss);
     const Sample1D sel = Distribution1D_sample((int)c.numLights, c.lightsCDF, 0, ss);
-    const float selProb = sel.prob * (float)n;
+    const float selProb = ORC_PLANT == 4 ? sel.prob : sel.prob * (float)n; // plant 4: sample count dropped
     LightSample ls = lightSample(*c.scene, c.lights[sel.idx], pv.dg, s);
     ls.weight = ls.weight / selProb;
     ls.pdf *= selProb;
@@ -2081,7 +2089,7 @@ static void varianceKernel(FrameBuffer &fb)
           if (den2 > 0.f) {
             cnt++;
             const float diff = std::fabs(c.x - va.x) + std::fabs(c.y - va.y) + std::fabs(c.z - va.z) + std::fabs(c.w - va.w);
-            v += diff * rsqrt(den2);
+            v += ORC_PLANT == 3 ? diff : diff * rsqrt(den2); // plant 3: no normalisation
           }
         }
       }
