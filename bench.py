@@ -528,11 +528,14 @@ def main() -> int:
                 "traffic": traffic, "kernel": f"k_{dom}",
                 "frac_alg": round(frac_alg, 4),
                 "frac_dram": round(frac_dram, 4) if frac_dram is not None else None,
-                "alg_bytes": f"{NODE_BYTES} B per node visit + {PRIM_BYTES} B per primitive test + {RAY_BYTES} B "
-                             f"per ray",
-                "frac_alg_128B_node_record": round(d["record_gbs"] / HBM_PEAK_GBS, 4),
+                "alg_bytes": (f"{SAMPLE_BYTES} B per voxel sample + {CELL_BYTES} B per macrocell interval + "
+                              f"{VOL_RAY_BYTES} B per queued ray") if dom == "volume" else
+                             (f"{NODE_BYTES} B per node visit + {PRIM_BYTES} B per primitive test + {RAY_BYTES} B "
+                              f"per ray"),
                 "bytes_per_launch": int(d["bytes_per_frame"] / lpf),
                 "launch_ms": round(launch_ms, 4)}
+    if "record_gbs" in d:
+        roofline["frac_alg_128B_node_record"] = round(d["record_gbs"] / HBM_PEAK_GBS, 4)
     valu = valu_profile(f"k_{dom}")
     if valu:
         roofline["valu"] = valu

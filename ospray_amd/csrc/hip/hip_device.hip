@@ -2672,8 +2672,10 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       launch("reset", [&]() { return hpt_reset_counters(P, -1, stream); });
       launch("generate", [&]() { return hpt_generate(Fp, P, stream); });
       // with self-training no path goes beyond the vertex after the cache's
+      // (in worlds without volumes: training paths from volume vertices run
+      // to their end)
       int lastBounce = F.maxDepth;
-      if (F.nrcBounce >= 0 && F.nrcSelfTrain)
+      if (F.nrcBounce >= 0 && F.nrcSelfTrain && !F.hasVolumes)
         lastBounce = std::min(F.maxDepth, F.nrcBounce + 1);
       int q = 0;
       for (int b = 0; b <= lastBounce; b++) {
@@ -2690,7 +2692,10 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
               const bool cacheVertex = b == F.nrcBounce || (F.nrcSelfTrain && b == F.nrcBounce + 1);
               return hpt_shade(S, cacheVertex ? Fp : FpNo, P, q, shadeBlocks, stream);
             });
-        if (F.numFirstLightSamples > 0) {
+        // the last bounce of a self-training radiance cache only ends paths
+        // at cache queries (no NEE there)
+        const bool suffixOnly = F.nrcBounce >= 0 && F.nrcSelfTrain && !F.hasVolumes && b == F.nrcBounce + 1;
+        if (F.numFirstLightSamples > 0 && !suffixOnly) {
           launch(nm("nee").c_str(), [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
           launch(nm("shadow").c_str(),
               [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });

@@ -353,6 +353,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(5))
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[2 * qin];
+  if (n == 0) // an empty queue: no work fetch
+    return;
   uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
@@ -635,7 +637,10 @@ __global__ __launch_bounds__(256, VOL ? 2 : 3) void k_shade(DScene S, DFrame F, 
             break;
           }
           nrcTrain = true;
-          if (F.nrcSelfTrain)
+          // a training path from a volume vertex runs to its end: through a
+          // medium whose albedo is near one, a target that includes the
+          // cache's own estimate feeds the cache's bias back into it
+          if (F.nrcSelfTrain && !volHit)
             flags |= FLAG_NRC_TRAIN;
           st4(P.nrcBefore + 4 * (size_t)p, contribution.x, contribution.y, contribution.z, 0.f);
         }
@@ -978,6 +983,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[kCntShadow];
+  if (n == 0) // an empty queue: no work fetch
+    return;
   uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
@@ -1042,6 +1049,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow_trans(DScene S, DFrame F
 {
   __shared__ uint32_t lds[kLdsDepth * kTraceBlock];
   const int n = P.counters[kCntTrans];
+  if (n == 0) // an empty queue: no work fetch
+    return;
   uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
