@@ -29,9 +29,23 @@ typedef struct HptPaths HptPaths; /* wavefront path state, queues and counters *
 #endif
 
 /* keepIndex -1: sets every counter to 0.  keepIndex 0 or 1: sets the
- * per-bounce queue counters to 0 except the entry count of extension queue
- * `keepIndex` (the radiance-cache counters of the pass are kept). */
+ * per-bounce queue counters to 0 except the two words of extension queue
+ * `keepIndex` (its entry count and the NEE samples appended with it; the
+ * radiance-cache counters of the pass are kept).  A cancelled frame (the
+ * host wrote the frame's generation into the paths' cancellation word)
+ * also empties the kept queue and latches the cancellation. */
 HPT_API hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s);
+/* The parts of the per-bounce reset, for a frame whose shadow-ray stages
+ * (hpt_nee .. hpt_resolve) of bounce b run on a second stream beside the
+ * extension of bounce b+1: before the extension, its work fetch and the
+ * near-tie queue (with the cancellation check); after bounce b's resolve
+ * (which still reads the queue it shaded), the two words of the queue the
+ * shading of bounce b+1 appends to (extension queue keepIndex ^ 1); on the
+ * second stream before bounce b+1's NEE, the shadow records, the
+ * transparency queue and their work fetch. */
+HPT_API hipError_t hpt_reset_extension_counters(const HptPaths *P, int keepIndex, hipStream_t s);
+HPT_API hipError_t hpt_reset_append_counters(const HptPaths *P, int keepIndex, hipStream_t s);
+HPT_API hipError_t hpt_reset_shadow_counters(const HptPaths *P, hipStream_t s);
 HPT_API hipError_t hpt_fill_u32(uint32_t *ptr, size_t n, uint32_t value, hipStream_t s);
 /* dst <- src, bytes a multiple of 16: a plain streaming copy that calibrates
  * bandwidth figures and HBM counters (no reference counterpart). */

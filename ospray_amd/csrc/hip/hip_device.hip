@@ -101,8 +101,25 @@ struct PathLane
   size_t pathSlots = 0, recSlots = 0;
   bool auxAllocated = false;
   float *auxBase = nullptr;
+  // the shadow-ray stages of a bounce run on `side` beside the next
+  // bounce's extension on `stream`, with traversal scratch rows of their
+  // own; evShade[b & 1] orders bounce b's shading before its shadow stages,
+  // evRes[b & 1] its NEE resolve before the shading of bounce b + 1
+  hipStream_t side = nullptr;
+  DevBuf sideOverflow;
+  hipEvent_t evShade[2] = {nullptr, nullptr}, evRes[2] = {nullptr, nullptr};
   ~PathLane()
   {
+    if (side)
+      (void)hipStreamSynchronize(side);
+    for (int k = 0; k < 2; k++) {
+      if (evShade[k])
+        (void)hipEventDestroy(evShade[k]);
+      if (evRes[k])
+        (void)hipEventDestroy(evRes[k]);
+    }
+    if (side)
+      (void)hipStreamDestroy(side);
     if (ownsStream && stream)
       (void)hipStreamDestroy(stream);
   }
@@ -1013,8 +1030,10 @@ class HipDevice : public Device
   int residentTraceBlocks = 0, shadeBlocks = 0;
   size_t pathBudgetBytes = (size_t)16 << 30;
   // profileBounces: the stage timers of the bounce loop are kept per bounce
-  // ("extend@2", ...)
-  bool profile = false, travStats = false, profileBounces = false;
+  // ("extend@2", ...).  overlapShadow: the shadow-ray stages of a bounce run
+  // on a second stream beside the next bounce's extension (eager frames of
+  // one lane).
+  bool profile = false, travStats = false, profileBounces = false, overlapShadow = true;
   // BLAS build: on the host (binned SAH, the default: about 10 % fewer node
   // visits per ray than the device build on the 4.2M-triangle terrain) or on
   // the device (Morton sort + PLOC, about 8x faster to commit); device
@@ -1073,6 +1092,7 @@ class HipDevice : public Device
     profile = getBool("profileKernels", false);
     travStats = getBool("traversalStats", false);
     profileBounces = getBool("profileBounces", false);
+    overlapShadow = getBool("overlapShadow", overlapShadow);
     traceBlocksPerCU = std::max(1, std::min(64, getInt("traceBlocksPerCU", traceBlocksPerCU)));
     shadeBlocksPerCU = std::max(1, std::min(64, getInt("shadeBlocksPerCU", shadeBlocksPerCU)));
     batchStreams = std::max(1, std::min(8, getInt("batchStreams", batchStreams)));
@@ -1161,6 +1181,26 @@ class HipDevice : public Device
       throw std::runtime_error("batched views: traversal scratch would grow while capturing");
     L.overflow.ensure(bytes);
     HIP_CHECK(hipMemset(L.overflow.ptr, 0, bytes));
+    if (L.side) { // the side stream's rows follow the grid size too
+      L.sideOverflow.ensure(bytes);
+      HIP_CHECK(hipMemset(L.sideOverflow.ptr, 0, bytes));
+    }
+  }
+
+  // the side stream of a lane, its scratch rows and events (first use)
+  hipStream_t ensureSide(PathLane &L)
+  {
+    if (!L.side) {
+      HIP_CHECK(hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking));
+      for (int k = 0; k < 2; k++) {
+        HIP_CHECK(hipEventCreateWithFlags(&L.evShade[k], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&L.evRes[k], hipEventDisableTiming));
+      }
+      const size_t bytes = (size_t)residentTraceBlocks * hpt_trace_block_size() * hpt_overflow_depth() * 4;
+      L.sideOverflow.ensure(bytes);
+      HIP_CHECK(hipMemset(L.sideOverflow.ptr, 0, bytes));
+    }
+    return L.side;
   }
 
   void commitObject(ManagedObject *obj) override
@@ -1184,19 +1224,22 @@ class HipDevice : public Device
   // grid of the near-tie pass: near-ties are rare, so a small resident grid
   // (at most the traversal grid, whose scratch rows it uses)
   int tieBlocks() const { return std::max(1, std::min(residentTraceBlocks, 2 * numCUs)); }
+  // enqueues fn (a launch on `on`, the lane's stream by default), timed on
+  // that stream with events when profiling
   template <typename F>
-  void launch(const char *name, F &&fn)
+  void launch(const char *name, F &&fn, hipStream_t on = nullptr)
   {
     if (!profile || capturing) {
       HIP_CHECK(fn());
       return;
     }
+    const hipStream_t st = on ? on : lane->stream;
     hipEvent_t a, b;
     HIP_CHECK(hipEventCreate(&a));
     HIP_CHECK(hipEventCreate(&b));
-    HIP_CHECK(hipEventRecord(a, lane->stream));
+    HIP_CHECK(hipEventRecord(a, st));
     HIP_CHECK(fn());
-    HIP_CHECK(hipEventRecord(b, lane->stream));
+    HIP_CHECK(hipEventRecord(b, st));
     pendingEvents.push_back({timerIndex(name), {a, b}});
   }
   void collectTimers()
@@ -2677,14 +2720,38 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
       int lastBounce = F.maxDepth;
       if (F.nrcBounce >= 0 && F.nrcSelfTrain && !F.hasVolumes)
         lastBounce = std::min(F.maxDepth, F.nrcBounce + 1);
+      // Shadow stages of bounce b on the lane's side stream, beside the
+      // extension of bounce b + 1 (they read and write disjoint path state;
+      // the side stream has its own traversal scratch rows and resets its
+      // own counters).  The shading of bounce b + 1 (and the free-path
+      // sampling, which shares the paths' volume sampler with the
+      // transmittance estimates) waits for bounce b's resolve, so every
+      // contribution is added in the same order as on one stream.
+      const bool overlap = overlapShadow && nl == 1 && !capturing && !inBatch && F.numFirstLightSamples > 0;
+      const hipStream_t side = overlap ? ensureSide(*lane) : nullptr;
+      hpt::DPaths sidePaths = lane->paths;
+      if (overlap)
+        sidePaths.overflow = lane->sideOverflow.as<uint32_t>();
+      const HptPaths *PS = overlap ? reinterpret_cast<const HptPaths *>(&sidePaths) : P;
+      const hipStream_t shadowStream = overlap ? side : stream;
+      int pendingRes = -1; // bounce whose resolve the next shading must wait for
       int q = 0;
       for (int b = 0; b <= lastBounce; b++) {
         const std::string at = profile && profileBounces ? "@" + std::to_string(b) : std::string();
         auto nm = [&](const char *stage) { return at.empty() ? std::string(stage) : stage + at; };
-        launch(nm("reset").c_str(), [&]() { return hpt_reset_counters(P, q, stream); });
+        if (overlap)
+          launch(nm("reset").c_str(), [&]() { return hpt_reset_extension_counters(P, q, stream); });
+        else
+          launch(nm("reset").c_str(), [&]() { return hpt_reset_counters(P, q, stream); });
         launch(nm("extend").c_str(),
             [&]() { return hpt_extend(S, P, q, residentTraceBlocks, travStats ? 1 : 0, stream); });
         launch(nm("ties").c_str(), [&]() { return hpt_ties(S, P, tieBlocks(), travStats ? 1 : 0, stream); });
+        if (pendingRes >= 0) {
+          HIP_CHECK(hipStreamWaitEvent(stream, lane->evRes[pendingRes & 1], 0));
+          pendingRes = -1;
+        }
+        if (overlap)
+          launch(nm("reset").c_str(), [&]() { return hpt_reset_append_counters(P, q, stream); });
         if (F.hasVolumes)
           launch(nm("volume").c_str(), [&]() { return hpt_volume(S, P, q, shadeBlocks, travStats ? 1 : 0, stream); });
         launch(nm("shade").c_str(),
@@ -2696,15 +2763,29 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         // at cache queries (no NEE there)
         const bool suffixOnly = F.nrcBounce >= 0 && F.nrcSelfTrain && !F.hasVolumes && b == F.nrcBounce + 1;
         if (F.numFirstLightSamples > 0 && !suffixOnly) {
-          launch(nm("nee").c_str(), [&]() { return hpt_nee(S, Fp, P, q, shadeBlocks, stream); });
+          if (overlap) {
+            HIP_CHECK(hipEventRecord(lane->evShade[b & 1], stream));
+            HIP_CHECK(hipStreamWaitEvent(side, lane->evShade[b & 1], 0));
+            launch(nm("reset").c_str(), [&]() { return hpt_reset_shadow_counters(PS, side); }, side);
+          }
+          launch(nm("nee").c_str(), [&]() { return hpt_nee(S, Fp, PS, q, shadeBlocks, shadowStream); }, shadowStream);
           launch(nm("shadow").c_str(),
-              [&]() { return hpt_shadow(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
+              [&]() { return hpt_shadow(S, Fp, PS, residentTraceBlocks, travStats ? 1 : 0, shadowStream); },
+              shadowStream);
           launch(nm("shadow_trans").c_str(),
-              [&]() { return hpt_shadow_trans(S, Fp, P, residentTraceBlocks, travStats ? 1 : 0, stream); });
-          launch(nm("resolve").c_str(), [&]() { return hpt_resolve(S, Fp, P, q, shadeBlocks, stream); });
+              [&]() { return hpt_shadow_trans(S, Fp, PS, residentTraceBlocks, travStats ? 1 : 0, shadowStream); },
+              shadowStream);
+          launch(nm("resolve").c_str(), [&]() { return hpt_resolve(S, Fp, PS, q, shadeBlocks, shadowStream); },
+              shadowStream);
+          if (overlap) {
+            HIP_CHECK(hipEventRecord(lane->evRes[b & 1], side));
+            pendingRes = b;
+          }
         }
         q ^= 1;
       }
+      if (pendingRes >= 0) // join: the pass's last resolve
+        HIP_CHECK(hipStreamWaitEvent(stream, lane->evRes[pendingRes & 1], 0));
       if (F.nrcBounce >= 0) {
         const int64_t before = r->nrcRecords;
         nrcPass(r, w);

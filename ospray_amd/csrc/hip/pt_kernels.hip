@@ -187,12 +187,12 @@ __device__ __forceinline__ vec2f pixelFilterSample(const DFrame &F, vec2f s, con
 
 // ---- kernels ----------------------------------------------------------------------
 
-// Resets the queue counters of a pass (keep -1) or of a bounce (keep: the
-// extension queue being read).  A cancelled frame (the host wrote its
-// generation into *cancelFlag, or the latch is already set) latches
-// *cancelLatch and also empties the kept queue, so the frame's remaining
-// kernels find no work.  One wave: lane 0's reads are broadcast.
-__global__ void k_reset(int32_t *counters, int n, int32_t keep, const uint32_t *cancelFlag, uint32_t gen,
+// Zeroes the counters whose bit is set in `mask`.  A cancelled frame (the
+// host wrote its generation into *cancelFlag, or the latch is already set)
+// latches *cancelLatch and also zeroes counter `keep` (the entry count of
+// the extension queue being read), so the frame's remaining kernels find no
+// work.  One wave: lane 0's reads are broadcast.
+__global__ void k_reset(int32_t *counters, uint32_t mask, int32_t keep, const uint32_t *cancelFlag, uint32_t gen,
     uint32_t *cancelLatch)
 {
   const int i = globalId();
@@ -205,7 +205,7 @@ __global__ void k_reset(int32_t *counters, int n, int32_t keep, const uint32_t *
     }
     cancelled = __shfl(cancelled, 0);
   }
-  if (i < n && (i != keep || cancelled))
+  if (i < kNumCounters && (((mask >> i) & 1u) || (cancelled && i == keep)))
     counters[i] = 0;
   if (cancelled && i == 0)
     __hip_atomic_store(cancelLatch, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1502,14 +1502,53 @@ inline int gridFor(size_t n, int block, int maxBlocks)
 
 extern "C" {
 
+namespace {
+// counter masks: the shadow side of a bounce (records, transparency queue and
+// their work fetch) and the extension side of the bounce reading queue q
+// (the next queue's pair, the extension work fetch, the near-tie queue)
+constexpr uint32_t kShadowSideMask =
+    (1u << kCntShadow) | (1u << kCntFetchShadow) | (1u << kCntTrans) | (1u << kCntFetchTrans);
+inline uint32_t extensionSideMask(int q)
+{
+  return (3u << (2 * (q ^ 1))) | (1u << kCntFetchExtend) | (1u << kCntTies);
+}
+hipError_t launchReset(const DPaths &p, uint32_t mask, int keep, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, mask, keep, p.cancelFlag, p.cancelGen,
+      p.cancelLatch);
+  return hipGetLastError();
+}
+} // namespace
+
 hipError_t hpt_reset_counters(const HptPaths *P, int keepIndex, hipStream_t s)
 {
   const DPaths &p = *reinterpret_cast<const DPaths *>(P);
-  // a full reset clears every counter; the per-bounce reset only the queue
-  // counters of a bounce, keeping the extension queue being read
-  const int n = keepIndex < 0 ? kNumCounters : kNumBounceCounters;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, n, keepIndex < 0 ? -1 : 2 * keepIndex,
-      p.cancelFlag, p.cancelGen, p.cancelLatch);
+  // a full reset clears every counter; the per-bounce reset both sides of a
+  // bounce, keeping the pair of the extension queue being read
+  if (keepIndex < 0)
+    return launchReset(p, (1u << kNumCounters) - 1u, -1, s);
+  return launchReset(p, extensionSideMask(keepIndex) | kShadowSideMask, 2 * keepIndex, s);
+}
+
+hipError_t hpt_reset_extension_counters(const HptPaths *P, int keepIndex, hipStream_t s)
+{
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  return launchReset(p, (1u << kCntFetchExtend) | (1u << kCntTies), 2 * keepIndex, s);
+}
+
+hipError_t hpt_reset_append_counters(const HptPaths *P, int keepIndex, hipStream_t s)
+{
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, 3u << (2 * (keepIndex ^ 1)), -1,
+      (const uint32_t *)nullptr, 0u, (uint32_t *)nullptr);
+  return hipGetLastError();
+}
+
+hipError_t hpt_reset_shadow_counters(const HptPaths *P, hipStream_t s)
+{
+  const DPaths &p = *reinterpret_cast<const DPaths *>(P);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(kNumCounters), 0, s, p.counters, kShadowSideMask, -1,
+      (const uint32_t *)nullptr, 0u, (uint32_t *)nullptr);
   return hipGetLastError();
 }
 
