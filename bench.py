@@ -56,8 +56,13 @@ HBM_PEAK_GBS = 8000.0
 # is one 128-byte line; NODE_RECORD_BYTES gives that figure beside it.
 NODE_BYTES, PRIM_BYTES, RAY_BYTES = 64, 48, 56
 NODE_RECORD_BYTES = 128
-SAMPLE_BYTES, CELL_BYTES, VOL_RAY_BYTES = 32, 8, 80
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_pathtracer_1080p.md")
+# volume tracking: 32 B per voxel sample (8 corner densities of the
+# trilinear stencil), 8 B per macrocell interval (its majorant), 80 B per
+# extension ray k_volume takes (ray in, hit in, path state out) and 32 B per
+# transmittance estimate of a shadow record in k_resolve (its ray)
+SAMPLE_BYTES, CELL_BYTES, VOL_RAY_BYTES, TRACK_BYTES = 32, 8, 80, 32
+TRAFFIC_FILES = {"c1": os.path.join(ROOT, "profiles", "hbm_traffic_pathtracer_1080p.md"),
+                 "c4": os.path.join(ROOT, "profiles", "hbm_traffic_pathtracer_c4_volume.md")}
 VALU_FILE = os.path.join(ROOT, "profiles", "valu_traversal_pathtracer_1080p.md")
 
 
@@ -168,14 +173,15 @@ def cpu_baseline(sc, target_s: float) -> dict:
             "host": cpus, "reference_probe": probe}
 
 
-def traffic_per_launch(kernel: str):
-    """Measured HBM bytes per launch of `kernel` from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), or None."""
-    if not os.path.exists(TRAFFIC_FILE):
+def traffic_per_launch(kernel: str, workload: str):
+    """Measured HBM bytes per launch of `kernel` on `workload` from the
+    committed rocprofv3 counter summary of that workload (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = TRAFFIC_FILES.get(workload)
+    if path is None or not os.path.exists(path):
         return None
     pat = re.compile(r"^\|\s*" + re.escape(kernel) + r"\s*\|.*\|\s*hbm_bytes_per_launch=(\d+)\s*\|\s*$")
-    with open(TRAFFIC_FILE) as fh:
+    with open(path) as fh:
         for line in fh:
             m = pat.match(line.strip())
             if m:
@@ -514,10 +520,20 @@ def main() -> int:
                           "samples_per_frame": samples_v, "cells_per_frame": cells_v,
                           "tracks_per_frame": tracks_v, "gbs": per_frame_bytes / (ms_frame * 1e-3) / 1e9}
         log(f"volume counters: free path {vst[:3]}, transmittance {vst[3:]}")
+    if kstats.get("resolve", (0, 0.0))[0] and vst[5]:
+        # with volumes k_resolve runs the shadow records' transmittance estimates
+        launches, ms = kstats["resolve"]
+        samples_t, cells_t, tracks_t = vst[3:6]
+        per_frame_bytes = SAMPLE_BYTES * samples_t + CELL_BYTES * cells_t + TRACK_BYTES * tracks_t
+        ms_frame = ms / args.steps
+        kern["resolve"] = {"ms_per_frame": ms_frame, "launches_per_frame": launches / args.steps,
+                           "bytes_per_frame": per_frame_bytes, "rays_per_frame": 0,
+                           "samples_per_frame": samples_t, "cells_per_frame": cells_t,
+                           "tracks_per_frame": tracks_t, "gbs": per_frame_bytes / (ms_frame * 1e-3) / 1e9}
     dom = max(kern, key=lambda k: kern[k]["ms_per_frame"])
     d = kern[dom]
     lpf = d["launches_per_frame"]
-    traffic = traffic_per_launch(f"k_{dom}")
+    traffic = traffic_per_launch(f"k_{dom}", args.workload)
     launch_ms = d["ms_per_frame"] / lpf
     frac_alg = d["gbs"] / HBM_PEAK_GBS
     # measured DRAM bytes per launch (committed counter summary) over the
@@ -530,13 +546,15 @@ def main() -> int:
                 "frac_dram": round(frac_dram, 4) if frac_dram is not None else None,
                 "alg_bytes": (f"{SAMPLE_BYTES} B per voxel sample + {CELL_BYTES} B per macrocell interval + "
                               f"{VOL_RAY_BYTES} B per queued ray") if dom == "volume" else
+                             (f"{SAMPLE_BYTES} B per voxel sample + {CELL_BYTES} B per macrocell interval + "
+                              f"{TRACK_BYTES} B per transmittance estimate") if dom == "resolve" else
                              (f"{NODE_BYTES} B per node visit + {PRIM_BYTES} B per primitive test + {RAY_BYTES} B "
                               f"per ray"),
                 "bytes_per_launch": int(d["bytes_per_frame"] / lpf),
                 "launch_ms": round(launch_ms, 4)}
     if "record_gbs" in d:
         roofline["frac_alg_128B_node_record"] = round(d["record_gbs"] / HBM_PEAK_GBS, 4)
-    valu = valu_profile(f"k_{dom}")
+    valu = valu_profile(f"k_{dom}") if args.workload == "c1" else None
     if valu:
         roofline["valu"] = valu
     rays_total = sum(kern[k]["rays_per_frame"] for k in kern)
@@ -567,7 +585,7 @@ def main() -> int:
            "kernels_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats.items())},
            "traversal": {k: {"nodes_per_ray": round(v["nodes_per_ray"], 2), "prims_per_ray": round(v["prims_per_ray"], 2),
                              "rays_per_frame": v["rays_per_frame"], "gbs": round(v["gbs"], 1)}
-                         for k, v in kern.items() if k != "volume"},
+                         for k, v in kern.items() if "nodes_per_ray" in v},
            "roofline": roofline, "cpu_baseline": cpu}
     if args.workload == "c1" and not args.no_c3:
         out["c3"] = measure_c3(args, 1920, 1080)
@@ -577,6 +595,9 @@ def main() -> int:
         out["volume"] = {"free_path": {"samples": vst[0], "cells": vst[1], "tracks": vst[2],
                                        "ms_per_frame": round(v["ms_per_frame"], 3), "gbs": round(v["gbs"], 1)},
                          "transmittance": {"samples": vst[3], "cells": vst[4], "tracks": vst[5]}}
+        if "resolve" in kern:
+            out["volume"]["transmittance"].update(ms_per_frame=round(kern["resolve"]["ms_per_frame"], 3),
+                                                  gbs=round(kern["resolve"]["gbs"], 1))
     print(json.dumps(out), flush=True)
     return 0
 

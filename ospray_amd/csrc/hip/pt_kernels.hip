@@ -346,6 +346,87 @@ __device__ __forceinline__ int nextBatch(int32_t *counter)
   return __shfl(base, 0);
 }
 
+// Lanes of a wave that wait for a ray before a traversal round refills
+// them: extension rays (long closest-hit traversals) and shadow rays
+// (shorter, ended early by an occluder) are refilled at different levels
+#ifndef HPT_EXTEND_REFILL
+#define HPT_EXTEND_REFILL 16
+#endif
+#ifndef HPT_SHADOW_REFILL
+#define HPT_SHADOW_REFILL 48
+#endif
+constexpr int kExtendRefill = HPT_EXTEND_REFILL, kShadowRefill = HPT_SHADOW_REFILL;
+
+// Traversal of every entry of a queue with the lanes of each wave kept
+// busy: a lane whose ray has ended waits; once at least REFILL lanes
+// wait (any number once the whole wave waits), the rays that ended are
+// finished together and the waiting lanes take the next entries of the
+// wave's current batch (64 consecutive queue entries, fetched with one
+// atomic and read ahead with one coalesced load) before the next
+// traversal round.  load(entry, org, dir, t0, tfar) gives an entry's ray;
+// finish(ended, entry, best, occluded, state) runs on every lane of the
+// wave (`ended` on the lanes whose ray ended), so it may use wave-wide
+// appends.  Each ray's traversal is the one traceRay runs: the result does
+// not depend on which rays share a wave.
+template <int MODE, bool STATS, bool RECORD, int TIES, int REFILL, class Load, class Finish>
+__device__ __forceinline__ void traceQueue(const DScene &S, const uint32_t *queue, int n, int32_t *fetch, uint32_t *lds,
+    uint32_t *ovf, TraceStats &st, Load load, Finish finish)
+{
+  const int wl = laneId();
+  const unsigned long long ltMask = (1ull << wl) - 1ull;
+  int bBase = nextBatch(fetch), bNext = 0; // wave-uniform batch cursor
+  uint32_t bEntry = bBase + wl < n ? queue[bBase + wl] : 0u;
+  TraceState s;
+  s.cur = kTraceDone;
+  s.post = 0;
+  s.sp = 0;
+  Hit best;
+  uint32_t entry = 0;
+  bool busy = false, ended = false, occluded = false;
+  for (;;) {
+    unsigned long long idle = __ballot(!busy);
+    if (idle == ~0ull || (bBase < n && __popcll(idle) >= REFILL)) {
+      if (__any(ended))
+        finish(ended, entry, best, occluded, s);
+      ended = false;
+      while (idle != 0ull && bBase < n) {
+        const int bSize = n - bBase < 64 ? n - bBase : 64;
+        const int avail = bSize - bNext;
+        const int rank = __popcll(idle & ltMask);
+        const int src = bNext + rank;
+        const uint32_t e = __shfl(bEntry, src < 63 ? src : 63);
+        if (!busy && rank < avail) {
+          entry = e;
+          busy = true;
+          occluded = false;
+          vec3f org, dir;
+          float t0, tfar;
+          load(entry, org, dir, t0, tfar);
+          traceBegin(s, S, org, dir, t0, tfar, best);
+        }
+        const int waiting = __popcll(idle);
+        bNext += waiting < avail ? waiting : avail;
+        idle = __ballot(!busy);
+        if (bNext == bSize) {
+          bBase = nextBatch(fetch);
+          bNext = 0;
+          bEntry = bBase + wl < n ? queue[bBase + wl] : 0u;
+        }
+      }
+      if (idle == ~0ull) // the queue is exhausted and every lane waits
+        break;
+    }
+    traceDescend<TIES, STATS>(s, S, best, lds, ovf, st);
+    if (busy && (s.post == 0 ? s.cur == kTraceDone : traceLeaf<MODE, STATS, RECORD, TIES>(s, S, best, lds, ovf, st, occluded))) {
+      busy = false;
+      ended = true;
+      s.cur = kTraceDone;
+      s.post = 0;
+      s.sp = 0;
+    }
+  }
+}
+
 // closest hit of every queued path (the rtcIntersectV of
 // traceGeometryRayIntervals, World.ih:247-286)
 template <bool STATS>
@@ -358,24 +439,25 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(5))
   uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
-  for (int base = nextBatch(&P.counters[kCntFetchExtend]); base < n; base = nextBatch(&P.counters[kCntFetchExtend])) {
-    const int i = base + laneId();
-    if (i < n) {
-      const uint32_t p = P.queue[qin][i];
-      const float4 o = ld4(P.org + 4 * (size_t)p);
-      const float4 d = ld4(P.dir + 4 * (size_t)p);
-      Hit h;
-      bool tie;
-      traceRay<MODE_CLOSEST, STATS, false, TIES_REPORT>(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, kInf, h, lds,
-          ovf, tie, st);
-      st4(P.hitA + 4 * (size_t)p, h.hit ? h.t : kInf, h.u, h.v, __uint_as_float(h.hit ? h.inst : kInvalidID));
-      st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
-      P.hitPrim[p] = h.prim | (h.half << 31);
-      rays++;
-      if (tie) // rare: k_ties re-resolves the hit in FP64
-        P.tieQueue[atomicAdd(&P.counters[kCntTies], 1)] = p;
-    }
-  }
+  traceQueue<MODE_CLOSEST, STATS, false, TIES_REPORT, kExtendRefill>(S, P.queue[qin], n, &P.counters[kCntFetchExtend], lds, ovf, st,
+      [&](uint32_t p, vec3f &org, vec3f &dir, float &t0, float &tfar) {
+        const float4 o = ld4(P.org + 4 * (size_t)p);
+        const float4 d = ld4(P.dir + 4 * (size_t)p);
+        org = v3(o.x, o.y, o.z);
+        dir = v3(d.x, d.y, d.z);
+        t0 = o.w;
+        tfar = kInf;
+      },
+      [&](bool ended, uint32_t p, const Hit &h, bool, const TraceState &ts) {
+        if (!ended)
+          return;
+        st4(P.hitA + 4 * (size_t)p, h.hit ? h.t : kInf, h.u, h.v, __uint_as_float(h.hit ? h.inst : kInvalidID));
+        st4(P.hitB + 4 * (size_t)p, h.Ng.x, h.Ng.y, h.Ng.z, __uint_as_float(h.model));
+        P.hitPrim[p] = h.prim | (h.half << 31);
+        rays++;
+        if (ts.tie) // rare: k_ties re-resolves the hit in FP64
+          P.tieQueue[atomicAdd(&P.counters[kCntTies], 1)] = p;
+      });
   if (STATS) {
     waveAdd(&P.stats[0], st.nodes);
     waveAdd(&P.stats[1], st.prims);
@@ -988,52 +1070,54 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6))
   uint32_t *ovf = P.overflow + (size_t)globalId() * kThreadRow;
   TraceStats st = {0, 0};
   uint32_t rays = 0;
-  for (int base = nextBatch(&P.counters[kCntFetchShadow]); base < n; base = nextBatch(&P.counters[kCntFetchShadow])) {
-    const int i = base + laneId();
-    bool toTrans = false, sawT = false;
-    uint32_t r = 0;
-    if (i < n) {
-      r = P.shadowQueue[i];
-      const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
-      const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
-      Hit h;
-      const bool occluded = traceRay<MODE_OCCLUSION, STATS, true>(S, v3(o4.x, o4.y, o4.z), v3(d4.x, d4.y, d4.z), 0.f,
-          o4.w, h, lds, ovf, sawT, st);
-      rays++;
-      if (!occluded) {
-        if (sawT) {
-          toTrans = true;
-        } else {
-          const float4 c4 = ld4(P.recContrib + 4 * (size_t)r);
-          const vec3f lc = v3(c4.x, c4.y, c4.z);
-          if (reduce_max(lc) > 0) {
-            // with volumes, transmittance and MIS are applied in k_resolve
-            const vec3f res = S.numVolInsts > 0 ? lc : lc * d4.w;
-            st4(P.recResult + 4 * (size_t)r, res.x, res.y, res.z, 1.f);
+  traceQueue<MODE_OCCLUSION, STATS, true, TIES_OFF, kShadowRefill>(S, P.shadowQueue, n, &P.counters[kCntFetchShadow], lds, ovf, st,
+      [&](uint32_t r, vec3f &org, vec3f &dir, float &t0, float &tfar) {
+        const float4 o4 = ld4(P.recOrg + 4 * (size_t)r);
+        const float4 d4 = ld4(P.recDir + 4 * (size_t)r);
+        org = v3(o4.x, o4.y, o4.z);
+        dir = v3(d4.x, d4.y, d4.z);
+        t0 = 0.f;
+        tfar = o4.w;
+      },
+      [&](bool ended, uint32_t r, const Hit &, bool occluded, const TraceState &ts) {
+        bool toTrans = false;
+        const bool sawT = ended && ts.sawTransparent;
+        if (ended) {
+          rays++;
+          if (!occluded) {
+            if (sawT) {
+              toTrans = true;
+            } else {
+              const float4 c4 = ld4(P.recContrib + 4 * (size_t)r);
+              const vec3f lc = v3(c4.x, c4.y, c4.z);
+              if (reduce_max(lc) > 0) {
+                // with volumes, transmittance and MIS are applied in k_resolve
+                const vec3f res = S.numVolInsts > 0 ? lc : lc * P.recDir[4 * (size_t)r + 3];
+                st4(P.recResult + 4 * (size_t)r, res.x, res.y, res.z, 1.f);
+              }
+            }
           }
         }
-      }
-    }
-    const int slot = enqueue(&P.counters[kCntTrans], toTrans);
-    if (toTrans) {
-      P.transQueue[slot] = r;
-      if (slot < P.transCap) {
-        // hand the hits traceRay recorded to the transparency pass
-        const uint32_t nrec = *recordedCount(ovf);
-        if (nrec <= (uint32_t)kTransRecHits) {
-          const float4 *rec = recordedHits(ovf);
-          float4 *dst = reinterpret_cast<float4 *>(P.transHits) + (size_t)slot * kTransRecHits;
-          for (uint32_t k = 0; k < nrec; k++)
-            dst[k] = rec[k];
+        const int slot = enqueue(&P.counters[kCntTrans], toTrans);
+        if (toTrans) {
+          P.transQueue[slot] = r;
+          if (slot < P.transCap) {
+            // hand the hits the traversal recorded to the transparency pass
+            const uint32_t nrec = *recordedCount(ovf);
+            if (nrec <= (uint32_t)kTransRecHits) {
+              const float4 *rec = recordedHits(ovf);
+              float4 *dst = reinterpret_cast<float4 *>(P.transHits) + (size_t)slot * kTransRecHits;
+              for (uint32_t k = 0; k < nrec; k++)
+                dst[k] = rec[k];
+            }
+            P.transCount[slot] = nrec <= (uint32_t)kTransRecHits ? nrec : 0xffffffffu;
+          }
         }
-        P.transCount[slot] = nrec <= (uint32_t)kTransRecHits ? nrec : 0xffffffffu;
-      }
-    }
-    // the recorded-hit count is 0 when a traversal starts: the rows are
-    // cleared when allocated, and only rays that recorded hits reset it
-    if (sawT)
-      *recordedCount(ovf) = 0;
-  }
+        // the recorded-hit count is 0 when a traversal starts: the rows are
+        // cleared when allocated, and only rays that recorded hits reset it
+        if (sawT)
+          *recordedCount(ovf) = 0;
+      });
   if (STATS) {
     waveAdd(&P.stats[3], st.nodes);
     waveAdd(&P.stats[4], st.prims);

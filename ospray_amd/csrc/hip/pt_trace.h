@@ -455,203 +455,267 @@ __device__ void resolveNearTie(const DScene &S, vec3f worg, vec3f wdir, float t0
 // and its registers out of the traversal kernel
 constexpr int TIES_OFF = 0, TIES_RESOLVE = 1, TIES_REPORT = 2;
 
-template <int MODE, bool STATS, bool RECORD = false, int TIES = TIES_OFF>
-__device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
-    uint32_t *lds, uint32_t *ovf, bool &sawTransparent, TraceStats &st)
+// Per-lane state of one traversal.  A traversal runs as rounds of
+// traceDescend (inner nodes, speculative while-while) and traceLeaf (the
+// leaf the lane holds); traceRay runs one ray to its end, traceQueue (in
+// the kernels) keeps the lanes of a wave busy with rays from a queue.
+struct TraceState
 {
-  bool tie = false; // a near-tie of the current closest hit has been seen (TIES != TIES_OFF)
-  const int lane = threadIdx.x;
+  vec3f worg, wdir, winv; // the world ray
+  vec3f o, d, inv;        // the current (world or object) ray
+  TriPre pc;
+  float t0, tfar;
+  int inst, modelOffset, sp, cur, post;
+  bool tie;            // a near-tie of the current closest hit has been seen (TIES != TIES_OFF)
+  bool sawTransparent; // MODE_OCCLUSION: a non-opaque primitive was hit
+};
+
+// `cur` of a lane whose traversal has no node left
+constexpr int kTraceDone = 0x7ffffffe;
+
+__device__ __forceinline__ void traceBegin(
+    TraceState &s, const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best)
+{
   best.hit = false;
   best.t = tfar;
   best.u = best.v = 0.f;
   best.Ng = v3(0.f);
   best.inst = best.model = best.prim = kInvalidID;
   best.half = 0;
-  sawTransparent = false;
-  const vec3f winv = v3(safeInv(wdir.x), safeInv(wdir.y), safeInv(wdir.z));
-  vec3f o = worg, d = wdir, inv = winv;
-  TriPre pc = {0, 1, 2, 0.f, 0.f, 0.f};
-  int inst = -1;
-  int modelOffset = 0;
-  int sp = 0;
-  int cur = S.tlasRoot;
-  // callers check sp < kLdsDepth + kOverflowDepth before pushing
-  auto push = [&](int v) {
-    if (sp < kLdsDepth)
-      lds[sp * kTraceBlock + lane] = (uint32_t)v;
-    else
-      ovf[sp - kLdsDepth] = (uint32_t)v;
-    sp++;
-  };
-  auto pop = [&]() -> int {
-    sp--;
-    if (sp < kLdsDepth)
-      return (int)lds[sp * kTraceBlock + lane];
-    return (int)ovf[sp - kLdsDepth];
-  };
-  // speculative while-while traversal: a lane that reaches a leaf keeps it
-  // aside (post) and goes on with inner nodes until every lane of the wave
-  // holds a leaf; then the wave processes the leaves together.  A held BLAS
-  // leaf stops the lane at the instance's exit marker (it needs the object
-  // ray); a held instance leaf is entered from the world level, with the
-  // next world node pushed below the marker.
-  constexpr int kDone = 0x7ffffffe;
-  int post = 0; // a held leaf (leaves are negative), 0: none
+  s.worg = worg;
+  s.wdir = wdir;
+  s.winv = v3(safeInv(wdir.x), safeInv(wdir.y), safeInv(wdir.z));
+  s.o = worg;
+  s.d = wdir;
+  s.inv = s.winv;
+  s.pc = {0, 1, 2, 0.f, 0.f, 0.f};
+  s.t0 = t0;
+  s.tfar = tfar;
+  s.inst = -1;
+  s.modelOffset = 0;
+  s.sp = 0;
+  s.cur = S.tlasRoot;
+  s.post = 0;
+  s.tie = false;
+  s.sawTransparent = false;
+}
+
+// callers check sp < kLdsDepth + kOverflowDepth before pushing
+__device__ __forceinline__ void tracePush(TraceState &s, uint32_t *lds, uint32_t *ovf, int v)
+{
+  if (s.sp < kLdsDepth)
+    lds[s.sp * kTraceBlock + threadIdx.x] = (uint32_t)v;
+  else
+    ovf[s.sp - kLdsDepth] = (uint32_t)v;
+  s.sp++;
+}
+
+__device__ __forceinline__ int tracePop(TraceState &s, const uint32_t *lds, const uint32_t *ovf)
+{
+  s.sp--;
+  if (s.sp < kLdsDepth)
+    return (int)lds[s.sp * kTraceBlock + threadIdx.x];
+  return (int)ovf[s.sp - kLdsDepth];
+}
+
+// Inner nodes of the speculative while-while traversal: a lane that reaches
+// a leaf keeps it aside (post) and goes on with inner nodes until every
+// lane of the wave holds a leaf or has no node left.  A held BLAS leaf
+// stops the lane at the instance's exit marker (it needs the object ray); a
+// held instance leaf is entered from the world level, with the next world
+// node pushed below the marker.  Lanes with cur == kTraceDone and no held
+// leaf (finished or idle) only take part in the wave vote.
+template <int TIES, bool STATS>
+__device__ __forceinline__ void traceDescend(
+    TraceState &s, const DScene &S, const Hit &best, uint32_t *lds, uint32_t *ovf, TraceStats &st)
+{
   for (;;) {
-    for (;;) {
-      if (cur >= 0 && cur < kDone) {
-        const Node4 nd = loadNode(S.nodes, cur);
-        if (STATS)
-          st.nodes++;
-        // with TIES, boxes up to a near-tie beyond the closest hit stay open
-        const float tb = TIES != TIES_OFF && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
-        float t0c = slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tb);
-        float t1c = slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tb);
-        float t2c = slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tb);
-        float t3c = slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, t0, tb);
-        int c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
-        cswap(t0c, c0, t1c, c1);
-        cswap(t2c, c2, t3c, c3);
-        cswap(t0c, c0, t2c, c2);
-        cswap(t1c, c1, t3c, c3);
-        cswap(t1c, c1, t2c, c2);
-        if (t0c == kInf) {
-          cur = sp == 0 ? kDone : pop();
+    if (s.cur >= 0 && s.cur < kTraceDone) {
+      const Node4 nd = loadNode(S.nodes, s.cur);
+      if (STATS)
+        st.nodes++;
+      // with TIES, boxes up to a near-tie beyond the closest hit stay open
+      const float tb = TIES != TIES_OFF && best.hit ? fminf(best.t + kTieEps * best.t, s.tfar) : best.t;
+      float t0c = slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, s.o, s.inv, s.t0, tb);
+      float t1c = slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, s.o, s.inv, s.t0, tb);
+      float t2c = slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, s.o, s.inv, s.t0, tb);
+      float t3c = slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, s.o, s.inv, s.t0, tb);
+      int c0 = nd.child.x, c1 = nd.child.y, c2 = nd.child.z, c3 = nd.child.w;
+      cswap(t0c, c0, t1c, c1);
+      cswap(t2c, c2, t3c, c3);
+      cswap(t0c, c0, t2c, c2);
+      cswap(t1c, c1, t3c, c3);
+      cswap(t1c, c1, t2c, c2);
+      if (t0c == kInf) {
+        s.cur = s.sp == 0 ? kTraceDone : tracePop(s, lds, ovf);
+      } else {
+        const int npush = (t1c != kInf) + (t2c != kInf) + (t3c != kInf);
+        if (s.sp + npush > kLdsDepth + kOverflowDepth) { // cannot happen: the world commit checks the stack need
+          s.cur = kTraceDone;
+          s.sp = 0;
         } else {
-          const int npush = (t1c != kInf) + (t2c != kInf) + (t3c != kInf);
-          if (sp + npush > kLdsDepth + kOverflowDepth) // cannot happen: the world commit checks the stack need
-            return best.hit;
           if (t3c != kInf)
-            push(c3);
+            tracePush(s, lds, ovf, c3);
           if (t2c != kInf)
-            push(c2);
+            tracePush(s, lds, ovf, c2);
           if (t1c != kInf)
-            push(c1);
-          cur = c0;
+            tracePush(s, lds, ovf, c1);
+          s.cur = c0;
         }
-      } else if (cur == kMarker && post == 0) {
-        o = worg;
-        d = wdir;
-        inv = winv;
-        inst = -1;
-        cur = sp == 0 ? kDone : pop();
       }
-      if (cur < 0 && post == 0) { // hold the leaf, go on
-        post = cur;
-        cur = sp == 0 ? kDone : pop();
-      }
-      const bool canStep = (cur >= 0 && cur < kDone) || (cur == kMarker && post == 0);
-      if (__all(!canStep || post != 0))
-        break;
+    } else if (s.cur == kMarker && s.post == 0) {
+      s.o = s.worg;
+      s.d = s.wdir;
+      s.inv = s.winv;
+      s.inst = -1;
+      s.cur = s.sp == 0 ? kTraceDone : tracePop(s, lds, ovf);
     }
-    if (post == 0) { // nothing held: the lane has no work left
-      if (cur == kDone)
+    if (s.cur < 0 && s.post == 0) { // hold the leaf, go on
+      s.post = s.cur;
+      s.cur = s.sp == 0 ? kTraceDone : tracePop(s, lds, ovf);
+    }
+    const bool canStep = (s.cur >= 0 && s.cur < kTraceDone) || (s.cur == kMarker && s.post == 0);
+    if (__all(!canStep || s.post != 0))
+      break;
+  }
+}
+
+// The leaf a lane holds (s.post != 0): an instance leaf enters the
+// instance's BLAS with the object ray; a BLAS leaf tests its primitives.
+// Returns true when the lane's traversal has ended: no node left, an
+// opaque occluder (MODE_OCCLUSION, `occluded` set) or a full stack.
+template <int MODE, bool STATS, bool RECORD, int TIES>
+__device__ __forceinline__ bool traceLeaf(TraceState &s, const DScene &S, Hit &best, uint32_t *lds, uint32_t *ovf,
+    TraceStats &st, bool &occluded)
+{
+  const uint32_t enc = ~(uint32_t)s.post;
+  s.post = 0;
+  const uint32_t first = enc >> 3;
+  const uint32_t count = (enc & 7u) + 1u;
+  if (s.inst < 0) {
+    // world leaf: one instance; continue in its BLAS with the object ray
+    s.inst = (int)first;
+    const DInstance &I = S.instances[s.inst];
+    const linear3f rl = {{I.rcp[0], I.rcp[1], I.rcp[2]}, {I.rcp[3], I.rcp[4], I.rcp[5]}, {I.rcp[6], I.rcp[7], I.rcp[8]}};
+    s.o = mul(rl, s.worg) + v3(I.rcp[9], I.rcp[10], I.rcp[11]);
+    s.d = mul(rl, s.wdir);
+    s.inv = v3(safeInv(s.d.x), safeInv(s.d.y), safeInv(s.d.z));
+    s.pc = triPrecompute(s.d);
+    s.modelOffset = I.modelOffset;
+    if (s.sp + 2 > kLdsDepth + kOverflowDepth)
+      return true;
+    if (s.cur != kTraceDone)
+      tracePush(s, lds, ovf, s.cur);
+    tracePush(s, lds, ovf, kMarker);
+    s.cur = I.blasRoot;
+    return false;
+  }
+  const float t0 = s.t0;
+  for (uint32_t k = 0; k < count; k++) {
+    const DPrim &P = S.prims[first + k];
+    const float4 pa = reinterpret_cast<const float4 *>(&P)[0];
+    const float4 pb = reinterpret_cast<const float4 *>(&P)[1];
+    const float4 pcv = reinterpret_cast<const float4 *>(&P)[2];
+    const uint32_t primID = __float_as_uint(pa.w);
+    const uint32_t mk = __float_as_uint(pb.w);
+    const uint32_t model = mk & 0x0fffffffu;
+    const uint32_t kind = mk >> 28;
+    if (STATS)
+      st.prims++;
+    if (MODE == MODE_OCCLUSION && kind == PRIM_SPHERE) {
+      float tE, tX;
+      vec3f NE, NX;
+      const int roots = sphereRoots(s.o, s.d, t0, v3(pa.x, pa.y, pa.z), pb.x, best.t, tE, NE, tX, NX);
+      if (!roots)
+        continue;
+      if (S.models[s.modelOffset + model].opaque) {
+        occluded = true;
+        return true;
+      }
+      s.sawTransparent = true;
+      if (RECORD) {
+        if (roots & 1)
+          recordHit(ovf, tE, (uint32_t)s.inst, first + k, 0);
+        if (roots & 2)
+          recordHit(ovf, tX, (uint32_t)s.inst, first + k, 1);
+      }
+      continue;
+    }
+    float t, u = 0.f, v = 0.f;
+    vec3f Ng;
+    bool h;
+    uint32_t half = 0;
+    const float tmaxP = TIES != TIES_OFF && best.hit ? fminf(best.t + kTieEps * best.t, s.tfar) : best.t;
+    if (kind == PRIM_SPHERE) {
+      h = intersectSphere(s.o, s.d, t0, v3(pa.x, pa.y, pa.z), pb.x, tmaxP, t, Ng);
+    } else {
+      const vec3f A = v3(pa.x, pa.y, pa.z), B = v3(pb.x, pb.y, pb.z), C = v3(pcv.x, pcv.y, pcv.z);
+      h = intersectTriangle(s.o, t0, s.pc, A, B, C, tmaxP, t, u, v);
+      if (h) {
+        Ng = cross(B - A, C - A);
+        if (kind == PRIM_QUAD1) {
+          half = 1;
+          u = 1.0f - u;
+          v = 1.0f - v;
+        }
+      }
+    }
+    if (!h)
+      continue;
+    if (MODE == MODE_OCCLUSION) {
+      if (S.models[s.modelOffset + model].opaque) {
+        occluded = true;
+        return true;
+      }
+      s.sawTransparent = true;
+      if (RECORD)
+        recordHit(ovf, t, (uint32_t)s.inst, first + k, 0);
+      continue;
+    }
+    const bool nt = TIES != TIES_OFF && best.hit && nearTie(t, best.t);
+    if (better(t, (uint32_t)s.inst, model, primID, half, best)) {
+      if (TIES != TIES_OFF)
+        s.tie = nt; // a hit below the window of the previous closest one clears the flag
+      best.hit = true;
+      best.t = t;
+      best.u = u;
+      best.v = v;
+      best.Ng = Ng;
+      best.inst = (uint32_t)s.inst;
+      best.model = model;
+      best.prim = primID;
+      best.half = half;
+    } else if (nt) {
+      s.tie = true;
+    }
+  }
+  return s.cur == kTraceDone;
+}
+
+template <int MODE, bool STATS, bool RECORD = false, int TIES = TIES_OFF>
+__device__ bool traceRay(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
+    uint32_t *lds, uint32_t *ovf, bool &sawTransparent, TraceStats &st)
+{
+  TraceState s;
+  traceBegin(s, S, worg, wdir, t0, tfar, best);
+  bool occluded = false;
+  for (;;) {
+    traceDescend<TIES, STATS>(s, S, best, lds, ovf, st);
+    if (s.post == 0) { // nothing held: the lane has no node left
+      if (s.cur == kTraceDone)
         break;
       continue; // (cannot happen: a lane without a held leaf can step)
     }
-    const uint32_t enc = ~(uint32_t)post;
-    post = 0;
-    const uint32_t first = enc >> 3;
-    const uint32_t count = (enc & 7u) + 1u;
-    if (inst < 0) {
-      // world leaf: one instance; continue in its BLAS with the object ray
-      inst = (int)first;
-      const DInstance &I = S.instances[inst];
-      const linear3f rl = {{I.rcp[0], I.rcp[1], I.rcp[2]}, {I.rcp[3], I.rcp[4], I.rcp[5]}, {I.rcp[6], I.rcp[7], I.rcp[8]}};
-      o = mul(rl, worg) + v3(I.rcp[9], I.rcp[10], I.rcp[11]);
-      d = mul(rl, wdir);
-      inv = v3(safeInv(d.x), safeInv(d.y), safeInv(d.z));
-      pc = triPrecompute(d);
-      modelOffset = I.modelOffset;
-      if (sp + 2 > kLdsDepth + kOverflowDepth)
-        return best.hit;
-      if (cur != kDone)
-        push(cur);
-      push(kMarker);
-      cur = I.blasRoot;
-      continue;
-    }
-    for (uint32_t k = 0; k < count; k++) {
-      const DPrim &P = S.prims[first + k];
-      const float4 pa = reinterpret_cast<const float4 *>(&P)[0];
-      const float4 pb = reinterpret_cast<const float4 *>(&P)[1];
-      const float4 pcv = reinterpret_cast<const float4 *>(&P)[2];
-      const uint32_t primID = __float_as_uint(pa.w);
-      const uint32_t mk = __float_as_uint(pb.w);
-      const uint32_t model = mk & 0x0fffffffu;
-      const uint32_t kind = mk >> 28;
-      if (STATS)
-        st.prims++;
-      if (MODE == MODE_OCCLUSION && kind == PRIM_SPHERE) {
-        float tE, tX;
-        vec3f NE, NX;
-        const int roots = sphereRoots(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, best.t, tE, NE, tX, NX);
-        if (!roots)
-          continue;
-        if (S.models[modelOffset + model].opaque)
-          return true;
-        sawTransparent = true;
-        if (RECORD) {
-          if (roots & 1)
-            recordHit(ovf, tE, (uint32_t)inst, first + k, 0);
-          if (roots & 2)
-            recordHit(ovf, tX, (uint32_t)inst, first + k, 1);
-        }
-        continue;
-      }
-      float t, u = 0.f, v = 0.f;
-      vec3f Ng;
-      bool h;
-      uint32_t half = 0;
-      const float tmaxP = TIES != TIES_OFF && best.hit ? fminf(best.t + kTieEps * best.t, tfar) : best.t;
-      if (kind == PRIM_SPHERE) {
-        h = intersectSphere(o, d, t0, v3(pa.x, pa.y, pa.z), pb.x, tmaxP, t, Ng);
-      } else {
-        const vec3f A = v3(pa.x, pa.y, pa.z), B = v3(pb.x, pb.y, pb.z), C = v3(pcv.x, pcv.y, pcv.z);
-        h = intersectTriangle(o, t0, pc, A, B, C, tmaxP, t, u, v);
-        if (h) {
-          Ng = cross(B - A, C - A);
-          if (kind == PRIM_QUAD1) {
-            half = 1;
-            u = 1.0f - u;
-            v = 1.0f - v;
-          }
-        }
-      }
-      if (!h)
-        continue;
-      if (MODE == MODE_OCCLUSION) {
-        if (S.models[modelOffset + model].opaque)
-          return true;
-        sawTransparent = true;
-        if (RECORD)
-          recordHit(ovf, t, (uint32_t)inst, first + k, 0);
-        continue;
-      }
-      const bool nt = TIES != TIES_OFF && best.hit && nearTie(t, best.t);
-      if (better(t, (uint32_t)inst, model, primID, half, best)) {
-        if (TIES != TIES_OFF)
-          tie = nt; // a hit below the window of the previous closest one clears the flag
-        best.hit = true;
-        best.t = t;
-        best.u = u;
-        best.v = v;
-        best.Ng = Ng;
-        best.inst = (uint32_t)inst;
-        best.model = model;
-        best.prim = primID;
-        best.half = half;
-      } else if (nt) {
-        tie = true;
-      }
-    }
-    if (cur == kDone)
+    if (traceLeaf<MODE, STATS, RECORD, TIES>(s, S, best, lds, ovf, st, occluded))
       break;
   }
-  if (TIES == TIES_RESOLVE && tie)
+  sawTransparent = s.sawTransparent;
+  if (MODE == MODE_OCCLUSION && occluded)
+    return true;
+  if (TIES == TIES_RESOLVE && s.tie)
     resolveNearTie<STATS>(S, worg, wdir, t0, tfar, best, lds, ovf, st);
   if (TIES == TIES_REPORT)
-    sawTransparent = tie;
+    sawTransparent = s.tie;
   return best.hit;
 }
 
