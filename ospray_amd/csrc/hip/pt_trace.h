@@ -510,12 +510,17 @@ __device__ __forceinline__ void tracePush(TraceState &s, uint32_t *lds, uint32_t
   s.sp++;
 }
 
+// the LDS read is unconditional (a clamped slot), the overflow row is read
+// only by the rare lanes whose stack is deeper than kLdsDepth (a volatile
+// read, so the compiler keeps the two reads apart instead of merging them
+// into one generic-address load)
 __device__ __forceinline__ int tracePop(TraceState &s, const uint32_t *lds, const uint32_t *ovf)
 {
   s.sp--;
-  if (s.sp < kLdsDepth)
-    return (int)lds[s.sp * kTraceBlock + threadIdx.x];
-  return (int)ovf[s.sp - kLdsDepth];
+  int v = (int)lds[(s.sp < kLdsDepth ? s.sp : kLdsDepth - 1) * kTraceBlock + threadIdx.x];
+  if (s.sp >= kLdsDepth)
+    v = (int)*reinterpret_cast<const volatile uint32_t *>(ovf + (s.sp - kLdsDepth));
+  return v;
 }
 
 // Inner nodes of the speculative while-while traversal: a lane that reaches
@@ -550,7 +555,17 @@ __device__ __forceinline__ void traceDescend(
         s.cur = s.sp == 0 ? kTraceDone : tracePop(s, lds, ovf);
       } else {
         const int npush = (t1c != kInf) + (t2c != kInf) + (t3c != kInf);
-        if (s.sp + npush > kLdsDepth + kOverflowDepth) { // cannot happen: the world commit checks the stack need
+        if (s.sp + 3 <= kLdsDepth) {
+          // the children after the nearest, farthest first: the sorted
+          // boxes that were hit are a prefix, so the npush entries pushed
+          // are c_npush .. c1; the slots above them get don't-care copies
+          uint32_t *top = lds + s.sp * kTraceBlock + threadIdx.x;
+          top[0] = (uint32_t)(npush == 3 ? c3 : (npush == 2 ? c2 : c1));
+          top[kTraceBlock] = (uint32_t)(npush == 3 ? c2 : c1);
+          top[2 * kTraceBlock] = (uint32_t)c1;
+          s.sp += npush;
+          s.cur = c0;
+        } else if (s.sp + npush > kLdsDepth + kOverflowDepth) { // cannot happen: the world commit checks the stack need
           s.cur = kTraceDone;
           s.sp = 0;
         } else {
