@@ -2708,6 +2708,11 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         F.nrcTrainStride = (uint32_t)std::max<double>(1.0, (double)slotsPass * r->nrcSurvival / r->nrcTrainPaths);
         F.nrcTrainOffset = (uint32_t)((size_t)fb->frameID * 7919u + (size_t)s0) % F.nrcTrainStride;
       }
+      // the delta-tracking kernels take one ray (path) per thread: their
+      // walks vary widely in length, and with every ray on a thread of its
+      // own the hardware starts a new wave as each one ends, so the kernel
+      // waits on its longest walk rather than on a thread's sum of walks
+      const int volBlocks = std::max<int>(shadeBlocks, (int)(((size_t)F.passCount * F.numPixPadded + 255) / 256));
       hpt::DFrame Fno = F; // the frame as the bounces without a cache vertex see it
       Fno.nrcBounce = -1;
       const HptFrame *Fp = reinterpret_cast<const HptFrame *>(&F);
@@ -2753,7 +2758,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
         if (overlap)
           launch(nm("reset").c_str(), [&]() { return hpt_reset_append_counters(P, q, stream); });
         if (F.hasVolumes)
-          launch(nm("volume").c_str(), [&]() { return hpt_volume(S, P, q, shadeBlocks, travStats ? 1 : 0, stream); });
+          launch(nm("volume").c_str(), [&]() { return hpt_volume(S, P, q, volBlocks, travStats ? 1 : 0, stream); });
         launch(nm("shade").c_str(),
             [&]() {
               const bool cacheVertex = b == F.nrcBounce || (F.nrcSelfTrain && b == F.nrcBounce + 1);
@@ -2775,7 +2780,7 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
           launch(nm("shadow_trans").c_str(),
               [&]() { return hpt_shadow_trans(S, Fp, PS, residentTraceBlocks, travStats ? 1 : 0, shadowStream); },
               shadowStream);
-          launch(nm("resolve").c_str(), [&]() { return hpt_resolve(S, Fp, PS, q, shadeBlocks, shadowStream); },
+          launch(nm("resolve").c_str(), [&]() { return hpt_resolve(S, Fp, PS, q, F.hasVolumes ? volBlocks : shadeBlocks, shadowStream); },
               shadowStream);
           if (overlap) {
             HIP_CHECK(hipEventRecord(lane->evRes[b & 1], side));

@@ -184,29 +184,35 @@ HPT_HD float deltaTracking(
   cnt.tracks++;
   const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
   const float ii[3] = {safeRcp(d.x), safeRcp(d.y), safeRcp(d.z)};
+  // the grid fields the macrocell steps read, in registers: each step then
+  // waits on one load (its majorant) rather than also on the fields it
+  // indexes by the step's axis
+  const float *majorant = v.majorant;
+  const int md[3] = {v.mdims[0], v.mdims[1], v.mdims[2]};
+  const float gorg[3] = {v.origin[0], v.origin[1], v.origin[2]};
   float msz[3];
   int m[3], step[3];
   float tnext[3];
   for (int a = 0; a < 3; a++) {
     msz[a] = v.spacing[a] * (float)kMacro;
     const float p = oo[a] + t0 * dd[a];
-    const float cf = floorf((p - v.origin[a]) / msz[a]);
-    const int c = !(cf > 0.f) ? 0 : (cf >= (float)v.mdims[a] ? v.mdims[a] - 1 : (int)cf);
+    const float cf = floorf((p - gorg[a]) / msz[a]);
+    const int c = !(cf > 0.f) ? 0 : (cf >= (float)md[a] ? md[a] - 1 : (int)cf);
     m[a] = c;
     step[a] = dd[a] > 0.f ? 1 : (dd[a] < 0.f ? -1 : 0);
-    tnext[a] = step[a] == 0 ? kInf : (v.origin[a] + (float)(c + (step[a] > 0 ? 1 : 0)) * msz[a] - oo[a]) * ii[a];
+    tnext[a] = step[a] == 0 ? kInf : (gorg[a] + (float)(c + (step[a] > 0 ? 1 : 0)) * msz[a] - oo[a]) * ii[a];
   }
   float t = t0;
   float tau = -flog(1.f - randomFloat(rs));
   int steps = 0;
-  for (int guard = v.mdims[0] + v.mdims[1] + v.mdims[2]; t < t1 && guard > 0; guard--) {
+  for (int guard = md[0] + md[1] + md[2]; t < t1 && guard > 0; guard--) {
     const int a = tnext[0] < tnext[1] ? (tnext[0] < tnext[2] ? 0 : 2) : (tnext[1] < tnext[2] ? 1 : 2);
     const float tb = minf(tnext[a], t1);
     if (tb > t) {
       // one interval [t, tb] of macrocell m
-      const size_t mi = ((size_t)m[2] * (size_t)v.mdims[1] + (size_t)m[1]) * (size_t)v.mdims[0] + (size_t)m[0];
+      const size_t mi = ((size_t)m[2] * (size_t)md[1] + (size_t)m[1]) * (size_t)md[0] + (size_t)m[0];
       cnt.cells++;
-      const float mu_max = v.majorant[mi];
+      const float mu_max = majorant[mi];
       if (mu_max > 0.f) {
         float tt = t;
         for (;;) {
@@ -236,9 +242,9 @@ HPT_HD float deltaTracking(
       t = tb;
     }
     m[a] += step[a];
-    if (m[a] < 0 || m[a] >= v.mdims[a])
+    if (m[a] < 0 || m[a] >= md[a])
       break;
-    tnext[a] = (v.origin[a] + (float)(m[a] + (step[a] > 0 ? 1 : 0)) * msz[a] - oo[a]) * ii[a];
+    tnext[a] = (gorg[a] + (float)(m[a] + (step[a] > 0 ? 1 : 0)) * msz[a] - oo[a]) * ii[a];
   }
   return kInf;
 }
