@@ -186,7 +186,9 @@ HPT_HD float deltaTracking(
   const float ii[3] = {safeRcp(d.x), safeRcp(d.y), safeRcp(d.z)};
   // the grid fields the macrocell steps read, in registers: each step then
   // waits on one load (its majorant) rather than also on the fields it
-  // indexes by the step's axis
+  // indexes by the step's axis; the tentative collisions read the fields
+  // of the copy `vc`, loaded once per track
+  const DVolume vc = v;
   const float *majorant = v.majorant;
   const int md[3] = {v.mdims[0], v.mdims[1], v.mdims[2]};
   const float gorg[3] = {v.origin[0], v.origin[1], v.origin[2]};
@@ -225,12 +227,12 @@ HPT_HD float deltaTracking(
           const float xi = randomFloat(rs);
           cnt.samples++;
           const vec3f p = o + tt * d;
-          const float smp = volumeSample(v, p);
+          const float smp = volumeSample(vc, p);
           if (smp == smp) {
-            const float u = tfUnit(v, smp);
-            const float mu_t = v.densityScale * tfOpacity(v, u);
+            const float u = tfUnit(vc, smp);
+            const float mu_t = vc.densityScale * tfOpacity(vc, u);
             if (xi < mu_t / mu_max) {
-              albedo = tfColor(v, u);
+              albedo = tfColor(vc, u);
               return tt;
             }
           }
