@@ -233,7 +233,7 @@ def bench_c2(args) -> int:
     t = time.perf_counter()
     best_ms, best = A.tune_batch(fbs, rens, cams, s.world, reps=1)
     log(f"autotuner: {time.perf_counter() - t:.1f} s, best batch {best_ms:.2f} ms with traceBlocksPerCU {best[0]}, "
-        f"shadeBlocksPerCU {best[1]}, pathBudgetMB {best[2]}, batchStreams {best[3]}")
+        f"shadeBlocksPerCU {best[1]}, pathBudgetMB {best[2]}, batchStreams {best[3]}, tileWidth {best[4]}")
     t = time.perf_counter()
     A.render_batch(fbs, rens, cams, s.world, graph=True)  # one eager frame, then the capture
     log(f"eager frame and graph capture in {time.perf_counter() - t:.2f} s")
@@ -260,7 +260,7 @@ def bench_c2(args) -> int:
                       **({"device_params": args.dparams} if args.dparams else {})},
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
            "autotuner": {"traceBlocksPerCU": best[0], "shadeBlocksPerCU": best[1], "pathBudgetMB": best[2],
-                         "batchStreams": best[3],
+                         "batchStreams": best[3], "tileWidth": best[4],
                          "best_eager_batch_ms": round(best_ms, 3)},
            "eager_batch_ms": round(eager_s * 1e3, 3)}
     print(json.dumps(out), flush=True)

@@ -50,11 +50,13 @@ OSPRAY_INTERFACE int ospHipRenderBatch(int n, const OSPFrameBuffer *framebuffers
  * candidate resident traversal grid (8, 12, 16 blocks per CU), shading
  * grid (4, 8, 16) and path-state budget (2 GiB, 16 GiB: the samples per
  * pass, i.e. the wavefront queue sizes), then, at the fastest of those,
- * for 1, 2, 4 and 8 lanes (batchStreams), and keeps the fastest as the
- * device's setting.  Returns its batch milliseconds; best4 (4 ints) =
- * traceBlocksPerCU, shadeBlocksPerCU, pathBudgetMB, batchStreams. */
+ * for 1, 2, 4 and 8 lanes (batchStreams), then for the pixel tiles of 64
+ * path slots 4x16, 8x8, 16x4 and 32x2 (tileWidth 4, 8, 16, 32), and keeps
+ * the fastest as the device's setting.  Returns its batch milliseconds;
+ * best5 (5 ints) = traceBlocksPerCU, shadeBlocksPerCU, pathBudgetMB,
+ * batchStreams, tileWidth. */
 OSPRAY_INTERFACE float ospHipTuneBatch(int n, const OSPFrameBuffer *framebuffers, const OSPRenderer *renderers,
-    const OSPCamera *cameras, OSPWorld world, int reps, int *best4);
+    const OSPCamera *cameras, OSPWorld world, int reps, int *best5);
 
 /* The radiance cache of a "pathtracer" renderer with "nrc" set: the loss of
  * its last training step (relative L2), the number of Adam steps taken so

@@ -351,8 +351,8 @@ def render_batch(fbs, renderers, cameras, world: Handle, graph: bool = True):
 
 def tune_batch(fbs, renderers, cameras, world: Handle, reps: int = 2):
     """Run the batch autotuner; returns (best batch ms, (traceBlocksPerCU,
-    shadeBlocksPerCU, pathBudgetMB, batchStreams))."""
-    best = (C.c_int * 4)()
+    shadeBlocksPerCU, pathBudgetMB, batchStreams, tileWidth))."""
+    best = (C.c_int * 5)()
     n = len(fbs)
     ms = K.ext().ospHipTuneBatch(n, _handles(fbs), _handles(renderers), _handles(cameras), world.h, reps, best)
     if ms < 0:

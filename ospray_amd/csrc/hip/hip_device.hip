@@ -1013,6 +1013,7 @@ class HipDevice : public Device
   std::vector<std::unique_ptr<PathLane>> lanes;
   PathLane *lane = nullptr;
   int batchStreams = 4; // lanes a batch of views spreads over
+  int slotTileWidth = 8; // pixel tile of 64 path slots: 4x16, 8x8, 16x4 or 32x2
   int frameLanes = 1; // lanes the passes of one frame alternate over (1: one stream)
   bool inBatch = false; // enqueuing the views of a batch
   int batchView = -1; // index of the view being enqueued in a batch
@@ -1106,6 +1107,11 @@ class HipDevice : public Device
     // defaults to the current budget, so a recommit keeps the tuned value
     const int budgetMB = getInt("pathBudgetMB", (int)(pathBudgetBytes >> 20));
     pathBudgetBytes = (size_t)std::max(64, budgetMB) << 20;
+    // last, so that an invalid value leaves the other parameters committed
+    const int tw = getInt("tileWidth", slotTileWidth);
+    if (tw != 4 && tw != 8 && tw != 16 && tw != 32)
+      throw std::runtime_error("device parameter tileWidth must be 4, 8, 16 or 32");
+    slotTileWidth = tw;
   }
 
   // GPU state is created on first use, so the object layer works (and
@@ -1710,8 +1716,9 @@ class HipDevice : public Device
   // Autotuner of the batch: times the eager batch for each candidate of
   // the resident traversal grid, the shading grid and the path-state
   // budget (which sets the samples per pass, i.e. the wavefront-queue
-  // sizes), keeps the fastest.  Returns its milliseconds; best[3] =
-  // traceBlocksPerCU, shadeBlocksPerCU, pathBudgetMB.
+  // sizes), then of the lane count, then of the slot tile's width; keeps
+  // the fastest.  Returns its milliseconds; best[5] = traceBlocksPerCU,
+  // shadeBlocksPerCU, pathBudgetMB, batchStreams, tileWidth.
   float tuneBatch(const std::vector<BatchView> &v, ManagedObject *world, int reps, int *best)
   {
     gpu();
@@ -1766,12 +1773,32 @@ class HipDevice : public Device
       }
     }
     batchStreams = bl;
+    // third stage: the pixel tile of the path slots (the shape of the pixel
+    // block a wave's primary rays cover)
+    const int tw0 = slotTileWidth;
+    int bw = tw0;
+    bestMs = 1e30f;
+    for (int tw : {4, 8, 16, 32}) {
+      slotTileWidth = tw;
+      renderBatch(v, world, 0, ms.data()); // warm-up (queue sizes)
+      float t = 1e30f;
+      for (int k = 0; k < std::max(1, reps); k++) {
+        renderBatch(v, world, 0, ms.data());
+        t = std::min(t, batchMs(ms));
+      }
+      if (t < bestMs) {
+        bestMs = t;
+        bw = tw;
+      }
+    }
+    slotTileWidth = bw;
     releaseGraph();
     if (best) {
       best[0] = bt;
       best[1] = bs;
       best[2] = (int)(bb >> 20);
       best[3] = bl;
+      best[4] = bw;
     }
     return bestMs;
   }
@@ -2560,7 +2587,11 @@ ManagedObject *HipDevice::renderFrame(ManagedObject *fbo, ManagedObject *ro, Man
   F.height = fb->h;
   F.tilesX = (fb->w + 7) / 8;
   F.tilesY = (fb->h + 7) / 8;
-  F.numPixPadded = F.tilesX * F.tilesY * 64;
+  // the path slots of a sample: 64-slot tiles of slotTileWidth x (64 /
+  // slotTileWidth) pixels, so a wave's primary rays cover one tile
+  F.slotShift = slotTileWidth == 4 ? 2 : (slotTileWidth == 16 ? 4 : (slotTileWidth == 32 ? 5 : 3));
+  F.slotTilesX = (fb->w + (1 << F.slotShift) - 1) >> F.slotShift;
+  F.numPixPadded = F.slotTilesX * ((fb->h + (64 >> F.slotShift) - 1) / (64 >> F.slotShift)) * 64;
   F.spp = r->spp;
   F.accumID = (uint32_t)fb->frameID;
   F.targetFrames = fb->targetFrames;
@@ -3010,14 +3041,14 @@ OSPRAY_INTERFACE int ospHipRenderBatch(int n, const OSPFrameBuffer *fbs, const O
 }
 
 OSPRAY_INTERFACE float ospHipTuneBatch(int n, const OSPFrameBuffer *fbs, const OSPRenderer *renderers,
-    const OSPCamera *cameras, OSPWorld world, int reps, int *best4)
+    const OSPCamera *cameras, OSPWorld world, int reps, int *best5)
 {
   HipDevice *d = current();
   std::vector<HipDevice::BatchView> v;
   if (!d || !world || !batchViews(n, fbs, renderers, cameras, v))
     return -1.f;
   try {
-    return d->tuneBatch(v, (ManagedObject *)world, reps, best4);
+    return d->tuneBatch(v, (ManagedObject *)world, reps, best5);
   } catch (const std::exception &e) {
     extensionError(d, e.what());
     return -2.f;

@@ -111,13 +111,14 @@ __device__ __forceinline__ int globalId() { return (int)(blockIdx.x * blockDim.x
 // path slot <-> pixel: slots are 8x8 tiles, one wave per tile and sample
 __device__ __forceinline__ void slotPixel(const DFrame &F, int q, int &ix, int &iy)
 {
-  const int tile = q >> 6, w = q & 63;
-  ix = (tile % F.tilesX) * 8 + (w & 7);
-  iy = (tile / F.tilesX) * 8 + (w >> 3);
+  const int tile = q >> 6, w = q & 63, sh = F.slotShift;
+  ix = ((tile % F.slotTilesX) << sh) + (w & ((1 << sh) - 1));
+  iy = (tile / F.slotTilesX) * (64 >> sh) + (w >> sh);
 }
 __device__ __forceinline__ int pixelSlot(const DFrame &F, int ix, int iy)
 {
-  return ((iy >> 3) * F.tilesX + (ix >> 3)) * 64 + (iy & 7) * 8 + (ix & 7);
+  const int sh = F.slotShift, th = 64 >> sh;
+  return ((iy / th) * F.slotTilesX + (ix >> sh)) * 64 + (iy % th) * (1 << sh) + (ix & ((1 << sh) - 1));
 }
 
 // the frame index (accumID): the host's, or the device-resident one of a

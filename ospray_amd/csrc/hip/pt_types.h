@@ -294,7 +294,10 @@ struct DFrame
 {
   // image and pass
   int32_t width, height;
-  int32_t tilesX, tilesY, numPixPadded;
+  int32_t tilesX, tilesY; // 8x8-pixel tasks (the variance estimate's units)
+  // path slots: 64-slot tiles of (1 << slotShift) x (64 >> slotShift)
+  // pixels, slotTilesX per row, numPixPadded slots per sample
+  int32_t slotShift, slotTilesX, numPixPadded;
   int32_t spp, passS0, passCount; // samples [passS0, passS0+passCount)
   uint32_t accumID;
   int32_t targetFrames;

@@ -85,14 +85,15 @@ def test_autotuner_keeps_images(hip_device):
     before = _images(fbs)
     ms, best = A.tune_batch(fbs, rens, cams, s.world, reps=1)
     assert ms > 0 and best[0] in (8, 12, 16) and best[1] in (4, 8, 16) and best[2] in (2048, 16384)
-    assert best[3] in (1, 2, 4, 8)
+    assert best[3] in (1, 2, 4, 8) and best[4] in (4, 8, 16, 32)
     A.render_batch(fbs, rens, cams, s.world, graph=True)
     for a, b in zip(_images(fbs), before):
         assert np.array_equal(a, b)
     # back to the defaults for the tests that follow
     L = K.lib()
     dev = L.ospGetCurrentDevice()
-    for k, v in (("traceBlocksPerCU", 12), ("shadeBlocksPerCU", 8), ("pathBudgetMB", 16384), ("batchStreams", 4)):
+    for k, v in (("traceBlocksPerCU", 12), ("shadeBlocksPerCU", 8), ("pathBudgetMB", 16384), ("batchStreams", 4),
+                 ("tileWidth", 8)):
         A.set_device_param(dev, k, v)
     L.ospDeviceCommit(dev)
 
