@@ -270,11 +270,23 @@ def bench_c2(args) -> int:
 def measure_c3(args, width: int, height: int) -> dict:
     """Plain vs radiance-cache rendering of the 1080p scene, 1 spp per frame,
     each accumulated over --c3-frames frames and compared (RMSE over RGB)
-    with a plain render of --c3-ref-spp samples per pixel.  Frame times are
-    ospRenderFrame + ospWait wall times (the cache's inference and training
-    included); framebuffer maps for the RMSE checkpoints are outside them.
-    value = (plain time to its final RMSE) / (cache time to the same RMSE).
-    Expects the hip device to be current (with profileKernels set)."""
+    with a plain render of --c3-ref-spp samples per pixel."""
+    from ospray_amd import scenes
+
+    sc = scenes.c1(width=width, height=height, spp=1)
+    return measure_cache(args, sc, args.c3_frames, "C1 scene")
+
+
+def measure_cache(args, sc, frames: int, scene_name: str) -> dict:
+    """Plain vs radiance-cache rendering of the Scene `sc` at its spp per
+    frame, each accumulated over `frames` frames and compared (RMSE over
+    RGB) with a plain render of --c3-ref-spp samples per pixel (cache after
+    bounce --c3-bounce, --c3-train-paths training paths per frame in
+    --c3-train-steps Adam steps).  Frame times are ospRenderFrame + ospWait
+    wall times (the cache's inference and training included); framebuffer
+    maps for the RMSE checkpoints are outside them.  value = (plain time to
+    its final RMSE) / (cache time to the same RMSE).  Expects the hip device
+    to be current (with profileKernels set)."""
     import math
 
     import numpy as np
@@ -283,8 +295,8 @@ def measure_c3(args, width: int, height: int) -> dict:
     from ospray_amd import capi as K
     from ospray_amd import scenes
 
-    w, h = width, height
-    sc = scenes.c1(width=w, height=h, spp=1)
+    w, h = sc.width, sc.height
+    spp_frame = int(sc.renderer.get("pixelSamples", 1))
     name, _ = A.device_info()
     L = K.lib()
 
@@ -304,13 +316,13 @@ def measure_c3(args, width: int, height: int) -> dict:
     for i in range(nref):
         frame(sref, fbr)
         if i % 64 == 63:
-            log(f"c3 reference {ref_spp_frame * (i + 1)} spp")
+            log(f"cache comparison: reference {ref_spp_frame * (i + 1)} spp")
     ref = A.map_color(fbr, w, h, K.OSP_FB_RGBA32F)[..., :3].astype(np.float64)
     ref_s = time.perf_counter() - t
-    log(f"c3 reference: {nref * ref_spp_frame} spp in {ref_s:.1f} s")
+    log(f"cache comparison: reference {nref * ref_spp_frame} spp in {ref_s:.1f} s")
     del fbr, sref
 
-    checkpoints = sorted({1, 2, 4, 8, 16, 32, 64, 128, 192, 256, args.c3_frames} & set(range(1, args.c3_frames + 1)))
+    checkpoints = sorted({1, 2, 4, 8, 16, 32, 64, 128, 192, 256, frames} & set(range(1, frames + 1)))
 
     def run(rp, label):
         s = scenes.to_ospray(sc, rp)
@@ -319,15 +331,15 @@ def measure_c3(args, width: int, height: int) -> dict:
         L.ospResetAccumulation(fb.h)
         A.reset_kernel_stats()
         curve, elapsed = [], 0.0
-        for i in range(1, args.c3_frames + 1):
+        for i in range(1, frames + 1):
             t0 = time.perf_counter()
             frame(s, fb)
             elapsed += time.perf_counter() - t0
             if i in checkpoints:
                 img = A.map_color(fb, w, h, K.OSP_FB_RGBA32F)[..., :3].astype(np.float64)
                 curve.append((i, elapsed, float(np.sqrt(np.mean((img - ref) ** 2)))))
-        log(f"c3 {label}: {args.c3_frames} frames {elapsed:.3f} s rmse {curve[-1][2]:.5f}")
-        ks = {k: round(v[1] / args.c3_frames, 3) for k, v in sorted(A.kernel_stats().items())}
+        log(f"cache comparison, {label}: {frames} frames {elapsed:.3f} s rmse {curve[-1][2]:.5f}")
+        ks = {k: round(v[1] / frames, 3) for k, v in sorted(A.kernel_stats().items())}
         return curve, s, ks
 
     plain, _, kplain = run({}, "plain")
@@ -348,18 +360,30 @@ def measure_c3(args, width: int, height: int) -> dict:
             prev = c
         return None
 
+    def rmse_at(curve, t):
+        # log-log interpolation between checkpoints; None outside the curve
+        for (f0, t0, e0), (f1, t1, e1) in zip(curve, curve[1:]):
+            if t0 <= t <= t1:
+                a = (math.log(t) - math.log(t0)) / max(math.log(t1) - math.log(t0), 1e-12)
+                return math.exp(math.log(e0) + a * (math.log(e1) - math.log(e0)))
+        return None
+
     tp, tn = time_to(plain, target), time_to(nrc, target)
+    # the two methods' RMSE after the same GPU time (the shorter run's)
+    t_eq = min(plain[-1][1], nrc[-1][1])
+    equal_time = {"seconds": round(t_eq, 4), "plain_rmse": rmse_at(plain, t_eq), "cache_rmse": rmse_at(nrc, t_eq)}
     return {"time_to_equal_rmse": round(tp / tn, 3) if tn else None, "unit": "x (plain time / cache time)",
-            "workload": f"C1 scene {w}x{h}, 1 spp per frame x {args.c3_frames} frames, cache after bounce "
+            "equal_time": equal_time,
+            "workload": f"{scene_name} {w}x{h}, {spp_frame} spp per frame x {frames} frames, cache after bounce "
                         f"{args.c3_bounce}, {args.c3_train_paths} training paths per frame in "
                         f"{args.c3_train_steps} Adam steps (lr 1e-2); reference {nref * ref_spp_frame} spp plain",
             "device": name, "target_rmse": target,
             "cache_bias_floor_rmse": nrc[-1][2],
             "plain": {"rmse": [c[2] for c in plain], "seconds": [round(c[1], 4) for c in plain],
-                      "frames": [c[0] for c in plain], "ms_per_frame": round(plain[-1][1] / args.c3_frames * 1e3, 3),
+                      "frames": [c[0] for c in plain], "ms_per_frame": round(plain[-1][1] / frames * 1e3, 3),
                       "kernels_ms_per_frame": kplain},
             "cache": {"rmse": [c[2] for c in nrc], "seconds": [round(c[1], 4) for c in nrc],
-                      "frames": [c[0] for c in nrc], "ms_per_frame": round(nrc[-1][1] / args.c3_frames * 1e3, 3),
+                      "frames": [c[0] for c in nrc], "ms_per_frame": round(nrc[-1][1] / frames * 1e3, 3),
                       "kernels_ms_per_frame": knrc, "final_training_loss": loss, "adam_steps": steps,
                       "queries_last_frame": queries, "training_records_last_frame": records},
             "time_to_target_s": {"plain": tp, "cache": tn}, "reference_seconds": round(ref_s, 2)}
@@ -397,7 +421,9 @@ def main() -> int:
                     help="c3: training paths per frame (nrcTrainPaths)")
     ap.add_argument("--c3-train-steps", type=int, default=4, help="c3: Adam steps per frame (nrcTrainSteps)")
     ap.add_argument("--no-c3", action="store_true",
-                    help="c1: skip the plain vs radiance-cache leg (time to equal RMSE) of the default run")
+                    help="c1, c4: skip the plain vs radiance-cache leg (time to equal RMSE)")
+    ap.add_argument("--c4-cache-frames", type=int, default=256,
+                    help="c4: frames (of the scene's spp) per method in the radiance-cache leg")
     ap.add_argument("--width", type=int, default=None)
     ap.add_argument("--height", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)
@@ -590,6 +616,13 @@ def main() -> int:
     if args.workload == "c1" and not args.no_c3:
         out["c3"] = measure_c3(args, 1920, 1080)
         log(f"c3: time to equal RMSE {out['c3']['time_to_equal_rmse']}x")
+    if args.workload == "c4" and not args.no_c3:
+        # the cache at surface and volume scattering vertices, against the
+        # plain render of the same scene
+        out["c4_cache"] = measure_cache(args, sc, args.c4_cache_frames, "surface+volume scene")
+        et = out["c4_cache"]["equal_time"]
+        log(f"c4 cache: time to equal RMSE {out['c4_cache']['time_to_equal_rmse']}x; after {et['seconds']} s "
+            f"RMSE plain {et['plain_rmse']}, cache {et['cache_rmse']}")
     if "volume" in kern:
         v = kern["volume"]
         out["volume"] = {"free_path": {"samples": vst[0], "cells": vst[1], "tracks": vst[2],
