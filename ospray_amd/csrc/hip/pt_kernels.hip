@@ -545,9 +545,12 @@ __global__ __launch_bounds__(256) void k_volume(DScene S, DPaths P, int qin, int
 // path.  At a smooth vertex it stores the NEE vertex record and appends the
 // vertex's light-sample slots to the NEE sample queue; k_nee evaluates them
 // (NextEventEstimation.ispc:25-160).  The loop runs over whole waves so the
-// appends can scan across the wave.
+// appends can scan across the wave.  Every instance holds 3 waves per SIMD:
+// the volume instances (whose transmittance walks would take 215
+// registers) spill the registers of their cold branches instead of
+// dropping to 2 waves.
 template <bool VOL, bool NRC>
-__global__ __launch_bounds__(256, VOL ? 2 : 3) void k_shade(DScene S, DFrame F, DPaths P, int qin)
+__global__ __launch_bounds__(256, 3) void k_shade(DScene S, DFrame F, DPaths P, int qin)
 {
   const int n = P.counters[2 * qin];
   const int qout = qin ^ 1;
