@@ -500,7 +500,14 @@ def main() -> int:
         frame()
     elapsed = time.perf_counter() - t0
     A.check()
-    kstats = A.kernel_stats()
+    kstats_raw = A.kernel_stats()
+    # with the profileBounces device parameter the stage timers are kept per
+    # bounce ("extend@3"): each stage's sum over the bounces is its time
+    kstats = {}
+    for k, (launches, ms) in kstats_raw.items():
+        stage = k.split("@")[0]
+        l0, m0 = kstats.get(stage, (0, 0.0))
+        kstats[stage] = (l0 + launches, m0 + ms)
     log(f"timed {args.steps} frames in {elapsed:.3f} s")
 
     samples = sc.width * sc.height * args.spp
@@ -609,6 +616,8 @@ def main() -> int:
                       **({"device_params": dparams} if dparams else {})},
            "rays_per_s": round(rays_total * args.steps / elapsed, 1),
            "kernels_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats.items())},
+           **({"stages_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats_raw.items())}}
+              if kstats_raw.keys() != kstats.keys() else {}),
            "traversal": {k: {"nodes_per_ray": round(v["nodes_per_ray"], 2), "prims_per_ray": round(v["prims_per_ray"], 2),
                              "rays_per_frame": v["rays_per_frame"], "gbs": round(v["gbs"], 1)}
                          for k, v in kern.items() if "nodes_per_ray" in v},

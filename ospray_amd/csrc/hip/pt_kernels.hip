@@ -337,15 +337,20 @@ __global__ __launch_bounds__(256) void k_generate(DFrame F, DPaths P)
 }
 
 // Work distribution of the traversal kernels: each wave takes batches of
-// 64 consecutive queue entries (neighbouring pixels' rays, which share node
-// fetches) with one atomic per batch, until the queue is exhausted.
-__device__ __forceinline__ int nextBatch(int32_t *counter)
+// `size` (64 unless the queue is short) consecutive queue entries
+// (neighbouring pixels' rays, which share node fetches) with one atomic
+// per batch, until the queue is exhausted.
+__device__ __forceinline__ int nextBatch(int32_t *counter, int size = 64)
 {
   int base = 0;
   if (laneId() == 0)
-    base = atomicAdd(counter, 64);
+    base = atomicAdd(counter, size);
   return __shfl(base, 0);
 }
+
+#ifndef HPT_SPREAD_SHORT_QUEUES
+#define HPT_SPREAD_SHORT_QUEUES 1
+#endif
 
 // Lanes of a wave that wait for a ray before a traversal round refills
 // them: extension rays (long closest-hit traversals) and shadow rays
@@ -367,16 +372,24 @@ constexpr int kExtendRefill = HPT_EXTEND_REFILL, kShadowRefill = HPT_SHADOW_REFI
 // traversal round.  load(entry, org, dir, t0, tfar) gives an entry's ray;
 // finish(ended, entry, best, occluded, state) runs on every lane of the
 // wave (`ended` on the lanes whose ray ended), so it may use wave-wide
-// appends.  Each ray's traversal is the one traceRay runs: the result does
-// not depend on which rays share a wave.
+// appends.  A queue shorter than 64 entries per wave of the grid (the
+// last bounces' few paths) is spread over every wave instead: each wave
+// takes one batch of ceil(n / waves) entries and no more, so the few rays
+// run in many waves side by side rather than 64 to a wave (where each
+// round waits for the slowest of 64 node fetches and the longest of 64
+// rays sets the wave's time).  Each ray's traversal is the one traceRay
+// runs: the result does not depend on which rays share a wave.
 template <int MODE, bool STATS, bool RECORD, int TIES, int REFILL, class Load, class Finish>
 __device__ __forceinline__ void traceQueue(const DScene &S, const uint32_t *queue, int n, int32_t *fetch, uint32_t *lds,
     uint32_t *ovf, TraceStats &st, Load load, Finish finish)
 {
   const int wl = laneId();
   const unsigned long long ltMask = (1ull << wl) - 1ull;
-  int bBase = nextBatch(fetch), bNext = 0; // wave-uniform batch cursor
-  uint32_t bEntry = bBase + wl < n ? queue[bBase + wl] : 0u;
+  const int waves = (int)((gridDim.x * blockDim.x) >> 6);
+  const bool spread = HPT_SPREAD_SHORT_QUEUES && n < 64 * waves;
+  const int bCap = spread ? (n + waves - 1) / waves : 64; // entries per batch, 1..64
+  int bBase = nextBatch(fetch, bCap), bNext = 0; // wave-uniform batch cursor
+  uint32_t bEntry = wl < bCap && bBase + wl < n ? queue[bBase + wl] : 0u;
   TraceState s;
   s.cur = kTraceDone;
   s.post = 0;
@@ -391,7 +404,7 @@ __device__ __forceinline__ void traceQueue(const DScene &S, const uint32_t *queu
         finish(ended, entry, best, occluded, s);
       ended = false;
       while (idle != 0ull && bBase < n) {
-        const int bSize = n - bBase < 64 ? n - bBase : 64;
+        const int bSize = n - bBase < bCap ? n - bBase : bCap;
         const int avail = bSize - bNext;
         const int rank = __popcll(idle & ltMask);
         const int src = bNext + rank;
@@ -409,9 +422,10 @@ __device__ __forceinline__ void traceQueue(const DScene &S, const uint32_t *queu
         bNext += waiting < avail ? waiting : avail;
         idle = __ballot(!busy);
         if (bNext == bSize) {
-          bBase = nextBatch(fetch);
+          // a spread queue: the first batches of all waves cover it
+          bBase = spread ? n : nextBatch(fetch);
           bNext = 0;
-          bEntry = bBase + wl < n ? queue[bBase + wl] : 0u;
+          bEntry = !spread && bBase + wl < n ? queue[bBase + wl] : 0u;
         }
       }
       if (idle == ~0ull) // the queue is exhausted and every lane waits
