@@ -194,6 +194,9 @@ struct TraceStats
 // count as a near-tie (64 ulp at 1.0 is 7.6e-6; the float t of the
 // watertight test is good to a few ulp away from grazing incidence)
 constexpr float kTieEps = 1.0e-5f;
+// lower end of the boxes the FP64 re-resolution opens, relative to the
+// float closest hit
+constexpr float kTieBoxLow = 0.9f;
 
 HPT_HD bool nearTie(float t, float tBest) { return fabsf(t - tBest) <= kTieEps * tBest; }
 
@@ -318,7 +321,11 @@ enum TraceMode
 // hit whose float t is a near-tie of best.t gets its FP64 distance, and the
 // smallest (FP64 t, key) becomes `best` (with its float hit record).  A
 // plain stack traversal: the result is a minimum over a set, so the visit
-// order does not matter.
+// order does not matter.  Boxes are opened over [tLo, tHi]: every hit of
+// the set has a float t within 1e-5 of tF, and tLo = 0.9 tF leaves a
+// margin far above the float error of a hit distance, so only the boxes
+// around the hit point are visited (the descent to it, not every box the
+// ray crosses before it).
 template <bool STATS>
 __device__ void resolveNearTie(const DScene &S, vec3f worg, vec3f wdir, float t0, float tfar, Hit &best,
     uint32_t *lds, uint32_t *ovf, TraceStats &st)
@@ -326,6 +333,7 @@ __device__ void resolveNearTie(const DScene &S, vec3f worg, vec3f wdir, float t0
   const int lane = threadIdx.x;
   const float tF = best.t;
   const float tHi = fminf(tF + kTieEps * tF, tfar);
+  const float tLo = fmaxf(t0, kTieBoxLow * tF);
   const vec3f winv = v3(safeInv(wdir.x), safeInv(wdir.y), safeInv(wdir.z));
   Hit w = best;
   double wD = 0.0;
@@ -349,10 +357,10 @@ __device__ void resolveNearTie(const DScene &S, vec3f worg, vec3f wdir, float t0
     const Node4 nd = loadNode(S.nodes, cur);
     if (STATS)
       st.nodes++;
-    const float tc[4] = {slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, t0, tHi),
-        slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, t0, tHi),
-        slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, t0, tHi),
-        slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, t0, tHi)};
+    const float tc[4] = {slabNear(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, tLo, tHi),
+        slabNear(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, tLo, tHi),
+        slabNear(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, tLo, tHi),
+        slabNear(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, tLo, tHi)};
     const int ch[4] = {nd.child.x, nd.child.y, nd.child.z, nd.child.w};
     for (int c = 0; c < 4; c++)
       if (tc[c] != kInf) {
