@@ -286,7 +286,9 @@ def measure_cache(args, sc, frames: int, scene_name: str) -> dict:
     wall times (the cache's inference and training included); framebuffer
     maps for the RMSE checkpoints are outside them.  value = (plain time to
     its final RMSE) / (cache time to the same RMSE).  Expects the hip device
-    to be current (with profileKernels set)."""
+    to be current (with profileKernels set); from here on its timers cover
+    the cache's inference and training launches only, so the frame times
+    carry no event records of the path-tracing stages."""
     import math
 
     import numpy as np
@@ -299,6 +301,9 @@ def measure_cache(args, sc, frames: int, scene_name: str) -> dict:
     spp_frame = int(sc.renderer.get("pixelSamples", 1))
     name, _ = A.device_info()
     L = K.lib()
+    dev = L.ospGetCurrentDevice()
+    A.set_device_param(dev, "profileStages", "nrc_infer,nrc_train")
+    L.ospDeviceCommit(dev)
 
     def frame(s, fb):
         f = L.ospRenderFrame(fb.h, s.renderer.h, s.camera.h, s.world.h)
@@ -393,7 +398,7 @@ def bench_c3(args) -> int:
     """--workload c3: the plain vs radiance-cache comparison alone."""
     from ospray_amd import api as A
 
-    A.init("hip", profileKernels=True, **args.dparams)
+    A.init("hip", **{"profileKernels": True, **args.dparams})
     r = measure_c3(args, args.width or 1920, args.height or 1080)
     out = {"metric": "plain vs neural-cache time to equal RMSE (1080p scene, 1 spp frames)",
            "value": r["time_to_equal_rmse"], "unit": r["unit"], "n_gpus": 1, "steps": args.c3_frames, "warmup": 1,
@@ -430,6 +435,9 @@ def main() -> int:
     ap.add_argument("--device-param", action="append", default=[], metavar="KEY=VALUE",
                     help="integer or boolean (true/false) parameter of the hip device (traceBlocksPerCU, "
                          "shadeBlocksPerCU, pathBudgetMB, traversalStats); repeatable")
+    ap.add_argument("--all-stage-timers", action="store_true",
+                    help="time every stage of the frame with HIP events (default: only the kernels the roofline "
+                         "line considers; each timed launch adds two event records to the stream)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only time the plain copy kernel on BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -448,7 +456,13 @@ def main() -> int:
     args.dparams = {}
     for kv in args.device_param:
         k, _, v = kv.partition("=")
-        args.dparams[k] = v.lower() == "true" if v.lower() in ("true", "false") else int(v)
+        if v.lower() in ("true", "false"):
+            args.dparams[k] = v.lower() == "true"
+        else:
+            try:
+                args.dparams[k] = int(v)
+            except ValueError:
+                args.dparams[k] = v
     if args.workload == "c3":
         return bench_c3(args)
     if args.workload == "c2":
@@ -472,7 +486,11 @@ def main() -> int:
     log(f"scene generated in {time.perf_counter() - t:.1f} s ({ntris} instanced triangles)")
 
     dparams = args.dparams
-    A.init("hip", profileKernels=True, **dparams)
+    # the timed frames carry HIP event timers around the roofline candidates
+    # only (the traversal and delta-tracking kernels): timers around every
+    # stage add about 0.9 ms of event records per frame
+    stages = "extend,shadow,shadow_trans" + (",volume,resolve" if args.workload == "c4" else "")
+    A.init("hip", **{"profileKernels": True, "profileStages": "" if args.all_stage_timers else stages, **dparams})
     name, cus = A.device_info()
     log(f"device {name}, {cus} CUs")
     t = time.perf_counter()

@@ -1057,12 +1057,18 @@ class HipDevice : public Device
   uint32_t frameGen = 0;
   std::vector<KernelTimer> timers;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pendingEvents;
+  std::vector<hipEvent_t> timerEvents; // recorded and read timer events, for reuse
+  // profileStages: the stages the kernel timers cover (comma-separated
+  // names as in the timer list, without the "@bounce" suffix; empty: all)
+  std::vector<std::string> profileStages;
   std::map<std::vector<std::pair<const Geometry *, uint64_t>>, std::shared_ptr<Blas>> blasCache;
 
   HipDevice() {}
   ~HipDevice() override
   {
     collectTimers();
+    for (auto e : timerEvents)
+      (void)hipEventDestroy(e);
     releaseGraph();
     for (auto e : viewEvents)
       (void)hipEventDestroy(e);
@@ -1093,6 +1099,17 @@ class HipDevice : public Device
     profile = getBool("profileKernels", false);
     travStats = getBool("traversalStats", false);
     profileBounces = getBool("profileBounces", false);
+    profileStages.clear();
+    {
+      const std::string list = getString("profileStages", "");
+      size_t a = 0;
+      while (a <= list.size()) {
+        const size_t e = std::min(list.find(',', a), list.size());
+        if (e > a)
+          profileStages.push_back(list.substr(a, e - a));
+        a = e + 1;
+      }
+    }
     overlapShadow = getBool("overlapShadow", overlapShadow);
     traceBlocksPerCU = std::max(1, std::min(64, getInt("traceBlocksPerCU", traceBlocksPerCU)));
     shadeBlocksPerCU = std::max(1, std::min(64, getInt("shadeBlocksPerCU", shadeBlocksPerCU)));
@@ -1230,19 +1247,40 @@ class HipDevice : public Device
   // grid of the near-tie pass: near-ties are rare, so a small resident grid
   // (at most the traversal grid, whose scratch rows it uses)
   int tieBlocks() const { return std::max(1, std::min(residentTraceBlocks, 2 * numCUs)); }
+  // whether the kernel timers cover stage `name` ("extend" or "extend@3")
+  bool timed(const char *name) const
+  {
+    if (profileStages.empty())
+      return true;
+    const char *at = std::strchr(name, '@');
+    const size_t len = at ? (size_t)(at - name) : std::strlen(name);
+    for (const auto &st : profileStages)
+      if (st.size() == len && st.compare(0, len, name, len) == 0)
+        return true;
+    return false;
+  }
+  hipEvent_t timerEvent()
+  {
+    if (timerEvents.empty()) {
+      hipEvent_t e;
+      HIP_CHECK(hipEventCreate(&e));
+      return e;
+    }
+    const hipEvent_t e = timerEvents.back();
+    timerEvents.pop_back();
+    return e;
+  }
   // enqueues fn (a launch on `on`, the lane's stream by default), timed on
-  // that stream with events when profiling
+  // that stream with events when profiling (events are reused once read)
   template <typename F>
   void launch(const char *name, F &&fn, hipStream_t on = nullptr)
   {
-    if (!profile || capturing) {
+    if (!profile || capturing || !timed(name)) {
       HIP_CHECK(fn());
       return;
     }
     const hipStream_t st = on ? on : lane->stream;
-    hipEvent_t a, b;
-    HIP_CHECK(hipEventCreate(&a));
-    HIP_CHECK(hipEventCreate(&b));
+    const hipEvent_t a = timerEvent(), b = timerEvent();
     HIP_CHECK(hipEventRecord(a, st));
     HIP_CHECK(fn());
     HIP_CHECK(hipEventRecord(b, st));
@@ -1257,8 +1295,8 @@ class HipDevice : public Device
         timers[pe.first].launches++;
         timers[pe.first].totalMs += ms;
       }
-      (void)hipEventDestroy(pe.second.first);
-      (void)hipEventDestroy(pe.second.second);
+      timerEvents.push_back(pe.second.first);
+      timerEvents.push_back(pe.second.second);
     }
     pendingEvents.clear();
   }
