@@ -303,9 +303,11 @@ def map_color(fb: Handle, w: int, h: int, fmt: int) -> np.ndarray:
 
 
 def kernel_stats() -> dict:
-    """{stage: (launches, total ms)} from ospHipGetKernelStats."""
-    arr = (K.OSPHipKernelStat * 64)()
-    n = K.ext().ospHipGetKernelStats(arr, 64)
+    """{stage: (launches, total ms)} from ospHipGetKernelStats (per-bounce
+    timers: up to 16 stages of each of up to 64 bounces)."""
+    cap = 1024
+    arr = (K.OSPHipKernelStat * cap)()
+    n = K.ext().ospHipGetKernelStats(arr, cap)
     return {arr[i].name.decode(): (arr[i].launches, arr[i].totalMs) for i in range(max(n, 0))}
 
 
