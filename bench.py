@@ -489,7 +489,8 @@ def main() -> int:
     # the timed frames carry HIP event timers around the roofline candidates
     # only (the traversal and delta-tracking kernels): timers around every
     # stage add about 0.9 ms of event records per frame
-    stages = "extend,shadow,shadow_trans" + (",volume,resolve" if args.workload == "c4" else "")
+    # (the transparent-shadow pass, a few rays per bounce, is never dominant)
+    stages = "extend,shadow" + (",volume,resolve" if args.workload == "c4" else "")
     A.init("hip", **{"profileKernels": True, "profileStages": "" if args.all_stage_timers else stages, **dparams})
     name, cus = A.device_info()
     log(f"device {name}, {cus} CUs")
@@ -636,9 +637,11 @@ def main() -> int:
            "kernels_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats.items())},
            **({"stages_ms_per_frame": {k: round(v[1] / args.steps, 3) for k, v in sorted(kstats_raw.items())}}
               if kstats_raw.keys() != kstats.keys() else {}),
-           "traversal": {k: {"nodes_per_ray": round(v["nodes_per_ray"], 2), "prims_per_ray": round(v["prims_per_ray"], 2),
-                             "rays_per_frame": v["rays_per_frame"], "gbs": round(v["gbs"], 1)}
-                         for k, v in kern.items() if "nodes_per_ray" in v},
+           "traversal": {k: {"nodes_per_ray": round(tr[key] / max(tr[key + 2], 1), 2),
+                             "prims_per_ray": round(tr[key + 1] / max(tr[key + 2], 1), 2),
+                             "rays_per_frame": tr[key + 2],
+                             **({"gbs": round(kern[k]["gbs"], 1)} if k in kern else {})}
+                         for k, key in (("extend", 0), ("shadow", 3), ("shadow_trans", 6))},
            "roofline": roofline, "cpu_baseline": cpu}
     if args.workload == "c1" and not args.no_c3:
         out["c3"] = measure_c3(args, 1920, 1080)
